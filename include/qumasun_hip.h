@@ -1,0 +1,139 @@
+/*
+ * qumasun_hip.h -- C ABI of the MI355X-native QUMASUN LOBPCG H.psi hot path.
+ *
+ * Plain C types only.  Every array argument named d_* is a DEVICE pointer
+ * (hipMalloc'd, or a torch CUDA tensor's data pointer); the others are host
+ * pointers.  `stream` is a hipStream_t passed as void* (NULL = default stream).
+ * All entry points return a status code (QM_OK = 0) instead of terminating the
+ * process as the reference's gyCheckError does (gyield/gy_hip.h:95-100);
+ * qm_last_error() gives the message of the last failure on the calling thread.
+ *
+ * Band-block layout (the reference WaveSet / SoAComplex, qumasun_base1.h:451-462):
+ *   grid index i = ix + nx*(iy + ny*iz)   (x fastest, initialize.h:340)
+ *   ncomp = 2 (SoA complex): band n real part at d_psi + 2n*Ng, imaginary at (2n+1)*Ng
+ *   ncomp = 1 (real Gamma-point bands): band n at d_psi + n*Ng
+ * Viewed as a real Ng x (ncomp*nb) column-major matrix for the Gram products,
+ * exactly as the reference hands it to blas_DGEMM_t (lobpcg_z_multi_mpi.h:300-558).
+ */
+#ifndef QUMASUN_HIP_H
+#define QUMASUN_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QM_OK 0
+#define QM_ERR_ARG (-1)      /* invalid argument or size */
+#define QM_ERR_HIP (-2)      /* HIP / rocBLAS / rocSOLVER runtime error */
+#define QM_ERR_STATE (-3)    /* call made before the state it needs was set */
+#define QM_ERR_NUMERIC (-4)  /* numerical breakdown (e.g. overlap not positive definite) */
+#define QM_ERR_NOMEM (-5)    /* device allocation failed */
+
+typedef struct qm_ctx qm_ctx;
+typedef struct qm_lobpcg qm_lobpcg;
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *qm_last_error(void);
+
+/* ---------------------------------------------------------------- context */
+
+/* Grid and operator state on device `device`.  Replaces the QUMASUN_BASE1
+ * members read by mHamiltonianMatrix_ddm_bundle (qumasun_base1.h:320,
+ * qumasun_base1.matrix.h:141-254): grid sizes and spacing
+ * (qumasun_base1.constructor.h:25-28).  One whole periodic grid, no DDM split. */
+int qm_create(qm_ctx **out, int nx, int ny, int nz, double hx, double hy, double hz, int device);
+void qm_destroy(qm_ctx *ctx);
+
+/* Local potential V_tot(r), Ng host doubles.  Replaces ml_Vtot (qumasun_base1.h:482)
+ * consumed by mPotentialMatrixAdd_ddm_bundle (qumasun_base1.matrix.h:329-337). */
+int qm_set_vloc(qm_ctx *ctx, const double *vloc);
+
+/* KB projector set (host arrays), replacing the tables that m_pp_SvF holds for
+ * ProjectionPP_bundle (PseudoPotNonlocal_SvF2_kpoint.h:4239): per atom a,
+ *   nproj[a] projectors (<= 16), voxels vox_off[a]..vox_off[a+1]-1 with UNWRAPPED
+ *   integer grid coordinates vox_xyz[3v..3v+2] (the grid index is their periodic
+ *   wrap; the Bloch phase uses the unwrapped value, ..._kpoint.h:3578-3596),
+ *   table[tab_off[a] + w*V_a + v] (column-major V_a x M_a, ..._kpoint.h:2255-2272),
+ *   energy[proj_off[a] + w] (projector_energy_RY, ..._kpoint.h:1069).
+ * dtable (may be NULL) holds d p/d r_x, d p/d r_y, d p/d r_z in the same layout,
+ * one block of tab_off[natoms] values per direction (FORCE_DIFF_PROJ tables,
+ * ..._kpoint.h:2255-2267); it is required by qm_nonlocal_forces_*.
+ * Overlapping spheres are allowed. */
+int qm_set_projectors(qm_ctx *ctx, int natoms, const int *nproj, const int64_t *vox_off,
+                      const int *vox_xyz, const double *table, const double *dtable,
+                      const int64_t *tab_off, const double *energy, const int *proj_off);
+
+/* k-points in 1/bohr (kvec[3*ik..3*ik+2]); precomputes e^{ik.r} on every projector
+ * voxel (mCreateBlochExp, ..._kpoint.h:3537-3618).  k-point 0 defaults to Gamma. */
+int qm_set_kpoints(qm_ctx *ctx, int nk, const double *kvec);
+
+/* ---------------------------------------------------------------- H.psi */
+
+/* d_hpsi = H(k_ik) d_psi on nb bands, overwriting d_hpsi:
+ *   H = -1/2 (Delta_FD8 + 2 i k.grad_FD8 - k^2) + V_loc + sum_a |p> E dV <p|
+ * This is the LOBPCG operator slot OpeA(Ax, x, num) (lobpcg_z_multi_mpi.h:1732-1742,
+ * called at 1863, 2064, 2084) bound to mHamiltonianMatrix_ddm_bundle
+ * (qumasun_kpoint.lobpcg.h:97-99) with the FD8 kinetic term of
+ * mKineticMatrixAdd_ddm_kpint (qumasun_base1.matrix.h:298-317).
+ * ncomp = 1 requires a Gamma k-point.  d_hpsi must not alias d_psi. */
+int qm_hpsi_apply_f64(qm_ctx *ctx, double *d_hpsi, const double *d_psi, int nb, int ncomp, int ik,
+                      void *stream);
+/* FP32 storage and arithmetic; the projector inner products are reduced in FP64
+ * across thread blocks. */
+int qm_hpsi_apply_f32(qm_ctx *ctx, float *d_hpsi, const float *d_psi, int nb, int ncomp, int ik,
+                      void *stream);
+
+/* ---------------------------------------------------------------- adjoint */
+
+/* d_rho[i] += sum_n occ[n] |psi_n(i)|^2 (FP64 accumulation).  dE_band/dV_loc(i) is
+ * d_rho[i]*dV.  Replaces SetDensityOne (qumasun_SetDensityOne.h:9-41).  occ: host, nb. */
+int qm_band_density_f64(qm_ctx *ctx, double *d_rho, const double *d_psi, const double *occ, int nb,
+                        int ncomp, void *stream);
+int qm_band_density_f32(qm_ctx *ctx, double *d_rho, const float *d_psi, const double *occ, int nb,
+                        int ncomp, void *stream);
+
+/* Nonlocal energy E_nl = sum_n occ_n sum_w E_w |<p_w|psi_n>|^2 and forces
+ * F_a = -dE_nl/dR_a = 2 sum_n occ_n sum_w E_w Re(<p_w|psi_n>^* <d p_w/d r|psi_n>)
+ * (host outputs: forces[3*natoms], e_nl[1]).  Replaces ForceNonlocal_bundle
+ * (PseudoPotNonlocal_SvF2_kpoint.h:3629-3731) with mInnerNonlocalPsi_diff_bundle
+ * (4775-4880).  Synchronises `stream`. */
+int qm_nonlocal_forces_f64(qm_ctx *ctx, const double *d_psi, const double *occ, int nb, int ncomp,
+                           int ik, double *forces, double *e_nl, void *stream);
+int qm_nonlocal_forces_f32(qm_ctx *ctx, const float *d_psi, const double *occ, int nb, int ncomp,
+                           int ik, double *forces, double *e_nl, void *stream);
+
+/* ---------------------------------------------------------------- LOBPCG */
+
+/* Block LOBPCG solver of nbands lowest states of H(k).  precision = 64: vectors,
+ * H.psi and Gram products in FP64.  precision = 32: vectors, H.psi, Gram and
+ * update products in FP32 (Gram partial sums reduced in FP64); the Cholesky,
+ * generalized subspace eigensolve and residual/norm sums stay FP64.
+ * Replaces LOBPCG::Eigen_z_multi_mpi (lobpcg_z_multi_mpi.h:1732-2852) with its
+ * work arena (1779-1786, 2016-2022).  The solver keeps the P block between calls
+ * (the reference's `keep`, qumasun_kpoint.h:83-100). */
+int qm_lobpcg_create(qm_lobpcg **out, qm_ctx *ctx, int nbands, int ncomp, int precision);
+void qm_lobpcg_destroy(qm_lobpcg *s);
+
+/* Runs iter_max iterations on d_x (device, nbands bands in the layout above,
+ * FP64 for precision 64, FP32 for 32; normalised so that sum |x|^2 dV = 1 on
+ * return, as lobpcg_z_multi_mpi.h:2796-2800).  first_time != 0 starts with P = 0
+ * (SCF_step == 0, lobpcg_z_multi_mpi.h:2052-2056); otherwise the kept P is reused
+ * and A.P recomputed (2058-2065).  Host outputs: eigenvalues[nbands] (Ha),
+ * residual_norms[nbands] (|| H x - e x ||^2 of the last residual, may be NULL).
+ * Returns the number of iterations done through *iters_done (may be NULL); the
+ * reference's early exit on loss of normalisation (|1 - |x|^2| > 1e-3,
+ * 2611-2666) is kept. */
+int qm_lobpcg_solve(qm_lobpcg *s, void *d_x, double *eigenvalues, double *residual_norms, int iter_max,
+                    int ik, int first_time, int *iters_done, void *stream);
+
+/* Milliseconds spent in the last qm_lobpcg_solve per phase:
+ * t[0] H.psi, t[1] Gram, t[2] subspace eigensolve, t[3] updates, t[4] other, t[5] total. */
+int qm_lobpcg_timings(qm_lobpcg *s, double *t6);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QUMASUN_HIP_H */

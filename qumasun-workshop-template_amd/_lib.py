@@ -1,0 +1,75 @@
+"""ctypes binding of libqumasun_hip.so (the C ABI in include/qumasun_hip.h).
+
+The library is built in-tree by build.py.  There is no fallback: if the
+library is missing, loading raises, so nothing silently runs on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libqumasun_hip.so")
+
+# every entry point declared in include/qumasun_hip.h
+EXPORTS = (
+    "qm_last_error", "qm_create", "qm_destroy", "qm_set_vloc", "qm_set_projectors", "qm_set_kpoints",
+    "qm_hpsi_apply_f64", "qm_hpsi_apply_f32", "qm_band_density_f64", "qm_band_density_f32",
+    "qm_nonlocal_forces_f64", "qm_nonlocal_forces_f32", "qm_lobpcg_create", "qm_lobpcg_destroy",
+    "qm_lobpcg_solve", "qm_lobpcg_timings",
+)
+
+QM_OK = 0
+_STATUS = {-1: "QM_ERR_ARG", -2: "QM_ERR_HIP", -3: "QM_ERR_STATE", -4: "QM_ERR_NUMERIC", -5: "QM_ERR_NOMEM"}
+
+
+class QMError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{_STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+_lib = None
+
+
+def load():
+    """Loads the HIP library (raises OSError if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise OSError(f"{LIB_PATH} is missing: run `python qumasun-workshop-template_amd/build.py` "
+                      "(or __graft_entry__.build()) first")
+    L = ctypes.CDLL(LIB_PATH)
+    I, D, P, L64 = ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_int64
+    L.qm_last_error.restype = ctypes.c_char_p
+    L.qm_last_error.argtypes = []
+    L.qm_create.argtypes = [ctypes.POINTER(P), I, I, I, D, D, D, I]
+    L.qm_destroy.argtypes = [P]
+    L.qm_destroy.restype = None
+    L.qm_set_vloc.argtypes = [P, P]
+    L.qm_set_projectors.argtypes = [P, I, P, P, P, P, P, P, P, P]
+    L.qm_set_kpoints.argtypes = [P, I, P]
+    for n in ("qm_hpsi_apply_f64", "qm_hpsi_apply_f32"):
+        getattr(L, n).argtypes = [P, P, P, I, I, I, P]
+    for n in ("qm_band_density_f64", "qm_band_density_f32"):
+        getattr(L, n).argtypes = [P, P, P, P, I, I, P]
+    for n in ("qm_nonlocal_forces_f64", "qm_nonlocal_forces_f32"):
+        getattr(L, n).argtypes = [P, P, P, I, I, I, P, P, P]
+    L.qm_lobpcg_create.argtypes = [ctypes.POINTER(P), P, I, I, I]
+    L.qm_lobpcg_destroy.argtypes = [P]
+    L.qm_lobpcg_destroy.restype = None
+    L.qm_lobpcg_solve.argtypes = [P, P, P, P, I, I, I, ctypes.POINTER(I), P]
+    L.qm_lobpcg_timings.argtypes = [P, P]
+    for n in EXPORTS:
+        if n not in ("qm_last_error", "qm_destroy", "qm_lobpcg_destroy"):
+            getattr(L, n).restype = I
+    _lib = L
+    return L
+
+
+def check(status):
+    if status != QM_OK:
+        msg = load().qm_last_error()
+        raise QMError(status, msg.decode() if msg else "")
+    return status

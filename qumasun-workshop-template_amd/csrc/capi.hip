@@ -1,0 +1,370 @@
+// C ABI entry points: context, potential, projector set, k-points, H.psi and
+// the adjoint.  See include/qumasun_hip.h for the contract of each function.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "qm_internal.h"
+
+namespace qm {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+int DevBuf::reserve(size_t n) {
+    if (n <= bytes) return QM_OK;
+    if (p) {
+        QM_HIP(hipFree(p));
+        p = nullptr;
+        bytes = 0;
+    }
+    hipError_t e = hipMalloc(&p, n);
+    if (e != hipSuccess) {
+        p = nullptr;
+        set_error("hipMalloc of " + std::to_string(n) + " bytes failed: " + hipGetErrorString(e));
+        return QM_ERR_NOMEM;
+    }
+    bytes = n;
+    return QM_OK;
+}
+
+void DevBuf::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
+int activate(const qm_ctx *ctx) {
+    QM_HIP(hipSetDevice(ctx->device));
+    return QM_OK;
+}
+
+template <typename T>
+static int upload(T **dst, const T *src, size_t n) {
+    if (*dst) {
+        QM_HIP(hipFree(*dst));
+        *dst = nullptr;
+    }
+    if (n == 0) return QM_OK;
+    hipError_t e = hipMalloc((void **)dst, sizeof(T) * n);
+    if (e != hipSuccess) {
+        *dst = nullptr;
+        set_error(std::string("hipMalloc failed: ") + hipGetErrorString(e));
+        return QM_ERR_NOMEM;
+    }
+    QM_HIP(hipMemcpy(*dst, src, sizeof(T) * n, hipMemcpyHostToDevice));
+    return QM_OK;
+}
+
+template <typename T>
+static void dfree(T *&p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+static void free_projectors(Projectors &p) {
+    dfree(p.d_nproj);
+    dfree(p.d_proj_off);
+    dfree(p.d_vox_off);
+    dfree(p.d_tab_off);
+    dfree(p.d_vox_grid);
+    dfree(p.d_vox_atom);
+    dfree(p.d_energy);
+    dfree(p.d_table64);
+    dfree(p.d_dtable64);
+    dfree(p.d_table32);
+    dfree(p.d_dtable32);
+    dfree(p.d_cov_grid);
+    dfree(p.d_cov_off);
+    dfree(p.d_cov_vox);
+    dfree(p.d_atom_order);
+    p = Projectors();
+}
+
+static void free_kpoints(qm_ctx *ctx) {
+    for (auto &k : ctx->kpts) {
+        dfree(k.d_phase64);
+        dfree(k.d_phase32);
+    }
+    ctx->kpts.clear();
+}
+
+static inline int wrap_host(int i, int n) {
+    int r = i % n;
+    return r < 0 ? r + n : r;
+}
+
+// e^{ik.r} on every voxel for one k-point (mCreateBlochExp, ..._kpoint.h:3537-3618):
+// phase = k_x h_x ix + k_y h_y iy + k_z h_z iz on the unwrapped coordinates.
+static int build_phase(qm_ctx *ctx, KPoint &kp) {
+    dfree(kp.d_phase64);
+    dfree(kp.d_phase32);
+    const Projectors &p = ctx->proj;
+    if (kp.gamma || p.vtot == 0) return QM_OK;
+    std::vector<double> ph(2 * p.vtot);
+    std::vector<float> ph32(2 * p.vtot);
+    const double gx = kp.k[0] * ctx->hx, gy = kp.k[1] * ctx->hy, gz = kp.k[2] * ctx->hz;
+    for (int64_t v = 0; v < p.vtot; ++v) {
+        const double a = gx * (double)p.vox_xyz[3 * v] + gy * (double)p.vox_xyz[3 * v + 1] +
+                         gz * (double)p.vox_xyz[3 * v + 2];
+        ph[2 * v] = std::cos(a);
+        ph[2 * v + 1] = std::sin(a);
+        ph32[2 * v] = (float)ph[2 * v];
+        ph32[2 * v + 1] = (float)ph[2 * v + 1];
+    }
+    QM_TRY(upload(&kp.d_phase64, ph.data(), ph.size()));
+    QM_TRY(upload(&kp.d_phase32, ph32.data(), ph32.size()));
+    return QM_OK;
+}
+
+template <typename T>
+int hpsi_apply(qm_ctx *ctx, T *hpsi, const T *psi, int nb, int ncomp, int ik, void *stream) {
+    QM_CHECK(ctx && hpsi && psi, QM_ERR_ARG, "qm_hpsi_apply: NULL argument");
+    QM_CHECK(nb >= 0 && (ncomp == 1 || ncomp == 2), QM_ERR_ARG, "qm_hpsi_apply: bad nb / ncomp");
+    QM_CHECK(ik >= 0 && ik < (int)ctx->kpts.size(), QM_ERR_ARG, "qm_hpsi_apply: k index out of range");
+    QM_CHECK(ctx->has_vloc, QM_ERR_STATE, "qm_hpsi_apply: call qm_set_vloc first");
+    QM_CHECK((const void *)hpsi != (const void *)psi, QM_ERR_ARG, "qm_hpsi_apply: output aliases input");
+    if (nb == 0) return QM_OK;
+    QM_TRY(activate(ctx));
+    hipStream_t st = (hipStream_t)stream;
+    const KPoint &kp = ctx->kpts[ik];
+    if constexpr (sizeof(T) == 8) {
+        QM_TRY(launch_fd8_local_f64(ctx, hpsi, psi, nb, ncomp, kp.k, st));
+        QM_TRY(launch_nonlocal_f64(ctx, hpsi, psi, nb, ncomp, kp, st));
+    } else {
+        QM_TRY(launch_fd8_local_f32(ctx, hpsi, psi, nb, ncomp, kp.k, st));
+        QM_TRY(launch_nonlocal_f32(ctx, hpsi, psi, nb, ncomp, kp, st));
+    }
+    return QM_OK;
+}
+
+template <typename T>
+int band_density(qm_ctx *ctx, double *rho, const T *psi, const double *occ, int nb, int ncomp,
+                        void *stream) {
+    QM_CHECK(ctx && rho && psi && occ, QM_ERR_ARG, "qm_band_density: NULL argument");
+    QM_CHECK(nb >= 0 && (ncomp == 1 || ncomp == 2), QM_ERR_ARG, "qm_band_density: bad nb / ncomp");
+    if (nb == 0) return QM_OK;
+    QM_TRY(activate(ctx));
+    hipStream_t st = (hipStream_t)stream;
+    QM_TRY(ctx->fwork.reserve(sizeof(double) * nb));
+    double *d_occ = ctx->fwork.as<double>();
+    QM_HIP(hipMemcpyAsync(d_occ, occ, sizeof(double) * nb, hipMemcpyHostToDevice, st));
+    if constexpr (sizeof(T) == 8) QM_TRY(launch_band_density_f64(ctx, rho, psi, d_occ, nb, ncomp, st));
+    else QM_TRY(launch_band_density_f32(ctx, rho, psi, d_occ, nb, ncomp, st));
+    // the occupancy staging buffer is reused by the next call: finish first
+    QM_HIP(hipStreamSynchronize(st));
+    return QM_OK;
+}
+
+template <typename T>
+int forces(qm_ctx *ctx, const T *psi, const double *occ, int nb, int ncomp, int ik, double *f, double *e,
+                  void *stream) {
+    QM_CHECK(ctx && psi && occ && f && e, QM_ERR_ARG, "qm_nonlocal_forces: NULL argument");
+    QM_CHECK(nb >= 1 && nb <= 65535, QM_ERR_ARG, "qm_nonlocal_forces: nb must be in [1, 65535]");
+    QM_CHECK(ik >= 0 && ik < (int)ctx->kpts.size(), QM_ERR_ARG, "qm_nonlocal_forces: k index out of range");
+    QM_TRY(activate(ctx));
+    if (ctx->proj.natoms == 0) {
+        *e = 0.0;
+        return QM_OK;
+    }
+    if constexpr (sizeof(T) == 8)
+        return nonlocal_forces_f64(ctx, psi, occ, nb, ncomp, ctx->kpts[ik], f, e, (hipStream_t)stream);
+    else
+        return nonlocal_forces_f32(ctx, psi, occ, nb, ncomp, ctx->kpts[ik], f, e, (hipStream_t)stream);
+}
+
+}  // namespace qm
+
+using namespace qm;
+
+extern "C" {
+
+const char *qm_last_error(void) { return g_last_error.c_str(); }
+
+int qm_create(qm_ctx **out, int nx, int ny, int nz, double hx, double hy, double hz, int device) {
+    QM_CHECK(out != nullptr, QM_ERR_ARG, "qm_create: out is NULL");
+    *out = nullptr;
+    QM_CHECK(nx >= 4 && ny >= 4 && nz >= 4, QM_ERR_ARG, "qm_create: grid dimensions must be >= 4");
+    QM_CHECK((int64_t)nx * ny * nz < ((int64_t)1 << 31), QM_ERR_ARG, "qm_create: grid must have < 2^31 points");
+    QM_CHECK(hx > 0 && hy > 0 && hz > 0, QM_ERR_ARG, "qm_create: spacings must be positive");
+    int ndev = 0;
+    QM_HIP(hipGetDeviceCount(&ndev));
+    QM_CHECK(device >= 0 && device < ndev, QM_ERR_ARG, "qm_create: no such device");
+    qm_ctx *c = new qm_ctx();
+    c->device = device;
+    c->nx = nx;
+    c->ny = ny;
+    c->nz = nz;
+    c->ng = (int64_t)nx * ny * nz;
+    c->hx = hx;
+    c->hy = hy;
+    c->hz = hz;
+    c->kpts.resize(1);
+    *out = c;
+    return QM_OK;
+}
+
+void qm_destroy(qm_ctx *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipDeviceSynchronize();
+    dfree(ctx->d_vloc64);
+    dfree(ctx->d_vloc32);
+    free_projectors(ctx->proj);
+    free_kpoints(ctx);
+    ctx->coef.release();
+    ctx->fwork.release();
+    delete ctx;
+}
+
+int qm_set_vloc(qm_ctx *ctx, const double *vloc) {
+    QM_CHECK(ctx && vloc, QM_ERR_ARG, "qm_set_vloc: NULL argument");
+    QM_TRY(activate(ctx));
+    std::vector<float> v32(vloc, vloc + ctx->ng);
+    QM_TRY(upload(&ctx->d_vloc64, vloc, ctx->ng));
+    QM_TRY(upload(&ctx->d_vloc32, v32.data(), ctx->ng));
+    ctx->has_vloc = true;
+    return QM_OK;
+}
+
+int qm_set_projectors(qm_ctx *ctx, int natoms, const int *nproj, const int64_t *vox_off, const int *vox_xyz,
+                      const double *table, const double *dtable, const int64_t *tab_off, const double *energy,
+                      const int *proj_off) {
+    QM_CHECK(ctx, QM_ERR_ARG, "qm_set_projectors: NULL context");
+    QM_CHECK(natoms >= 0, QM_ERR_ARG, "qm_set_projectors: natoms < 0");
+    QM_TRY(activate(ctx));
+    QM_HIP(hipDeviceSynchronize());
+    free_projectors(ctx->proj);
+    Projectors &p = ctx->proj;
+    if (natoms == 0) {
+        for (auto &k : ctx->kpts) QM_TRY(build_phase(ctx, k));
+        return QM_OK;
+    }
+    QM_CHECK(nproj && vox_off && vox_xyz && table && tab_off && energy && proj_off, QM_ERR_ARG,
+             "qm_set_projectors: NULL array");
+    QM_CHECK(vox_off[0] == 0 && tab_off[0] == 0 && proj_off[0] == 0, QM_ERR_ARG,
+             "qm_set_projectors: offsets must start at 0");
+    p.natoms = natoms;
+    p.nproj.assign(nproj, nproj + natoms);
+    p.vox_off.assign(vox_off, vox_off + natoms + 1);
+    p.tab_off.assign(tab_off, tab_off + natoms + 1);
+    p.proj_off.assign(proj_off, proj_off + natoms + 1);
+    for (int a = 0; a < natoms; ++a) {
+        const int64_t V = vox_off[a + 1] - vox_off[a];
+        QM_CHECK(nproj[a] >= 0 && nproj[a] <= kMaxProj, QM_ERR_ARG, "qm_set_projectors: nproj must be in [0, 16]");
+        QM_CHECK(V >= 0 && V < ((int64_t)1 << 31), QM_ERR_ARG, "qm_set_projectors: bad voxel range");
+        QM_CHECK(tab_off[a + 1] - tab_off[a] == V * nproj[a], QM_ERR_ARG,
+                 "qm_set_projectors: tab_off inconsistent with nproj * voxels");
+        QM_CHECK(proj_off[a + 1] - proj_off[a] == nproj[a], QM_ERR_ARG,
+                 "qm_set_projectors: proj_off inconsistent with nproj");
+        p.max_m = std::max(p.max_m, nproj[a]);
+    }
+    p.vtot = vox_off[natoms];
+    p.ttot = tab_off[natoms];
+    p.ptot = proj_off[natoms];
+    QM_CHECK(p.vtot < ((int64_t)1 << 31), QM_ERR_ARG, "qm_set_projectors: too many voxels");
+    p.vox_xyz.assign(vox_xyz, vox_xyz + 3 * p.vtot);
+    p.energy.assign(energy, energy + p.ptot);
+    p.has_grad = dtable != nullptr;
+
+    // wrapped grid index and owner atom of every voxel
+    std::vector<int> grid(p.vtot), atom(p.vtot);
+    for (int a = 0; a < natoms; ++a)
+        for (int64_t v = vox_off[a]; v < vox_off[a + 1]; ++v) {
+            grid[v] = wrap_host(vox_xyz[3 * v], ctx->nx) +
+                      ctx->nx * (wrap_host(vox_xyz[3 * v + 1], ctx->ny) +
+                                 ctx->ny * wrap_host(vox_xyz[3 * v + 2], ctx->nz));
+            atom[v] = a;
+        }
+    // covered-point map for the owner-computes scatter: voxels sorted by grid point
+    std::vector<int> order(p.vtot);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return grid[x] < grid[y]; });
+    std::vector<int> cov_grid, cov_off;
+    cov_off.reserve(p.vtot + 1);
+    for (int64_t i = 0; i < p.vtot; ++i) {
+        if (i == 0 || grid[order[i]] != grid[order[i - 1]]) {
+            cov_grid.push_back(grid[order[i]]);
+            cov_off.push_back((int)i);
+        }
+    }
+    cov_off.push_back((int)p.vtot);
+    p.ncov = (int)cov_grid.size();
+    // heaviest atoms first in the projection launch
+    std::vector<int> aord(natoms);
+    std::iota(aord.begin(), aord.end(), 0);
+    std::stable_sort(aord.begin(), aord.end(), [&](int x, int y) {
+        return (vox_off[x + 1] - vox_off[x]) * nproj[x] > (vox_off[y + 1] - vox_off[y]) * nproj[y];
+    });
+
+    std::vector<float> t32(table, table + p.ttot);
+    QM_TRY(upload(&p.d_nproj, nproj, natoms));
+    QM_TRY(upload(&p.d_proj_off, proj_off, natoms + 1));
+    QM_TRY(upload(&p.d_vox_off, vox_off, natoms + 1));
+    QM_TRY(upload(&p.d_tab_off, tab_off, natoms + 1));
+    QM_TRY(upload(&p.d_vox_grid, grid.data(), p.vtot));
+    QM_TRY(upload(&p.d_vox_atom, atom.data(), p.vtot));
+    QM_TRY(upload(&p.d_energy, energy, p.ptot));
+    QM_TRY(upload(&p.d_table64, table, p.ttot));
+    QM_TRY(upload(&p.d_table32, t32.data(), p.ttot));
+    if (dtable) {
+        std::vector<float> d32(dtable, dtable + 3 * p.ttot);
+        QM_TRY(upload(&p.d_dtable64, dtable, 3 * p.ttot));
+        QM_TRY(upload(&p.d_dtable32, d32.data(), 3 * p.ttot));
+    }
+    QM_TRY(upload(&p.d_cov_grid, cov_grid.data(), cov_grid.size()));
+    QM_TRY(upload(&p.d_cov_off, cov_off.data(), cov_off.size()));
+    QM_TRY(upload(&p.d_cov_vox, order.data(), order.size()));
+    QM_TRY(upload(&p.d_atom_order, aord.data(), aord.size()));
+    for (auto &k : ctx->kpts) QM_TRY(build_phase(ctx, k));
+    return QM_OK;
+}
+
+int qm_set_kpoints(qm_ctx *ctx, int nk, const double *kvec) {
+    QM_CHECK(ctx && kvec && nk >= 1, QM_ERR_ARG, "qm_set_kpoints: bad argument");
+    QM_TRY(activate(ctx));
+    QM_HIP(hipDeviceSynchronize());
+    free_kpoints(ctx);
+    ctx->kpts.resize(nk);
+    for (int i = 0; i < nk; ++i) {
+        KPoint &k = ctx->kpts[i];
+        for (int d = 0; d < 3; ++d) k.k[d] = kvec[3 * i + d];
+        // the reference treats a k-point as Gamma only when flagged so
+        // (m_is_gamma_point_list); here: exactly zero k
+        k.gamma = (k.k[0] == 0.0 && k.k[1] == 0.0 && k.k[2] == 0.0);
+        QM_TRY(build_phase(ctx, k));
+    }
+    return QM_OK;
+}
+
+int qm_hpsi_apply_f64(qm_ctx *ctx, double *d_hpsi, const double *d_psi, int nb, int ncomp, int ik, void *stream) {
+    return hpsi_apply<double>(ctx, d_hpsi, d_psi, nb, ncomp, ik, stream);
+}
+
+int qm_hpsi_apply_f32(qm_ctx *ctx, float *d_hpsi, const float *d_psi, int nb, int ncomp, int ik, void *stream) {
+    return hpsi_apply<float>(ctx, d_hpsi, d_psi, nb, ncomp, ik, stream);
+}
+
+int qm_band_density_f64(qm_ctx *ctx, double *d_rho, const double *d_psi, const double *occ, int nb, int ncomp,
+                        void *stream) {
+    return band_density<double>(ctx, d_rho, d_psi, occ, nb, ncomp, stream);
+}
+
+int qm_band_density_f32(qm_ctx *ctx, double *d_rho, const float *d_psi, const double *occ, int nb, int ncomp,
+                        void *stream) {
+    return band_density<float>(ctx, d_rho, d_psi, occ, nb, ncomp, stream);
+}
+
+int qm_nonlocal_forces_f64(qm_ctx *ctx, const double *d_psi, const double *occ, int nb, int ncomp, int ik,
+                           double *forces_out, double *e_nl, void *stream) {
+    return forces<double>(ctx, d_psi, occ, nb, ncomp, ik, forces_out, e_nl, stream);
+}
+
+int qm_nonlocal_forces_f32(qm_ctx *ctx, const float *d_psi, const double *occ, int nb, int ncomp, int ik,
+                           double *forces_out, double *e_nl, void *stream) {
+    return forces<float>(ctx, d_psi, occ, nb, ncomp, ik, forces_out, e_nl, stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,134 @@
+// Internal declarations shared by the HIP translation units of libqumasun_hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/qumasun_hip.h"
+
+namespace qm {
+
+void set_error(const std::string &msg);
+
+// Records a HIP failure with the call site and returns QM_ERR_HIP from the
+// enclosing function.
+#define QM_HIP(call)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess) {                                                              \
+            ::qm::set_error(std::string(#call) + ": " + hipGetErrorString(e_) + " at " +     \
+                            __FILE__ + ":" + std::to_string(__LINE__));                      \
+            return QM_ERR_HIP;                                                               \
+        }                                                                                    \
+    } while (0)
+
+#define QM_CHECK(cond, code, msg)                                                            \
+    do {                                                                                     \
+        if (!(cond)) {                                                                       \
+            ::qm::set_error(msg);                                                            \
+            return (code);                                                                   \
+        }                                                                                    \
+    } while (0)
+
+#define QM_TRY(expr)                      \
+    do {                                  \
+        int s_ = (expr);                  \
+        if (s_ != QM_OK) return s_;       \
+    } while (0)
+
+constexpr int kMaxProj = 16;
+
+// Device buffer with grow-only reallocation.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    int reserve(size_t n);
+    void release();
+    template <typename T>
+    T *as() const { return static_cast<T *>(p); }
+};
+
+// Projector set on device.  Voxels are stored atom-major; each atom's voxels
+// are ordered z, y, x so consecutive voxels are mostly consecutive grid points.
+struct Projectors {
+    int natoms = 0;
+    int ptot = 0;          // sum_a M_a
+    int64_t vtot = 0;      // sum_a V_a
+    int64_t ttot = 0;      // sum_a V_a M_a
+    int max_m = 0;
+    bool has_grad = false;
+    std::vector<int> nproj, proj_off;
+    std::vector<int64_t> vox_off, tab_off;
+    std::vector<int> vox_xyz;            // host copy of unwrapped coordinates (phases)
+    std::vector<double> energy;
+    // device arrays
+    int *d_nproj = nullptr, *d_proj_off = nullptr;
+    int64_t *d_vox_off = nullptr, *d_tab_off = nullptr;
+    int *d_vox_grid = nullptr;           // wrapped grid index per voxel
+    int *d_vox_atom = nullptr;           // owning atom per voxel
+    double *d_energy = nullptr;
+    double *d_table64 = nullptr, *d_dtable64 = nullptr;
+    float *d_table32 = nullptr, *d_dtable32 = nullptr;
+    // owner-computes scatter map: covered grid points, each with its list of voxels
+    int ncov = 0;
+    int *d_cov_grid = nullptr;           // [ncov] grid index
+    int *d_cov_off = nullptr;            // [ncov+1]
+    int *d_cov_vox = nullptr;            // [vtot] global voxel ids grouped by covered point
+    // atom work list for the projection launch, longest atoms first
+    int *d_atom_order = nullptr;
+};
+
+struct KPoint {
+    double k[3] = {0, 0, 0};
+    bool gamma = true;
+    double *d_phase64 = nullptr;  // [2*vtot]: cos, sin interleaved
+    float *d_phase32 = nullptr;
+};
+
+}  // namespace qm
+
+struct qm_ctx {
+    int device = 0;
+    int nx = 0, ny = 0, nz = 0;
+    int64_t ng = 0;
+    double hx = 0, hy = 0, hz = 0;
+    bool has_vloc = false;
+    double *d_vloc64 = nullptr;
+    float *d_vloc32 = nullptr;
+    qm::Projectors proj;
+    std::vector<qm::KPoint> kpts;
+    qm::DevBuf coef;     // projector coefficients for the nonlocal apply
+    qm::DevBuf fwork;    // force partials
+};
+
+namespace qm {
+
+int activate(const qm_ctx *ctx);
+
+// stencil.hip
+int launch_fd8_local_f64(const qm_ctx *ctx, double *out, const double *in, int nb, int ncomp,
+                         const double k[3], hipStream_t st);
+int launch_fd8_local_f32(const qm_ctx *ctx, float *out, const float *in, int nb, int ncomp,
+                         const double k[3], hipStream_t st);
+
+// nonlocal.hip
+int launch_nonlocal_f64(qm_ctx *ctx, double *hpsi, const double *psi, int nb, int ncomp,
+                        const KPoint &kp, hipStream_t st);
+int launch_nonlocal_f32(qm_ctx *ctx, float *hpsi, const float *psi, int nb, int ncomp,
+                        const KPoint &kp, hipStream_t st);
+int nonlocal_forces_f64(qm_ctx *ctx, const double *psi, const double *occ, int nb, int ncomp,
+                        const KPoint &kp, double *forces, double *e_nl, hipStream_t st);
+int nonlocal_forces_f32(qm_ctx *ctx, const float *psi, const double *occ, int nb, int ncomp,
+                        const KPoint &kp, double *forces, double *e_nl, hipStream_t st);
+
+// adjoint.hip
+int launch_band_density_f64(const qm_ctx *ctx, double *rho, const double *psi, const double *d_occ,
+                            int nb, int ncomp, hipStream_t st);
+int launch_band_density_f32(const qm_ctx *ctx, double *rho, const float *psi, const double *d_occ,
+                            int nb, int ncomp, hipStream_t st);
+
+}  // namespace qm

@@ -1,0 +1,116 @@
+"""Host-side mirror of the reference's H.psi operator on the GPU.
+
+`Hamiltonian` plays the role of QUMASUN_BASE1::mHamiltonianMatrix_ddm_bundle
+(qumasun_base1.h:320, qumasun_base1.matrix.h:141-254): it owns the grid, V_tot
+and the projector tables, and `apply(hpsi, psi, nb, k_index)` is the operator
+slot OpeA(Ax, x, num) that LOBPCG::Eigen_z_multi_mpi calls
+(lobpcg_z_multi_mpi.h:1732-1742; the caller lambda is qumasun_kpoint.lobpcg.h:97-99).
+Arrays are torch tensors on the GPU in the reference's SoA band layout, shape
+(nb * ncomp, Ng); the compute is the HIP library behind the C ABI.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _stream(torch, device):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+class Hamiltonian:
+    def __init__(self, system, device=0):
+        self.lib = _lib.load()
+        self.sys = system
+        self.device = device
+        ctx = ctypes.c_void_p()
+        hx, hy, hz = system.h
+        _lib.check(self.lib.qm_create(ctypes.byref(ctx), system.nx, system.ny, system.nz, hx, hy, hz, device))
+        self.ctx = ctx
+        self.set_vloc(system.vloc)
+        self.set_projectors(system)
+        self.set_kpoints(system.kpoints)
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.lib.qm_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- state ---------------------------------------------------------------
+    def set_vloc(self, vloc):
+        v = np.ascontiguousarray(vloc, dtype=np.float64)
+        assert v.size == self.sys.ng
+        _lib.check(self.lib.qm_set_vloc(self.ctx, _ptr(v)))
+
+    def set_projectors(self, s):
+        arrs = [np.ascontiguousarray(s.nproj, np.int32), np.ascontiguousarray(s.vox_off, np.int64),
+                np.ascontiguousarray(s.vox_xyz, np.int32), np.ascontiguousarray(s.table, np.float64),
+                np.ascontiguousarray(s.dtable, np.float64) if s.dtable is not None else None,
+                np.ascontiguousarray(s.tab_off, np.int64), np.ascontiguousarray(s.energy, np.float64),
+                np.ascontiguousarray(s.proj_off, np.int32)]
+        ptrs = [(_ptr(a) if a is not None else None) for a in arrs]
+        _lib.check(self.lib.qm_set_projectors(self.ctx, s.natoms, *ptrs))
+
+    def set_kpoints(self, kpoints):
+        k = np.ascontiguousarray(kpoints, dtype=np.float64).reshape(-1, 3)
+        _lib.check(self.lib.qm_set_kpoints(self.ctx, k.shape[0], _ptr(k)))
+
+    # -- operator ------------------------------------------------------------
+    def apply(self, hpsi, psi, nb=None, k_index=0, ncomp=None):
+        """hpsi <- H(k) psi on the current torch stream (hpsi overwritten)."""
+        import torch
+        ncomp = self.sys.ncomp if ncomp is None else ncomp
+        nb = psi.shape[0] // ncomp if nb is None else nb
+        assert psi.is_cuda and hpsi.is_cuda and psi.is_contiguous() and hpsi.is_contiguous()
+        assert psi.dtype == hpsi.dtype and psi.numel() >= nb * ncomp * self.sys.ng
+        fn = {torch.float64: self.lib.qm_hpsi_apply_f64, torch.float32: self.lib.qm_hpsi_apply_f32}[psi.dtype]
+        _lib.check(fn(self.ctx, ctypes.c_void_p(hpsi.data_ptr()), ctypes.c_void_p(psi.data_ptr()), nb, ncomp,
+                      k_index, _stream(torch, psi.device)))
+        return hpsi
+
+    def __call__(self, psi, k_index=0, ncomp=None):
+        import torch
+        out = torch.empty_like(psi)
+        return self.apply(out, psi, k_index=k_index, ncomp=ncomp)
+
+    # -- adjoint -------------------------------------------------------------
+    def band_density(self, psi, occ, ncomp=None, rho=None):
+        """rho += sum_n occ_n |psi_n|^2 (FP64, device); dE_band/dV_loc = rho * dV."""
+        import torch
+        ncomp = self.sys.ncomp if ncomp is None else ncomp
+        nb = psi.shape[0] // ncomp
+        if rho is None:
+            rho = torch.zeros(self.sys.ng, dtype=torch.float64, device=psi.device)
+        o = np.ascontiguousarray(occ, dtype=np.float64)
+        assert o.size == nb
+        fn = {torch.float64: self.lib.qm_band_density_f64, torch.float32: self.lib.qm_band_density_f32}[psi.dtype]
+        _lib.check(fn(self.ctx, ctypes.c_void_p(rho.data_ptr()), ctypes.c_void_p(psi.data_ptr()), _ptr(o), nb,
+                      ncomp, _stream(torch, psi.device)))
+        return rho
+
+    def nonlocal_forces(self, psi, occ, k_index=0, ncomp=None):
+        """(E_nl, forces (natoms, 3)) with forces = -dE_nl/dR."""
+        import torch
+        ncomp = self.sys.ncomp if ncomp is None else ncomp
+        nb = psi.shape[0] // ncomp
+        o = np.ascontiguousarray(occ, dtype=np.float64)
+        f = np.zeros((self.sys.natoms, 3))
+        e = np.zeros(1)
+        fn = {torch.float64: self.lib.qm_nonlocal_forces_f64,
+              torch.float32: self.lib.qm_nonlocal_forces_f32}[psi.dtype]
+        _lib.check(fn(self.ctx, ctypes.c_void_p(psi.data_ptr()), _ptr(o), nb, ncomp, k_index, _ptr(f), _ptr(e),
+                      _stream(torch, psi.device)))
+        return float(e[0]), f

@@ -1,0 +1,137 @@
+"""GPU parity of H.psi, the band density and the nonlocal forces (HIP library
+through the C ABI) against the CPU FP64 oracle.
+
+Tolerances: FP64 path <= 1e-12 relative (max-norm of the difference over the
+max-norm of the oracle result); FP32 path <= 2e-6 relative, the FP32 rounding
+of a 25-point stencil whose diagonal dominates (c0 h^2 ~ 4.3).
+"""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+F64_TOL = 1e-12
+F32_TOL = 2e-6
+
+
+def _rel(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+def _gpu_apply(torch, H, psi, dtype, k_index=0, ncomp=None):
+    x = torch.from_numpy(psi).to(device="cuda", dtype=dtype)
+    y = torch.full_like(x, float("nan"))  # every output element must be written
+    H.apply(y, x, k_index=k_index, ncomp=ncomp)
+    torch.cuda.synchronize()
+    return y.double().cpu().numpy()
+
+
+CASES = [
+    # (label, builder)
+    ("C0 32^3 real", lambda qm: qm.system.config(0, nbands=6)),
+    ("C2 48^3 complex 4k", lambda qm: qm.system.config(2, nbands=3, grid=48)),
+    ("C3 80^3 uneven overlapping", lambda qm: qm.system.config(3, nbands=5, grid=80)),
+    ("C4 40^3 real", lambda qm: qm.system.config(4, nbands=4, grid=40)),
+    ("small 12^3 wrapped spheres k", lambda qm: qm.system.small_system(n=12, L=4.0, natoms=3, rc=2.6, nbands=3,
+                                                                       ncomp=2, k_red=(0.5, 0.25, -0.125))),
+    ("ragged 36x20x28", lambda qm: _ragged(qm)),
+]
+
+
+def _ragged(qm):
+    s = qm.system.System("ragged", 36, 20, 28, (6.0, 3.5, 5.0), np.array([[1.0, 1.0, 1.0], [4.0, 2.5, 3.0]]),
+                         np.array([2, 1], np.int32), np.array([1.6, 1.2]), 3, 2,
+                         np.array([[0.3, -0.4, 0.2]]), 21)
+    return s.build()
+
+
+@pytest.mark.parametrize("label,build", CASES, ids=[c[0] for c in CASES])
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_hpsi_parity(qm, orc, torch_cuda, label, build, prec):
+    torch = torch_cuda
+    s = build(qm)
+    print(f"case {label} prec {prec}", flush=True)
+    H = qm.hamiltonian.Hamiltonian(s)
+    psi = s.initial_bands()
+    dtype = torch.float64 if prec == "f64" else torch.float32
+    for ik in range(s.kpoints.shape[0]):
+        want = orc.hpsi(s, psi if prec == "f64" else psi.astype(np.float32).astype(np.float64), k=s.kpoints[ik])
+        got = _gpu_apply(torch, H, psi, dtype, k_index=ik)
+        assert np.isfinite(got).all()
+        err = _rel(got, want)
+        assert err <= (F64_TOL if prec == "f64" else F32_TOL), (label, ik, err)
+    H.close()
+
+
+def test_hpsi_c1_full_size_subset(qm, orc, torch_cuda):
+    """C1 at full size (64^3, 128 bands): every band applied on the GPU in one
+    call, 8 of them checked against the oracle, plus Hermiticity over all."""
+    torch = torch_cuda
+    s = qm.system.config(1)
+    H = qm.hamiltonian.Hamiltonian(s)
+    psi = s.initial_bands()
+    for prec, dtype, tol in (("f64", torch.float64, F64_TOL), ("f32", torch.float32, F32_TOL)):
+        x = torch.from_numpy(psi).to(device="cuda", dtype=dtype)
+        y = torch.full_like(x, float("nan"))
+        H.apply(y, x)
+        torch.cuda.synchronize()
+        sel = [0, 1, 37, 64, 90, 125, 126, 127]
+        src = psi[sel] if prec == "f64" else psi[sel].astype(np.float32).astype(np.float64)
+        want = orc.hpsi(s, src)
+        got = y[sel].double().cpu().numpy()
+        assert _rel(got, want) <= tol, prec
+        # <x_i | H x_j> symmetric over the whole block (size-independent property)
+        G = (x.double() @ y.double().T).cpu().numpy()
+        assert np.abs(G - G.T).max() <= (1e-10 if prec == "f64" else 1e-4) * np.abs(G).max()
+    H.close()
+
+
+def test_hpsi_errors(qm, torch_cuda):
+    torch = torch_cuda
+    s = qm.system.config(0, nbands=2)
+    H = qm.hamiltonian.Hamiltonian(s)
+    x = torch.zeros((2, s.ng), device="cuda", dtype=torch.float64)
+    with pytest.raises(qm._lib.QMError):
+        H.apply(x, x)  # aliasing
+    y = torch.zeros_like(x)
+    with pytest.raises(qm._lib.QMError):
+        H.apply(y, x, k_index=5)
+    H.close()
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_band_density_parity(qm, orc, torch_cuda, prec):
+    torch = torch_cuda
+    s = qm.system.config(2, nbands=3, grid=40)
+    H = qm.hamiltonian.Hamiltonian(s)
+    psi = s.initial_bands()
+    occ = np.array([2.0, 1.25, 0.5])
+    dtype = torch.float64 if prec == "f64" else torch.float32
+    x = torch.from_numpy(psi).to(device="cuda", dtype=dtype)
+    rho = H.band_density(x, occ)
+    torch.cuda.synchronize()
+    src = psi if prec == "f64" else psi.astype(np.float32).astype(np.float64)
+    want = orc.band_density(s, src, occ)
+    assert _rel(rho.cpu().numpy(), want) <= 1e-13
+    H.close()
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("cfg,kidx", [(4, 0), (2, 3)])
+def test_nonlocal_forces_parity(qm, orc, torch_cuda, prec, cfg, kidx):
+    torch = torch_cuda
+    s = qm.system.config(cfg, nbands=4, grid=40)
+    H = qm.hamiltonian.Hamiltonian(s)
+    psi = s.initial_bands()
+    occ = np.array([2.0, 2.0, 1.0, 0.5])
+    dtype = torch.float64 if prec == "f64" else torch.float32
+    x = torch.from_numpy(psi).to(device="cuda", dtype=dtype)
+    e, f = H.nonlocal_forces(x, occ, k_index=kidx)
+    src = psi if prec == "f64" else psi.astype(np.float32).astype(np.float64)
+    e0, f0 = orc.nonlocal_forces(s, src, occ, k=s.kpoints[kidx])
+    tol = 1e-12 if prec == "f64" else 1e-5
+    assert abs(e - e0) <= tol * abs(e0)
+    assert np.abs(f - f0).max() <= tol * np.abs(f0).max()
+    H.close()
