@@ -85,6 +85,17 @@ int qm_hpsi_apply_f64(qm_ctx *ctx, double *d_hpsi, const double *d_psi, int nb, 
 int qm_hpsi_apply_f32(qm_ctx *ctx, float *d_hpsi, const float *d_psi, int nb, int ncomp, int ik,
                       void *stream);
 
+/* The two halves of qm_hpsi_apply_*, for callers that assemble H.psi
+ * themselves (the reference's deprecated per-band path does, qumasun_base1.matrix.h:262-291):
+ * qm_local_apply_*   d_out = -1/2 (Delta + 2 i k.grad - k^2) psi + V_loc psi (overwrites;
+ *                    LaplacianKpoint8th_ddm + mPotentialMatrix_ddm, matrix.h:298-337);
+ * qm_nonlocal_add_*  d_out += sum_a |p> E dV <p| psi (ProjectionPP_bundle,
+ *                    PseudoPotNonlocal_SvF2_kpoint.h:4239-4247). */
+int qm_local_apply_f64(qm_ctx *ctx, double *d_out, const double *d_psi, int nb, int ncomp, int ik, void *stream);
+int qm_local_apply_f32(qm_ctx *ctx, float *d_out, const float *d_psi, int nb, int ncomp, int ik, void *stream);
+int qm_nonlocal_add_f64(qm_ctx *ctx, double *d_out, const double *d_psi, int nb, int ncomp, int ik, void *stream);
+int qm_nonlocal_add_f32(qm_ctx *ctx, float *d_out, const float *d_psi, int nb, int ncomp, int ik, void *stream);
+
 /* ---------------------------------------------------------------- adjoint */
 
 /* d_rho[i] += sum_n occ[n] |psi_n(i)|^2 (FP64 accumulation).  dE_band/dV_loc(i) is
@@ -103,6 +114,27 @@ int qm_nonlocal_forces_f64(qm_ctx *ctx, const double *d_psi, const double *occ, 
                            int ik, double *forces, double *e_nl, void *stream);
 int qm_nonlocal_forces_f32(qm_ctx *ctx, const float *d_psi, const double *occ, int nb, int ncomp,
                            int ik, double *forces, double *e_nl, void *stream);
+
+/* ---------------------------------------------------------------- dense products */
+
+/* Tall-skinny Gram product C = A^T B: A is k x m, B is k x n (column-major,
+ * leading dimension k, device), C is m x n FP64 column-major (device, ldc = m).
+ * FP32 inputs are multiplied on MFMA in FP32 and summed in FP64 across K blocks.
+ * Replaces blas_DGEMM_t(M, N, K, A, B, C, ldc, 1, 0) (gyield/gy_w_dgemm.h:314-318,
+ * rocblas_dgemm at 14-38) as used for the Rayleigh-Ritz blocks
+ * (lobpcg_z_multi_mpi.h:300-558) and the projector inner products. */
+int qm_gram_f64(qm_ctx *ctx, const double *d_A, int m, const double *d_B, int n, int64_t k, double *d_C,
+                void *stream);
+int qm_gram_f32(qm_ctx *ctx, const float *d_A, int m, const float *d_B, int n, int64_t k, double *d_C,
+                void *stream);
+
+/* Y = Z C: Z is k x m (column-major, ld k, device), C is m x n (column-major, ld m,
+ * device, same type), Y is k x n (ld k, device).  Replaces blas_DGEMM_n
+ * (gyield/gy_w_dgemm.h:356-360) as used by NextVector_z2 (lobpcg_z_multi_mpi.h:1515-1598). */
+int qm_update_f64(qm_ctx *ctx, const double *d_Z, int m, const double *d_C, int n, int64_t k, double *d_Y,
+                  void *stream);
+int qm_update_f32(qm_ctx *ctx, const float *d_Z, int m, const float *d_C, int n, int64_t k, float *d_Y,
+                  void *stream);
 
 /* ---------------------------------------------------------------- LOBPCG */
 
@@ -127,6 +159,17 @@ void qm_lobpcg_destroy(qm_lobpcg *s);
  * 2611-2666) is kept. */
 int qm_lobpcg_solve(qm_lobpcg *s, void *d_x, double *eigenvalues, double *residual_norms, int iter_max,
                     int ik, int first_time, int *iters_done, void *stream);
+
+/* The subspace problem H c = l S c is solved as in lapack_ZHEGVD_emu (w_zhegvd.h:227-272):
+ * Cholesky of S, two triangular solves, a standard eigensolve, back substitution.
+ * When S is not positive definite, or the squared Cholesky pivots span less than
+ * this ratio (min/max; defaults 1e-14 for precision 64, 1e-4 for precision 32), the
+ * step is redone without the P block, and then on the eigenbasis of S with the
+ * near-null directions dropped.  No counterpart in the reference, whose
+ * LAPACK info is only printed (gy_w_zhegvd_hip.h:58-61). */
+int qm_lobpcg_set_conditioning(qm_lobpcg *s, double min_pivot_ratio_fp64, double min_pivot_ratio_fp32);
+/* Number of such fallbacks taken in the last qm_lobpcg_solve (or QM_ERR_ARG). */
+int qm_lobpcg_fallbacks(qm_lobpcg *s);
 
 /* Milliseconds spent in the last qm_lobpcg_solve per phase:
  * t[0] H.psi, t[1] Gram, t[2] subspace eigensolve, t[3] updates, t[4] other, t[5] total. */

@@ -1,0 +1,229 @@
+"""FP64 CPU restatement of QUMASUN's block LOBPCG (LOBPCG::Eigen_z_multi_mpi).
+
+TEST INFRASTRUCTURE ONLY (checker for the GPU solver; never shipped or timed
+as the product).  Follows lobpcg_z_multi_mpi.h:1732-2852:
+
+* x is scaled by sqrt(dV) on entry so that plain inner products are
+  normalised (1851), and unscaled / renormalised on exit (2796-2800).
+* eps_n = Re <x_n|A x_n>, r_n = A x_n - eps_n x_n (1873-1897).
+* P = 0 on the first call (2052-2056); otherwise A P is recomputed (2058-2065).
+* per iteration: A R (2084); Rayleigh-Ritz on [X R P] (or [X R] while P is
+  empty, 2124-2288) through the generalized eigenproblem H c = l S c (the
+  lapack_ZHEGVD(_emu) step, w_zhegvd.h:227-272; here scipy.linalg.eigh, which
+  is LAPACK's *sygvd/*hegvd);
+  X' = X Cx + R Cr + P Cp, P' = R Cr + P Cp, AX', AP' likewise (NextVector_z2, 1515-1598);
+  loss of normalisation |1 - |x'|^2| > 1e-3 -> P = R, renormalise the old X,
+  leave the loop (2611-2666); otherwise R' = AX' - X' L (2750).
+* The subspace problem is solved through the Cholesky factor of S (the
+  lapack_ZHEGVD_emu route, w_zhegvd.h:227-272).  When S is not numerically
+  positive definite, or its squared Cholesky pivots span less than `cond`, the
+  step is redone on [X R] (P dropped) and then on the S eigenbasis with
+  near-null directions dropped (rr_solve, subspace_eig).
+* Gram blocks: every block computed directly on iteration 0 and every 6th
+  iteration (S_MAT_REST_LIMIT = 6, 1808, 2093); in between the X/P blocks come
+  from the subspace recurrence (NextMatrix_z_blasmp, 1104-1436:
+  X'^H X' = Cx^H S Cx, X'^H P' = Cx^H S Cp', P'^H P' = Cp'^H S Cp', and the same with H).
+  This build computes the R blocks (X^H R, R^H R, P^H R and their H versions)
+  directly every iteration, where the reference takes X^H R and R^H P from the
+  recurrence (DIRECT_S_MATRIX_XR_RP, 58).
+* R and P columns are normalised before the Rayleigh-Ritz step (the reference's
+  EVERY_NORMALIZE_P option, 2690-2707, extended to R).  Column scaling of R and
+  P leaves the iterates unchanged in exact arithmetic and keeps S well
+  conditioned.
+* No locking: residual_threshold = 0 gives num_fix = 0 (1962-1986).
+
+Vectors are complex internally; the SoA real layout (re, im rows per band, or
+one row per band for real Gamma-point bands) is converted at the boundary.
+"""
+from __future__ import annotations
+
+import numpy as np
+import scipy.linalg
+
+LIMIT_NORM_X = 1.0e-3   # LIMIT_Z_NORM_X_FOR_RESET_MATRIX (lobpcg_z_multi_mpi.h:101)
+S_MAT_REST_LIMIT = 6    # lobpcg_z_multi_mpi.h:1808
+
+
+def soa_to_cplx(a, ncomp):
+    if ncomp == 1:
+        return np.array(a, dtype=np.float64)
+    return a[0::2] + 1j * a[1::2]
+
+
+def cplx_to_soa(z, ncomp):
+    if ncomp == 1:
+        return np.ascontiguousarray(z.real if np.iscomplexobj(z) else z)
+    out = np.empty((2 * z.shape[0], z.shape[1]))
+    out[0::2] = z.real
+    out[1::2] = z.imag
+    return out
+
+
+SVQB_DROP = 1.0e-12     # relative eigenvalue of S below which a direction is dropped
+
+
+def subspace_eig(Hm, S):
+    """H c = l S c.  Cholesky-based generalized solve (LAPACK *sygvd/*hegvd, the
+    reference's lapack_ZHEGVD path); when S is not numerically positive
+    definite (R and P nearly dependent near convergence, where the reference's
+    LAPACK call would report info > 0), fall back to the S eigenbasis with the
+    near-null directions dropped (SVQB): B = V s^-1/2, eigh(B^H H B), C = B y.
+    Dropped directions get a huge diagonal so they sort above every kept one."""
+    try:
+        return scipy.linalg.eigh(Hm, S, driver="gvd")
+    except np.linalg.LinAlgError:
+        s, V = scipy.linalg.eigh(S, driver="evd")
+        keep = s > SVQB_DROP * s.max()
+        w = np.where(keep, 1.0 / np.sqrt(np.where(keep, s, 1.0)), 0.0)
+        B = V * w[None, :]
+        Hr = np.conj(B).T @ Hm @ B
+        Hr = 0.5 * (Hr + np.conj(Hr).T)
+        Hr[np.diag_indices_from(Hr)] += np.where(keep, 0.0, 1.0e30)
+        lam, Y = scipy.linalg.eigh(Hr, driver="evd")
+        return lam, B @ Y
+
+
+def rr_solve(Hm, S, N, have_p, cond):
+    """Rayleigh-Ritz step as the GPU driver takes it: Cholesky of S with a
+    squared-pivot ratio >= cond, else retry on [X R] (P dropped), else the
+    S-eigenbasis solve.  Returns (lam, C, used_p)."""
+    sizes = [3 * N, 2 * N] if have_p else [2 * N]
+    for n3 in sizes:
+        Ss, Hs = S[:n3, :n3], Hm[:n3, :n3]
+        try:
+            U = scipy.linalg.cholesky(Ss, lower=False)
+        except np.linalg.LinAlgError:
+            continue
+        d = np.abs(np.diag(U)) ** 2
+        if d.min() < cond * d.max():
+            continue
+        lam, C = scipy.linalg.eigh(Hs, Ss, driver="gvd")
+        return lam, C, n3 == 3 * N
+    n3 = 2 * N
+    lam, C = subspace_eig(Hm[:n3, :n3], S[:n3, :n3])
+    return lam, C, False
+
+
+class LOBPCGRef:
+    """Keeps P between calls like the reference's `keep` block."""
+
+    def __init__(self, apply_soa, ncomp, dV, cond=1.0e-14):
+        self.apply_soa = apply_soa
+        self.ncomp = ncomp
+        self.dV = dV
+        self.cond = cond  # minimum squared Cholesky pivot ratio of S
+        self.P = None
+
+    def _A(self, z):
+        return soa_to_cplx(self.apply_soa(cplx_to_soa(z, self.ncomp)), self.ncomp)
+
+    @staticmethod
+    def _gram(a, b):
+        """a^H b with bands as rows: (Na x Ng) . (Nb x Ng)^T."""
+        return np.conj(a) @ b.T
+
+    def solve(self, X_soa, iter_max, first_time=True, return_history=False):
+        nc = self.ncomp
+        dt = np.complex128 if nc == 2 else np.float64
+        x = soa_to_cplx(X_soa, nc).astype(dt) * np.sqrt(self.dV)
+        N = x.shape[0]
+        Ax = self._A(x)
+        eps = np.real(np.sum(np.conj(x) * Ax, axis=1))
+        r = Ax - eps[:, None] * x
+        if first_time or self.P is None:
+            p = np.zeros_like(x)
+            Ap = np.zeros_like(x)
+            have_p = False
+        else:
+            p = self.P.astype(dt)
+            Ap = self._A(p)
+            have_p = True
+        blocks = {}
+        history = []
+        rnorm2 = np.sum(np.abs(r) ** 2, axis=1)
+        it_done = 0
+        for it in range(iter_max):
+            it_done = it + 1
+            # normalise R (exact-arithmetic invariant), then A R
+            rn = np.sqrt(np.sum(np.abs(r) ** 2, axis=1))
+            rn[rn == 0] = 1.0
+            r = r / rn[:, None]
+            Ar = self._A(r)
+            G = self._gram
+            recompute = (it % S_MAT_REST_LIMIT == 0)
+            if recompute:
+                blocks["XX"], blocks["XAX"] = G(x, x), G(x, Ax)
+                if have_p:
+                    blocks["XP"], blocks["PP"] = G(x, p), G(p, p)
+                    blocks["XAP"], blocks["PAP"] = G(x, Ap), G(p, Ap)
+            blocks["XR"], blocks["RR"], blocks["XAR"], blocks["RAR"] = G(x, r), G(r, r), G(x, Ar), G(r, Ar)
+            if have_p:
+                # R^H P and R^H A P taken as (P^H R)^H and (P^H A R)^H, as the GPU Gram does
+                blocks["RP"], blocks["RAP"] = G(p, r).conj().T, G(p, Ar).conj().T
+            nb = 3 if have_p else 2
+            S = np.zeros((nb * N, nb * N), dtype=dt)
+            Hm = np.zeros((nb * N, nb * N), dtype=dt)
+            b = blocks
+            S[:N, :N], S[:N, N:2 * N], S[N:2 * N, N:2 * N] = b["XX"], b["XR"], b["RR"]
+            Hm[:N, :N], Hm[:N, N:2 * N], Hm[N:2 * N, N:2 * N] = b["XAX"], b["XAR"], b["RAR"]
+            if have_p:
+                S[:N, 2 * N:], S[N:2 * N, 2 * N:], S[2 * N:, 2 * N:] = b["XP"], b["RP"], b["PP"]
+                Hm[:N, 2 * N:], Hm[N:2 * N, 2 * N:], Hm[2 * N:, 2 * N:] = b["XAP"], b["RAP"], b["PAP"]
+            # Hermitian from the upper triangle of blocks (Conj_U_to_L, conjugate.h)
+            S = np.triu(S) + np.triu(S, 1).conj().T
+            Hm = np.triu(Hm) + np.triu(Hm, 1).conj().T
+            lam, C, used_p = rr_solve(Hm, S, N, have_p, self.cond)
+            if not used_p:  # solved on [X R]: continue as if P were empty
+                S, Hm = S[:2 * N, :2 * N], Hm[:2 * N, :2 * N]
+                have_p = False
+            lam, C = lam[:N], C[:, :N]
+            Cx, Cr = C[:N], C[N:2 * N]
+            Cp = C[2 * N:] if have_p else None
+            # X' = X Cx + R Cr + P Cp ; P' = R Cr + P Cp  (bands are rows: Z' = C^T Z)
+            pn = Cr.T @ r + (Cp.T @ p if have_p else 0)
+            xn = Cx.T @ x + pn
+            Apn = Cr.T @ Ar + (Cp.T @ Ap if have_p else 0)
+            Axn = Cx.T @ Ax + Apn
+            norms = np.sum(np.abs(xn) ** 2, axis=1)
+            if np.max(np.abs(1.0 - norms)) > LIMIT_NORM_X:
+                # rollback and escape (2611-2666): P = R and leave the loop.  The
+                # reference rescales the old x by the norms of the rejected update
+                # (Mul_invSqV_bundle with sum_norms, 2639), which leaves x
+                # unnormalised exactly when the update lost normalisation; the old
+                # x is renormalised with its own norms here.
+                p = r.copy()
+                old = np.sum(np.abs(x) ** 2, axis=1)
+                x = x / np.sqrt(old * self.dV)[:, None]
+                self.P = p
+                out = cplx_to_soa(x, nc)
+                return (out, eps, rnorm2, it_done, history) if return_history else (out, eps, rnorm2, it_done)
+            final = (it + 1 == iter_max)
+            # recurrence for the X / P blocks of the next iteration (1104-1436)
+            Cp0 = np.concatenate([np.zeros((N, N), dtype=dt), Cr] + ([Cp] if have_p else []), axis=0)
+            Cb = np.concatenate([C, Cp0], axis=1)
+            SS = np.conj(Cb).T @ (S @ Cb)
+            HH = np.conj(Cb).T @ (Hm @ Cb)
+            # normalise P' (and A P') by its norms, scaling the recurrence blocks alike
+            pnorm = np.sqrt(np.real(np.diag(SS)[N:]))
+            pnorm[pnorm == 0] = 1.0
+            pn = pn / pnorm[:, None]
+            Apn = Apn / pnorm[:, None]
+            D = 1.0 / pnorm
+            blocks["XX"] = SS[:N, :N]
+            blocks["XP"] = SS[:N, N:] * D[None, :]
+            blocks["PP"] = SS[N:, N:] * D[:, None] * D[None, :]
+            blocks["XAX"] = HH[:N, :N]
+            blocks["XAP"] = HH[:N, N:] * D[None, :]
+            blocks["PAP"] = HH[N:, N:] * D[:, None] * D[None, :]
+            x, p, Ax, Ap = xn, pn, Axn, Apn
+            have_p = True
+            eps = lam
+            r = Ax - eps[:, None] * x
+            rnorm2 = np.sum(np.abs(r) ** 2, axis=1)
+            if return_history:
+                history.append(lam.copy())
+            if final:
+                x = x / np.sqrt(norms * self.dV)[:, None]
+        self.P = p
+        out = cplx_to_soa(x, nc)
+        return (out, eps, rnorm2, it_done, history) if return_history else (out, eps, rnorm2, it_done)

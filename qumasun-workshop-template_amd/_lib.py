@@ -14,9 +14,11 @@ LIB_PATH = os.path.join(_PKG, "libqumasun_hip.so")
 # every entry point declared in include/qumasun_hip.h
 EXPORTS = (
     "qm_last_error", "qm_create", "qm_destroy", "qm_set_vloc", "qm_set_projectors", "qm_set_kpoints",
-    "qm_hpsi_apply_f64", "qm_hpsi_apply_f32", "qm_band_density_f64", "qm_band_density_f32",
-    "qm_nonlocal_forces_f64", "qm_nonlocal_forces_f32", "qm_lobpcg_create", "qm_lobpcg_destroy",
-    "qm_lobpcg_solve", "qm_lobpcg_timings",
+    "qm_hpsi_apply_f64", "qm_hpsi_apply_f32", "qm_local_apply_f64", "qm_local_apply_f32",
+    "qm_nonlocal_add_f64", "qm_nonlocal_add_f32", "qm_band_density_f64", "qm_band_density_f32",
+    "qm_nonlocal_forces_f64", "qm_nonlocal_forces_f32", "qm_gram_f64", "qm_gram_f32", "qm_update_f64",
+    "qm_update_f32", "qm_lobpcg_create", "qm_lobpcg_destroy",
+    "qm_lobpcg_solve", "qm_lobpcg_set_conditioning", "qm_lobpcg_fallbacks", "qm_lobpcg_timings",
 )
 
 QM_OK = 0
@@ -50,17 +52,22 @@ def load():
     L.qm_set_vloc.argtypes = [P, P]
     L.qm_set_projectors.argtypes = [P, I, P, P, P, P, P, P, P, P]
     L.qm_set_kpoints.argtypes = [P, I, P]
-    for n in ("qm_hpsi_apply_f64", "qm_hpsi_apply_f32"):
+    for n in ("qm_hpsi_apply_f64", "qm_hpsi_apply_f32", "qm_local_apply_f64", "qm_local_apply_f32",
+              "qm_nonlocal_add_f64", "qm_nonlocal_add_f32"):
         getattr(L, n).argtypes = [P, P, P, I, I, I, P]
     for n in ("qm_band_density_f64", "qm_band_density_f32"):
         getattr(L, n).argtypes = [P, P, P, P, I, I, P]
     for n in ("qm_nonlocal_forces_f64", "qm_nonlocal_forces_f32"):
         getattr(L, n).argtypes = [P, P, P, I, I, I, P, P, P]
+    for n in ("qm_gram_f64", "qm_gram_f32", "qm_update_f64", "qm_update_f32"):
+        getattr(L, n).argtypes = [P, P, I, P, I, L64, P, P]
     L.qm_lobpcg_create.argtypes = [ctypes.POINTER(P), P, I, I, I]
     L.qm_lobpcg_destroy.argtypes = [P]
     L.qm_lobpcg_destroy.restype = None
     L.qm_lobpcg_solve.argtypes = [P, P, P, P, I, I, I, ctypes.POINTER(I), P]
     L.qm_lobpcg_timings.argtypes = [P, P]
+    L.qm_lobpcg_set_conditioning.argtypes = [P, D, D]
+    L.qm_lobpcg_fallbacks.argtypes = [P]
     for n in EXPORTS:
         if n not in ("qm_last_error", "qm_destroy", "qm_lobpcg_destroy"):
             getattr(L, n).restype = I

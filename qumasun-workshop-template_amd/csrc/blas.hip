@@ -1,0 +1,409 @@
+// Tall-skinny MFMA products and band reductions for the LOBPCG driver.
+//
+// gram:   C (m x n, FP64) = A^T B with A = Ng x m, B = Ng x n, K = Ng (the
+//         Rayleigh-Ritz Gram blocks of MakeMatrix_z_blas, lobpcg_z_multi_mpi.h:300-558,
+//         there one blas_DGEMM_t per block on the Ng x 2N real view of SoA bands).
+//         One launch covers every block of [X R P]^T [R AR] (or the full
+//         recompute) by chaining column blocks.  K is split into chunks across
+//         workgroups; each wave accumulates with MFMA (FP32: v_mfma_f32_16x16x4f32,
+//         FP64: v_mfma_f64_16x16x4f64), flushes the FP32 accumulators into FP64
+//         registers every 256 K values, and the per-chunk FP64 partials are summed
+//         in a fixed order by a second kernel (deterministic FP64 cross-block reduction).
+// update: Y (Ng x n) = Z (Ng x m) C (m x n) with small C (NextVector_z2,
+//         lobpcg_z_multi_mpi.h:1515-1598), computed as Y^T = C^T Z^T so the MFMA
+//         B operand and the result are 16 consecutive grid points per lane group.
+// band reductions: per-band sums over Ng (both SoA components) in FP64 for
+//         eps = Re <x|Ax>, |r|^2 with r = Ax - eps x, |x|^2 (innerprod_hermite.h:12-133).
+#include <algorithm>
+
+#include "qm_internal.h"
+
+namespace qm {
+namespace {
+
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<float> {
+    typedef float acc_t __attribute__((ext_vector_type(4)));
+    typedef float vec_t __attribute__((ext_vector_type(4)));
+    static constexpr int VEC = 4;
+    static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    // output row of item r in lane group g (lane l: g = l / 16, column l % 16)
+    static __device__ __forceinline__ int row(int g, int r) { return 4 * g + r; }
+};
+template <>
+struct Mfma<double> {
+    typedef double acc_t __attribute__((ext_vector_type(4)));
+    typedef double vec_t __attribute__((ext_vector_type(2)));
+    static constexpr int VEC = 2;
+    static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    // the FP64 16x16x4 result interleaves rows across lane groups
+    static __device__ __forceinline__ int row(int g, int r) { return g + 4 * r; }
+};
+
+template <typename T>
+__device__ __forceinline__ const T *col_ptr(const ColBlocks &c, int j, int64_t ng) {
+#pragma unroll
+    for (int b = 0; b < kMaxColBlocks; ++b) {
+        if (b >= c.nblk) break;
+        if (j < c.ncols[b]) return static_cast<const T *>(c.ptr[b]) + (int64_t)j * ng;
+        j -= c.ncols[b];
+    }
+    return nullptr;
+}
+
+template <typename T>
+__device__ __forceinline__ T *col_ptr_out(const ColBlocks &c, int j, int64_t ng) {
+    return const_cast<T *>(col_ptr<T>(c, j, ng));
+}
+
+constexpr int GRAM_FLUSH = 16;  // MFMA K-steps between FP32 -> FP64 flushes
+
+// Wave tile 32 x 32 (2 x 2 MFMA tiles), workgroup tile 64 x 64 (4 waves).
+// partial[chunk][j * m + i] = sum_{k in chunk} A[k][i] B[k][j]
+template <typename T, bool VECLOAD>
+__global__ __launch_bounds__(256) void gram_partial_kernel(ColBlocks A, ColBlocks B, int m, int n, int64_t ng,
+                                                           int64_t kchunk, int tiles_n, double *__restrict__ partial) {
+    using M = Mfma<T>;
+    using acc_t = typename M::acc_t;
+    using vec_t = typename M::vec_t;
+    constexpr int VEC = M::VEC;
+    constexpr int KSTEP = 4 * VEC;
+    const int tile = blockIdx.x;
+    const int ti = tile / tiles_n, tj = tile % tiles_n;
+    const int chunk = blockIdx.y;
+    const int64_t k0 = (int64_t)chunk * kchunk;
+    const int64_t k1 = min(k0 + kchunk, ng);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int li = lane & 15, g = lane >> 4;
+    const int i0 = ti * 64 + (wave >> 1) * 32;
+    const int j0 = tj * 64 + (wave & 1) * 32;
+
+    const T *pa[2], *pb[2];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        const int ia = i0 + 16 * s + li, jb = j0 + 16 * s + li;
+        pa[s] = ia < m ? col_ptr<T>(A, ia, ng) : nullptr;
+        pb[s] = jb < n ? col_ptr<T>(B, jb, ng) : nullptr;
+    }
+    acc_t acc[2][2];
+    double acc64[2][2][4];
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            acc[s][t] = acc_t{0, 0, 0, 0};
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc64[s][t][r] = 0.0;
+        }
+
+    int steps = 0;
+    int64_t k = k0;
+    for (; k + KSTEP <= k1; k += KSTEP) {
+        T va[2][VEC], vb[2][VEC];
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            if constexpr (VECLOAD) {
+                vec_t a = pa[s] ? *reinterpret_cast<const vec_t *>(pa[s] + k + VEC * g) : vec_t{};
+                vec_t b = pb[s] ? *reinterpret_cast<const vec_t *>(pb[s] + k + VEC * g) : vec_t{};
+#pragma unroll
+                for (int c = 0; c < VEC; ++c) {
+                    va[s][c] = a[c];
+                    vb[s][c] = b[c];
+                }
+            } else {
+#pragma unroll
+                for (int c = 0; c < VEC; ++c) {
+                    va[s][c] = pa[s] ? pa[s][k + VEC * g + c] : T(0);
+                    vb[s][c] = pb[s] ? pb[s][k + VEC * g + c] : T(0);
+                }
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < VEC; ++c)
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int t = 0; t < 2; ++t) acc[s][t] = M::mma(va[s][c], vb[t][c], acc[s][t]);
+        if constexpr (sizeof(T) == 4) {
+            if (++steps == GRAM_FLUSH) {
+                steps = 0;
+#pragma unroll
+                for (int s = 0; s < 2; ++s)
+#pragma unroll
+                    for (int t = 0; t < 2; ++t) {
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) acc64[s][t][r] += (double)acc[s][t][r];
+                        acc[s][t] = acc_t{0, 0, 0, 0};
+                    }
+            }
+        }
+    }
+    if (k < k1) {  // tail: guarded scalar loads
+        T va[2][VEC], vb[2][VEC];
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int c = 0; c < VEC; ++c) {
+                const int64_t kk = k + VEC * g + c;
+                va[s][c] = (pa[s] && kk < k1) ? pa[s][kk] : T(0);
+                vb[s][c] = (pb[s] && kk < k1) ? pb[s][kk] : T(0);
+            }
+#pragma unroll
+        for (int c = 0; c < VEC; ++c)
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int t = 0; t < 2; ++t) acc[s][t] = M::mma(va[s][c], vb[t][c], acc[s][t]);
+    }
+    double *out = partial + (int64_t)chunk * m * n;
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int j = j0 + 16 * t + li;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = i0 + 16 * s + M::row(g, r);
+                if (i < m && j < n) out[(int64_t)j * m + i] = acc64[s][t][r] + (double)acc[s][t][r];
+            }
+        }
+}
+
+__global__ void gram_reduce_kernel(double *__restrict__ C, const double *__restrict__ partial, int64_t mn,
+                                   int nchunk) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= mn) return;
+    double s = 0.0;
+    for (int c = 0; c < nchunk; ++c) s += partial[(int64_t)c * mn + e];
+    C[e] = s;
+}
+
+// Y^T (n x Ng) = C^T (n x m) Z^T (m x Ng).  Workgroup: 64 output columns x 128
+// grid points; each wave 64 columns x 32 points (4 x 2 MFMA tiles).  C tiles of
+// 32 K rows x 64 columns are staged in LDS.
+constexpr int UPD_KC = 32;
+template <typename T>
+__global__ __launch_bounds__(256) void update_kernel(ColBlocks Z, ColBlocks Y, const T *__restrict__ Cm, int m,
+                                                     int n, int64_t ng, int tiles_n) {
+    using M = Mfma<T>;
+    using acc_t = typename M::acc_t;
+    __shared__ T cs[UPD_KC][64];
+    const int tj = blockIdx.x % tiles_n;   // output-column tile (fastest: neighbours share Z rows in L2)
+    const int64_t gt = blockIdx.x / tiles_n;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int li = lane & 15, g = lane >> 4;
+    const int c0 = tj * 64;
+    const int64_t p0 = gt * 128 + wave * 32;
+    acc_t acc[4][2];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
+    const bool pin[2] = {p0 + li < ng, p0 + 16 + li < ng};
+
+    for (int kc = 0; kc < m; kc += UPD_KC) {
+        __syncthreads();
+        for (int e = threadIdx.x; e < UPD_KC * 64; e += 256) {
+            const int kk = e % UPD_KC, cc = e / UPD_KC;  // consecutive threads: consecutive K (column-major C)
+            cs[kk][cc] = (kc + kk < m && c0 + cc < n) ? Cm[(int64_t)(c0 + cc) * m + kc + kk] : T(0);
+        }
+        __syncthreads();
+#pragma unroll 2
+        for (int ks = 0; ks < UPD_KC; ks += 4) {
+            const int kcol = kc + ks + g;
+            const T *zc = kcol < m ? col_ptr<T>(Z, kcol, ng) : nullptr;
+            T bv[2];
+#pragma unroll
+            for (int b = 0; b < 2; ++b) bv[b] = (zc && pin[b]) ? zc[p0 + 16 * b + li] : T(0);
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                const T av = cs[ks + g][16 * a + li];
+#pragma unroll
+                for (int b = 0; b < 2; ++b) acc[a][b] = M::mma(av, bv[b], acc[a][b]);
+            }
+        }
+    }
+    // D[row = out column 16a + row(g, r)][col = grid point 16b + li]
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int col = c0 + 16 * a + M::row(g, r);
+            if (col >= n) continue;
+            T *yc = col_ptr_out<T>(Y, col, ng);
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+                if (pin[b]) yc[p0 + 16 * b + li] = acc[a][b][r];
+        }
+}
+
+// Per-band FP64 partial sums over a chunk of the grid.  mode 0: sum x*ax;
+// mode 1: r = ax - lam x (written), sums |r|^2 and |x|^2; mode 2: sum x*x.
+constexpr int RED_THREADS = 256;
+template <typename T, int MODE>
+__global__ __launch_bounds__(RED_THREADS) void band_reduce_kernel(double *__restrict__ part, T *__restrict__ r,
+                                                                   const T *__restrict__ x, const T *__restrict__ ax,
+                                                                   const double *__restrict__ lam, int64_t ng, int ncomp,
+                                                                   int64_t chunk, int nchunk) {
+    __shared__ double sm[2][RED_THREADS / 64];
+    const int n = blockIdx.y;
+    const int64_t b0 = (int64_t)blockIdx.x * chunk;
+    const int64_t b1 = min(b0 + chunk, ng);
+    double s0 = 0.0, s1 = 0.0;
+    const T l = MODE == 1 ? (T)lam[n] : T(0);
+    for (int c = 0; c < ncomp; ++c) {
+        const int64_t off = ((int64_t)n * ncomp + c) * ng;
+        for (int64_t i = b0 + threadIdx.x; i < b1; i += RED_THREADS) {
+            const T xv = x[off + i];
+            if constexpr (MODE == 0) {
+                s0 += (double)xv * (double)ax[off + i];
+            } else if constexpr (MODE == 1) {
+                const T rv = ax[off + i] - l * xv;
+                r[off + i] = rv;
+                s0 += (double)rv * (double)rv;
+                s1 += (double)xv * (double)xv;
+            } else {
+                s0 += (double)xv * (double)xv;
+            }
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        s0 += __shfl_xor(s0, off, 64);
+        s1 += __shfl_xor(s1, off, 64);
+    }
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        sm[0][wave] = s0;
+        sm[1][wave] = s1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t0 = 0.0, t1 = 0.0;
+        for (int w = 0; w < RED_THREADS / 64; ++w) {
+            t0 += sm[0][w];
+            t1 += sm[1][w];
+        }
+        part[((int64_t)n * nchunk + blockIdx.x) * 2] = t0;
+        part[((int64_t)n * nchunk + blockIdx.x) * 2 + 1] = t1;
+    }
+}
+
+__global__ void band_reduce_finish(double *__restrict__ out0, double *__restrict__ out1, const double *__restrict__ part,
+                                   int nb, int nchunk) {
+    const int n = blockIdx.x * 64 + threadIdx.x;
+    if (n >= nb) return;
+    double t0 = 0.0, t1 = 0.0;
+    for (int c = 0; c < nchunk; ++c) {
+        t0 += part[((int64_t)n * nchunk + c) * 2];
+        t1 += part[((int64_t)n * nchunk + c) * 2 + 1];
+    }
+    if (out0) out0[n] = t0;
+    if (out1) out1[n] = t1;
+}
+
+template <typename T>
+__global__ void scale_bands_kernel(T *__restrict__ y, const T *__restrict__ x, const double *__restrict__ f,
+                                   double fconst, int64_t ng, int ncomp) {
+    const int n = blockIdx.y;
+    const double s = f ? f[n] : fconst;
+    for (int c = 0; c < ncomp; ++c) {
+        const int64_t off = ((int64_t)n * ncomp + c) * ng;
+        for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < ng; i += (int64_t)gridDim.x * 256)
+            y[off + i] = (T)(s * (double)x[off + i]);
+    }
+}
+
+int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
+
+}  // namespace
+
+template <typename T>
+int gram(DevBuf &partial, double *C, const ColBlocks &A, const ColBlocks &B, int m, int n, int64_t ng,
+         hipStream_t st) {
+    constexpr int KSTEP = 4 * Mfma<T>::VEC;
+    const int tiles_m = (m + 63) / 64, tiles_n = (n + 63) / 64;
+    const int tiles = tiles_m * tiles_n;
+    // enough workgroups to fill the 256 CUs a few times; K chunks of >= 1024
+    int64_t nchunk = std::max<int64_t>(1, std::min<int64_t>(64, 2048 / tiles));
+    int64_t kchunk = round_up((ng + nchunk - 1) / nchunk, (int64_t)KSTEP * GRAM_FLUSH);
+    kchunk = std::max<int64_t>(kchunk, 1024);
+    nchunk = (ng + kchunk - 1) / kchunk;
+    QM_TRY(partial.reserve(sizeof(double) * (size_t)nchunk * m * n));
+    const bool vec = (ng % Mfma<T>::VEC) == 0;
+    dim3 grid(tiles, (unsigned)nchunk);
+    if (vec)
+        hipLaunchKernelGGL((gram_partial_kernel<T, true>), grid, dim3(256), 0, st, A, B, m, n, ng, kchunk, tiles_n,
+                           partial.as<double>());
+    else
+        hipLaunchKernelGGL((gram_partial_kernel<T, false>), grid, dim3(256), 0, st, A, B, m, n, ng, kchunk, tiles_n,
+                           partial.as<double>());
+    QM_HIP(hipGetLastError());
+    const int64_t mn = (int64_t)m * n;
+    hipLaunchKernelGGL(gram_reduce_kernel, dim3((unsigned)((mn + 255) / 256)), dim3(256), 0, st, C,
+                       partial.as<double>(), mn, (int)nchunk);
+    QM_HIP(hipGetLastError());
+    return QM_OK;
+}
+
+template <typename T>
+int update(const ColBlocks &Z, const ColBlocks &Y, const T *Cm, int m, int n, int64_t ng, hipStream_t st) {
+    const int tiles_n = (n + 63) / 64;
+    const int64_t gtiles = (ng + 127) / 128;
+    QM_CHECK(gtiles * tiles_n < ((int64_t)1 << 31), QM_ERR_ARG, "update: grid too large");
+    hipLaunchKernelGGL((update_kernel<T>), dim3((unsigned)(gtiles * tiles_n)), dim3(256), 0, st, Z, Y, Cm, m, n, ng,
+                       tiles_n);
+    QM_HIP(hipGetLastError());
+    return QM_OK;
+}
+
+template <typename T>
+int band_reduce(DevBuf &part, int mode, double *out0, double *out1, T *r, const T *x, const T *ax, const double *lam,
+                int nb, int ncomp, int64_t ng, hipStream_t st) {
+    const int64_t chunk = std::max<int64_t>(4096, round_up((ng + 63) / 64, 256));
+    const int nchunk = (int)((ng + chunk - 1) / chunk);
+    QM_CHECK(nb <= 65535, QM_ERR_ARG, "band_reduce: too many bands");
+    QM_TRY(part.reserve(sizeof(double) * 2 * (size_t)nb * nchunk));
+    dim3 grid(nchunk, nb);
+    if (mode == 0)
+        hipLaunchKernelGGL((band_reduce_kernel<T, 0>), grid, dim3(RED_THREADS), 0, st, part.as<double>(), r, x, ax, lam,
+                           ng, ncomp, chunk, nchunk);
+    else if (mode == 1)
+        hipLaunchKernelGGL((band_reduce_kernel<T, 1>), grid, dim3(RED_THREADS), 0, st, part.as<double>(), r, x, ax, lam,
+                           ng, ncomp, chunk, nchunk);
+    else
+        hipLaunchKernelGGL((band_reduce_kernel<T, 2>), grid, dim3(RED_THREADS), 0, st, part.as<double>(), r, x, ax, lam,
+                           ng, ncomp, chunk, nchunk);
+    QM_HIP(hipGetLastError());
+    hipLaunchKernelGGL(band_reduce_finish, dim3((nb + 63) / 64), dim3(64), 0, st, out0, out1, part.as<double>(), nb,
+                       nchunk);
+    QM_HIP(hipGetLastError());
+    return QM_OK;
+}
+
+template <typename T>
+int scale_bands(T *y, const T *x, const double *f, double fconst, int nb, int ncomp, int64_t ng, hipStream_t st) {
+    const int64_t bx = std::min<int64_t>((ng + 255) / 256, 1024);
+    hipLaunchKernelGGL((scale_bands_kernel<T>), dim3((unsigned)bx, nb), dim3(256), 0, st, y, x, f, fconst, ng, ncomp);
+    QM_HIP(hipGetLastError());
+    return QM_OK;
+}
+
+template int gram<float>(DevBuf &, double *, const ColBlocks &, const ColBlocks &, int, int, int64_t, hipStream_t);
+template int gram<double>(DevBuf &, double *, const ColBlocks &, const ColBlocks &, int, int, int64_t, hipStream_t);
+template int update<float>(const ColBlocks &, const ColBlocks &, const float *, int, int, int64_t, hipStream_t);
+template int update<double>(const ColBlocks &, const ColBlocks &, const double *, int, int, int64_t, hipStream_t);
+template int band_reduce<float>(DevBuf &, int, double *, double *, float *, const float *, const float *,
+                                const double *, int, int, int64_t, hipStream_t);
+template int band_reduce<double>(DevBuf &, int, double *, double *, double *, const double *, const double *,
+                                 const double *, int, int, int64_t, hipStream_t);
+template int scale_bands<float>(float *, const float *, const double *, double, int, int, int64_t, hipStream_t);
+template int scale_bands<double>(double *, const double *, const double *, double, int, int, int64_t, hipStream_t);
+
+}  // namespace qm

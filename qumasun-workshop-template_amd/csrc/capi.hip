@@ -119,8 +119,9 @@ static int build_phase(qm_ctx *ctx, KPoint &kp) {
     return QM_OK;
 }
 
+// part: 3 = local + nonlocal (overwrite), 1 = local only (overwrite), 2 = nonlocal only (accumulate)
 template <typename T>
-int hpsi_apply(qm_ctx *ctx, T *hpsi, const T *psi, int nb, int ncomp, int ik, void *stream) {
+int hpsi_apply(qm_ctx *ctx, T *hpsi, const T *psi, int nb, int ncomp, int ik, void *stream, int part = 3) {
     QM_CHECK(ctx && hpsi && psi, QM_ERR_ARG, "qm_hpsi_apply: NULL argument");
     QM_CHECK(nb >= 0 && (ncomp == 1 || ncomp == 2), QM_ERR_ARG, "qm_hpsi_apply: bad nb / ncomp");
     QM_CHECK(ik >= 0 && ik < (int)ctx->kpts.size(), QM_ERR_ARG, "qm_hpsi_apply: k index out of range");
@@ -131,17 +132,17 @@ int hpsi_apply(qm_ctx *ctx, T *hpsi, const T *psi, int nb, int ncomp, int ik, vo
     hipStream_t st = (hipStream_t)stream;
     const KPoint &kp = ctx->kpts[ik];
     if constexpr (sizeof(T) == 8) {
-        QM_TRY(launch_fd8_local_f64(ctx, hpsi, psi, nb, ncomp, kp.k, st));
-        QM_TRY(launch_nonlocal_f64(ctx, hpsi, psi, nb, ncomp, kp, st));
+        if (part & 1) QM_TRY(launch_fd8_local_f64(ctx, hpsi, psi, nb, ncomp, kp.k, st));
+        if (part & 2) QM_TRY(launch_nonlocal_f64(ctx, hpsi, psi, nb, ncomp, kp, st));
     } else {
-        QM_TRY(launch_fd8_local_f32(ctx, hpsi, psi, nb, ncomp, kp.k, st));
-        QM_TRY(launch_nonlocal_f32(ctx, hpsi, psi, nb, ncomp, kp, st));
+        if (part & 1) QM_TRY(launch_fd8_local_f32(ctx, hpsi, psi, nb, ncomp, kp.k, st));
+        if (part & 2) QM_TRY(launch_nonlocal_f32(ctx, hpsi, psi, nb, ncomp, kp, st));
     }
     return QM_OK;
 }
 
 template <typename T>
-int band_density(qm_ctx *ctx, double *rho, const T *psi, const double *occ, int nb, int ncomp,
+int band_density_impl(qm_ctx *ctx, double *rho, const T *psi, const double *occ, int nb, int ncomp,
                         void *stream) {
     QM_CHECK(ctx && rho && psi && occ, QM_ERR_ARG, "qm_band_density: NULL argument");
     QM_CHECK(nb >= 0 && (ncomp == 1 || ncomp == 2), QM_ERR_ARG, "qm_band_density: bad nb / ncomp");
@@ -173,6 +174,32 @@ int forces(qm_ctx *ctx, const T *psi, const double *occ, int nb, int ncomp, int 
         return nonlocal_forces_f64(ctx, psi, occ, nb, ncomp, ctx->kpts[ik], f, e, (hipStream_t)stream);
     else
         return nonlocal_forces_f32(ctx, psi, occ, nb, ncomp, ctx->kpts[ik], f, e, (hipStream_t)stream);
+}
+
+template <typename T>
+int gram_api(qm_ctx *ctx, const T *A, int m, const T *B, int n, int64_t k, double *C, void *stream) {
+    QM_CHECK(ctx && A && B && C, QM_ERR_ARG, "qm_gram: NULL argument");
+    QM_CHECK(m >= 1 && n >= 1 && k >= 1, QM_ERR_ARG, "qm_gram: sizes must be positive");
+    QM_TRY(activate(ctx));
+    ColBlocks a, b;
+    a.add(A, m);
+    b.add(B, n);
+    hipStream_t st = (hipStream_t)stream;
+    QM_TRY(gram<T>(ctx->gemm_part, C, a, b, m, n, k, st));
+    // the partial buffer is reused by the next call: finish before returning
+    QM_HIP(hipStreamSynchronize(st));
+    return QM_OK;
+}
+
+template <typename T>
+int update_api(qm_ctx *ctx, const T *Z, int m, const T *C, int n, int64_t k, T *Y, void *stream) {
+    QM_CHECK(ctx && Z && C && Y, QM_ERR_ARG, "qm_update: NULL argument");
+    QM_CHECK(m >= 1 && n >= 1 && k >= 1, QM_ERR_ARG, "qm_update: sizes must be positive");
+    QM_TRY(activate(ctx));
+    ColBlocks z, y;
+    z.add(Z, m);
+    y.add(Y, n);
+    return update<T>(z, y, C, m, n, k, (hipStream_t)stream);
 }
 
 }  // namespace qm
@@ -216,6 +243,7 @@ void qm_destroy(qm_ctx *ctx) {
     free_kpoints(ctx);
     ctx->coef.release();
     ctx->fwork.release();
+    ctx->gemm_part.release();
     delete ctx;
 }
 
@@ -347,14 +375,50 @@ int qm_hpsi_apply_f32(qm_ctx *ctx, float *d_hpsi, const float *d_psi, int nb, in
     return hpsi_apply<float>(ctx, d_hpsi, d_psi, nb, ncomp, ik, stream);
 }
 
+int qm_gram_f64(qm_ctx *ctx, const double *d_A, int m, const double *d_B, int n, int64_t k, double *d_C,
+                void *stream) {
+    return gram_api<double>(ctx, d_A, m, d_B, n, k, d_C, stream);
+}
+
+int qm_gram_f32(qm_ctx *ctx, const float *d_A, int m, const float *d_B, int n, int64_t k, double *d_C,
+                void *stream) {
+    return gram_api<float>(ctx, d_A, m, d_B, n, k, d_C, stream);
+}
+
+int qm_update_f64(qm_ctx *ctx, const double *d_Z, int m, const double *d_C, int n, int64_t k, double *d_Y,
+                  void *stream) {
+    return update_api<double>(ctx, d_Z, m, d_C, n, k, d_Y, stream);
+}
+
+int qm_update_f32(qm_ctx *ctx, const float *d_Z, int m, const float *d_C, int n, int64_t k, float *d_Y,
+                  void *stream) {
+    return update_api<float>(ctx, d_Z, m, d_C, n, k, d_Y, stream);
+}
+
+int qm_local_apply_f64(qm_ctx *ctx, double *d_out, const double *d_psi, int nb, int ncomp, int ik, void *stream) {
+    return hpsi_apply<double>(ctx, d_out, d_psi, nb, ncomp, ik, stream, 1);
+}
+
+int qm_local_apply_f32(qm_ctx *ctx, float *d_out, const float *d_psi, int nb, int ncomp, int ik, void *stream) {
+    return hpsi_apply<float>(ctx, d_out, d_psi, nb, ncomp, ik, stream, 1);
+}
+
+int qm_nonlocal_add_f64(qm_ctx *ctx, double *d_out, const double *d_psi, int nb, int ncomp, int ik, void *stream) {
+    return hpsi_apply<double>(ctx, d_out, d_psi, nb, ncomp, ik, stream, 2);
+}
+
+int qm_nonlocal_add_f32(qm_ctx *ctx, float *d_out, const float *d_psi, int nb, int ncomp, int ik, void *stream) {
+    return hpsi_apply<float>(ctx, d_out, d_psi, nb, ncomp, ik, stream, 2);
+}
+
 int qm_band_density_f64(qm_ctx *ctx, double *d_rho, const double *d_psi, const double *occ, int nb, int ncomp,
                         void *stream) {
-    return band_density<double>(ctx, d_rho, d_psi, occ, nb, ncomp, stream);
+    return band_density_impl<double>(ctx, d_rho, d_psi, occ, nb, ncomp, stream);
 }
 
 int qm_band_density_f32(qm_ctx *ctx, double *d_rho, const float *d_psi, const double *occ, int nb, int ncomp,
                         void *stream) {
-    return band_density<float>(ctx, d_rho, d_psi, occ, nb, ncomp, stream);
+    return band_density_impl<float>(ctx, d_rho, d_psi, occ, nb, ncomp, stream);
 }
 
 int qm_nonlocal_forces_f64(qm_ctx *ctx, const double *d_psi, const double *occ, int nb, int ncomp, int ik,

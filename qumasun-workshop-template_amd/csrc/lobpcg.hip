@@ -1,28 +1,794 @@
-// Block LOBPCG driver (placeholder until the solver lands).
+// Block LOBPCG on one GPU (replaces LOBPCG::Eigen_z_multi_mpi,
+// lobpcg_z_multi_mpi.h:1732-2852).
+//
+// Per iteration (all on the GPU, FP64 unless noted):
+//   A R               H.psi of the residual block (T = FP64 or FP32)
+//   G1 = [X R P]^T [R AR], and on recompute iterations G2 = [X P]^T [X P AX AP]
+//                     tall-skinny MFMA Gram products (T inputs, FP64 result)
+//   S, H              assembled from G1, G2 or the subspace recurrence, with R
+//                     normalised by the residual norms; complex blocks packed from
+//                     the real Gram of the SoA columns (AoSComplexFrom2by2Real,
+//                     matrix_2x2real_to_complex.h:8-56)
+//   H c = l S c       rocSOLVER sygvd / hegvd (the lapack_ZHEGVD(_emu) step,
+//                     lobpcg_z_multi_mpi.h:2254-2263, 2492-2511)
+//   recurrence        S' = Cb^H S Cb, H' = Cb^H H Cb with Cb = [C | C_P] (rocBLAS;
+//                     NextMatrix_z_blasmp, 1104-1436); P' normalised by its norm
+//   update            [X' P'] = [X R P] U, [AX' AP'] = [AX AR AP] U (MFMA, T)
+//   residual          R' = AX' - X' L with |R'|^2, |X'|^2 (FP64 sums); the
+//                     |1 - |x'|^2| > 1e-3 rollback of 2611-2666
+// Gram blocks are recomputed from the vectors at iteration 0 and every 6th
+// iteration (S_MAT_REST_LIMIT, 1808); R blocks every iteration.
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <utility>
+
 #include "qm_internal.h"
+
+namespace qm {
+namespace {
+
+constexpr int kRestLimit = 6;          // S_MAT_REST_LIMIT
+constexpr double kLimitNormX = 1.0e-3; // LIMIT_Z_NORM_X_FOR_RESET_MATRIX
+
+// ------------------------------------------------------------------ small FP64 matrices
+// Complex elements are interleaved (re, im) pairs, as rocblas_double_complex.
+
+struct Cplx {
+    double re, im;
+};
+
+template <bool CPLX>
+struct Sm;
+template <>
+struct Sm<false> {
+    typedef double t;
+    static __device__ __forceinline__ t zero() { return 0.0; }
+    static __device__ __forceinline__ t conj(t a) { return a; }
+    static __device__ __forceinline__ t scale(t a, double s) { return a * s; }
+    // packed element of a real Gram: band rows a, b -> G[r0 + a][c0 + b]
+    static __device__ __forceinline__ t pack(const double *G, int ldg, int r0, int c0, int a, int b) {
+        return G[(int64_t)(c0 + b) * ldg + r0 + a];
+    }
+    static __device__ __forceinline__ double realpart(t a) { return a; }
+};
+template <>
+struct Sm<true> {
+    typedef Cplx t;
+    static __device__ __forceinline__ t zero() { return {0.0, 0.0}; }
+    static __device__ __forceinline__ t conj(t a) { return {a.re, -a.im}; }
+    static __device__ __forceinline__ t scale(t a, double s) { return {a.re * s, a.im * s}; }
+    // (Z^H W)_{ab} from the real Gram of SoA columns (re = 2a, im = 2a+1)
+    static __device__ __forceinline__ t pack(const double *G, int ldg, int r0, int c0, int a, int b) {
+        auto g = [&](int i, int j) { return G[(int64_t)(c0 + j) * ldg + r0 + i]; };
+        return {g(2 * a, 2 * b) + g(2 * a + 1, 2 * b + 1), g(2 * a, 2 * b + 1) - g(2 * a + 1, 2 * b)};
+    }
+    static __device__ __forceinline__ double realpart(t a) { return a.re; }
+};
+
+// Blocks of the subspace matrices kept between iterations (N x N each, column-major):
+// 0 XX, 1 XP, 2 PP, 3 XAX, 4 XAP, 5 PAP
+enum { B_XX = 0, B_XP, B_PP, B_XAX, B_XAP, B_PAP, B_COUNT };
+
+// Extract X / P blocks from G2 = [X P]^T [X P AX AP] (real, m2 = c*N*(1+hp) rows).
+template <bool CPLX>
+__global__ void extract_blocks_kernel(typename Sm<CPLX>::t *blk, const double *G2, int N, int c, bool hp) {
+    using S = Sm<CPLX>;
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= N * N) return;
+    const int a = e % N, b = e / N;
+    const int nc = c * N;
+    const int m2 = hp ? 2 * nc : nc;
+    const int64_t nn = (int64_t)N * N;
+    // column blocks of G2: X at 0, P at nc (if hp), AX at nc*(1+hp), AP after it
+    const int cX = 0, cP = nc, cAX = hp ? 2 * nc : nc, cAP = 3 * nc;
+    blk[B_XX * nn + e] = S::pack(G2, m2, 0, cX, a, b);
+    blk[B_XAX * nn + e] = S::pack(G2, m2, 0, cAX, a, b);
+    if (hp) {
+        blk[B_XP * nn + e] = S::pack(G2, m2, 0, cP, a, b);
+        blk[B_PP * nn + e] = S::pack(G2, m2, nc, cP, a, b);
+        blk[B_XAP * nn + e] = S::pack(G2, m2, 0, cAP, a, b);
+        blk[B_PAP * nn + e] = S::pack(G2, m2, nc, cAP, a, b);
+    }
+}
+
+// S and H (n3 = nbk*N, full Hermitian, column-major) from the kept blocks and
+// G1 = [X R P]^T [R AR] (real, ld1 rows, columns 2*c*N).  R rows/columns are
+// scaled by dr = 1/|r|.
+template <bool CPLX>
+__global__ void assemble_kernel(typename Sm<CPLX>::t *Sm_, typename Sm<CPLX>::t *Hm_,
+                                const typename Sm<CPLX>::t *blk, const double *G1, const double *dr, int N, int c,
+                                int nbk, int ld1) {
+    using S = Sm<CPLX>;
+    using t = typename S::t;
+    const int n3 = nbk * N;
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (int64_t)n3 * n3) return;
+    const int I = (int)(e % n3), J = (int)(e / n3);
+    const int bi = I / N, bj = J / N;
+    const bool lower = bi > bj;
+    // evaluate the upper block (u, v) at (a, b); lower blocks are its conjugate transpose
+    const int u = lower ? bj : bi, v = lower ? bi : bj;
+    const int a = lower ? J % N : I % N, b = lower ? I % N : J % N;
+    const int nc = c * N;
+    const int64_t nn = (int64_t)N * N;
+    t s, h;
+    if (u == 0 && v == 0) {
+        s = blk[B_XX * nn + a + (int64_t)b * N];
+        h = blk[B_XAX * nn + a + (int64_t)b * N];
+    } else if (u == 0 && v == 2) {
+        s = blk[B_XP * nn + a + (int64_t)b * N];
+        h = blk[B_XAP * nn + a + (int64_t)b * N];
+    } else if (u == 2 && v == 2) {
+        s = blk[B_PP * nn + a + (int64_t)b * N];
+        h = blk[B_PAP * nn + a + (int64_t)b * N];
+    } else if (v == 1) {  // X R or R R: G1 rows of block u, columns R / AR
+        s = S::scale(S::pack(G1, ld1, u * nc, 0, a, b), dr[b] * (u == 1 ? dr[a] : 1.0));
+        h = S::scale(S::pack(G1, ld1, u * nc, nc, a, b), dr[b] * (u == 1 ? dr[a] : 1.0));
+    } else {  // (1, 2): R^H P = (P^H R)^H from G1 rows P
+        s = S::conj(S::scale(S::pack(G1, ld1, 2 * nc, 0, b, a), dr[a]));
+        h = S::conj(S::scale(S::pack(G1, ld1, 2 * nc, nc, b, a), dr[a]));
+    }
+    if (lower) {
+        s = S::conj(s);
+        h = S::conj(h);
+    }
+    Sm_[e] = s;
+    Hm_[e] = h;
+}
+
+// Cb = [C | C_P] (n3 x 2N): C = first N eigenvectors, C_P = C with the X rows zeroed.
+template <bool CPLX>
+__global__ void build_cb_kernel(typename Sm<CPLX>::t *Cb, const typename Sm<CPLX>::t *evec, int N, int n3) {
+    using S = Sm<CPLX>;
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (int64_t)n3 * 2 * N) return;
+    const int row = (int)(e % n3), col = (int)(e / n3);
+    const int src = col % N;
+    auto v = evec[(int64_t)src * n3 + row];
+    if (col >= N && row < N) v = S::zero();
+    Cb[e] = v;
+}
+
+// From SS = Cb^H S Cb and HH = Cb^H H Cb (2N x 2N): P' norms, the kept blocks of
+// the next iteration, and the real update matrix U (T) for the SoA columns:
+//   [X' P'] = [X R P] U, U = [C | C_P diag(dp)] with R rows scaled by dr.
+template <bool CPLX, typename T>
+__global__ void next_blocks_kernel(typename Sm<CPLX>::t *blk, double *dp, const typename Sm<CPLX>::t *SS,
+                                   const typename Sm<CPLX>::t *HH, int N) {
+    using S = Sm<CPLX>;
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= N * N) return;
+    const int a = e % N, b = e / N;
+    const int n2 = 2 * N;
+    auto ss = [&](int i, int j) { return SS[(int64_t)j * n2 + i]; };
+    auto hh = [&](int i, int j) { return HH[(int64_t)j * n2 + i]; };
+    auto inv = [&](int i) {
+        const double d = S::realpart(ss(N + i, N + i));
+        return d > 0.0 ? 1.0 / sqrt(d) : 1.0;
+    };
+    const double da = inv(a), db = inv(b);
+    const int64_t nn = (int64_t)N * N;
+    blk[B_XX * nn + e] = ss(a, b);
+    blk[B_XAX * nn + e] = hh(a, b);
+    blk[B_XP * nn + e] = S::scale(ss(a, N + b), db);
+    blk[B_XAP * nn + e] = S::scale(hh(a, N + b), db);
+    blk[B_PP * nn + e] = S::scale(ss(N + a, N + b), da * db);
+    blk[B_PAP * nn + e] = S::scale(hh(N + a, N + b), da * db);
+    if (a == 0) dp[b] = db;
+}
+
+template <bool CPLX, typename T>
+__global__ void build_update_kernel(T *U, const typename Sm<CPLX>::t *Cb, const double *dr, const double *dp, int N,
+                                    int n3) {
+    using S = Sm<CPLX>;
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (int64_t)n3 * 2 * N) return;
+    const int k = (int)(e % n3), n = (int)(e / n3);
+    auto u = Cb[e];
+    double sc = (k >= N && k < 2 * N) ? dr[k - N] : 1.0;
+    if (n >= N) sc *= dp[n - N];
+    u = S::scale(u, sc);
+    if constexpr (!CPLX) {
+        U[(int64_t)n * n3 + k] = (T)u;
+    } else {
+        const int ld = 2 * n3;
+        U[(int64_t)(2 * n) * ld + 2 * k] = (T)u.re;
+        U[(int64_t)(2 * n) * ld + 2 * k + 1] = (T)(-u.im);
+        U[(int64_t)(2 * n + 1) * ld + 2 * k] = (T)u.im;
+        U[(int64_t)(2 * n + 1) * ld + 2 * k + 1] = (T)u.re;
+    }
+}
+
+__global__ void inv_sqrt_kernel(double *out, const double *in, int n, double scale) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    if (i >= n) return;
+    const double v = in[i] * scale;
+    out[i] = v > 0.0 ? 1.0 / sqrt(v) : 1.0;
+}
+
+// min / max of the squared Cholesky pivots |U_ii|^2 (a cheap conditioning probe of S)
+template <bool CPLX>
+__global__ void pivot_ratio_kernel(double *out, const typename Sm<CPLX>::t *U, int n3) {
+    __shared__ double mn[256], mx[256];
+    double lo = 1.0e300, hi = 0.0;
+    for (int i = threadIdx.x; i < n3; i += 256) {
+        const double d = Sm<CPLX>::realpart(U[(int64_t)i * n3 + i]);
+        lo = fmin(lo, d * d);
+        hi = fmax(hi, d * d);
+    }
+    mn[threadIdx.x] = lo;
+    mx[threadIdx.x] = hi;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) {
+            mn[threadIdx.x] = fmin(mn[threadIdx.x], mn[threadIdx.x + st]);
+            mx[threadIdx.x] = fmax(mx[threadIdx.x], mx[threadIdx.x + st]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = mx[0] > 0.0 ? mn[0] / mx[0] : 0.0;
+}
+
+// SVQB fallback: columns of the S eigenvectors scaled by s^-1/2, near-null
+// directions (s <= tau * s_max) zeroed; w[j] records the scale (0 = dropped).
+template <bool CPLX>
+__global__ void svqb_basis_kernel(typename Sm<CPLX>::t *V, double *w, const double *sv, int n3, double tau) {
+    using S = Sm<CPLX>;
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (int64_t)n3 * n3) return;
+    const int j = (int)(e / n3);
+    const double smax = sv[n3 - 1];
+    const double wj = sv[j] > tau * smax ? 1.0 / sqrt(sv[j]) : 0.0;
+    V[e] = S::scale(V[e], wj);
+    if (e % n3 == 0) w[j] = wj;
+}
+
+// dropped directions of the reduced H get a huge diagonal so they sort last
+template <bool CPLX>
+__global__ void svqb_penalty_kernel(typename Sm<CPLX>::t *Hr, const double *w, int n3) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    if (i >= n3) return;
+    if (w[i] == 0.0) {
+        if constexpr (CPLX) Hr[(int64_t)i * n3 + i].re += 1.0e30;
+        else Hr[(int64_t)i * n3 + i] += 1.0e30;
+    }
+}
+
+}  // namespace
+}  // namespace qm
+
+struct qm_lobpcg {
+    qm_ctx *ctx = nullptr;
+    int N = 0, c = 1, prec = 64;
+    int64_t ng = 0;
+    size_t tsz = 8;
+    // vector blocks (Ng x c*N each, T)
+    void *X = nullptr, *R = nullptr, *P = nullptr, *AX = nullptr, *AR = nullptr, *AP = nullptr;
+    void *W[4] = {nullptr, nullptr, nullptr, nullptr};
+    bool have_p = false;  // P kept from an earlier call
+    // small FP64 buffers
+    double *G1 = nullptr, *G2 = nullptr;
+    void *S = nullptr, *H = nullptr, *Sw = nullptr, *Hw = nullptr, *blk = nullptr, *Cb = nullptr, *T1 = nullptr,
+         *SS = nullptr, *HH = nullptr;
+    void *U = nullptr;  // update matrix (T)
+    void *Ev = nullptr; // eigenvectors of the SVQB fallback (n3 x n3)
+    int fallbacks = 0;  // subspace-solve fallbacks (P dropped or SVQB) in the last solve
+    double cond64 = 1.0e-14, cond32 = 1.0e-4;  // minimum squared Cholesky pivot ratio of S
+    double *evals = nullptr, *ework = nullptr, *eps = nullptr, *dr = nullptr, *dp = nullptr, *rnorm = nullptr,
+           *xnorm = nullptr, *svals = nullptr, *svw = nullptr;
+    int *info = nullptr;
+    qm::DevBuf gram_part, red_part;
+    rocblas_handle blas = nullptr;
+    double t[6] = {0, 0, 0, 0, 0, 0};
+};
+
+namespace qm {
+namespace {
+
+void lfree(qm_lobpcg *s) {
+    void **ptrs[] = {&s->X, &s->R, &s->P, &s->AX, &s->AR, &s->AP, &s->W[0], &s->W[1], &s->W[2], &s->W[3],
+                     &s->S, &s->H, &s->Sw, &s->Hw, &s->blk, &s->Cb, &s->T1, &s->SS, &s->HH, &s->U, &s->Ev};
+    for (void **p : ptrs)
+        if (*p) {
+            (void)hipFree(*p);
+            *p = nullptr;
+        }
+    double **dptrs[] = {&s->G1, &s->G2, &s->evals, &s->ework, &s->eps, &s->dr, &s->dp, &s->rnorm, &s->xnorm,
+                        &s->svals, &s->svw};
+    for (double **p : dptrs)
+        if (*p) {
+            (void)hipFree(*p);
+            *p = nullptr;
+        }
+    if (s->info) (void)hipFree(s->info);
+    s->info = nullptr;
+    s->gram_part.release();
+    s->red_part.release();
+    if (s->blas) rocblas_destroy_handle(s->blas);
+    s->blas = nullptr;
+}
+
+int dalloc(void **p, size_t bytes) {
+    hipError_t e = hipMalloc(p, std::max<size_t>(bytes, 256));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        set_error("qm_lobpcg_create: hipMalloc of " + std::to_string(bytes) + " bytes failed");
+        return QM_ERR_NOMEM;
+    }
+    return QM_OK;
+}
+
+#define QM_BLAS(call)                                                                                       \
+    do {                                                                                                    \
+        rocblas_status s_ = (call);                                                                         \
+        if (s_ != rocblas_status_success) {                                                                 \
+            ::qm::set_error(std::string(#call) + " failed with rocblas status " + std::to_string((int)s_)); \
+            return QM_ERR_HIP;                                                                              \
+        }                                                                                                   \
+    } while (0)
+
+template <typename T>
+int hpsi(qm_lobpcg *s, T *out, const T *in, int ik, hipStream_t st) {
+    if constexpr (sizeof(T) == 8) return qm_hpsi_apply_f64(s->ctx, out, in, s->N, s->c, ik, st);
+    else return qm_hpsi_apply_f32(s->ctx, out, in, s->N, s->c, ik, st);
+}
+
+// H c = l S c as in lapack_ZHEGVD_emu (w_zhegvd.h:227-272): S = U^H U (potrf),
+// C = U^-H H U^-1 (two trsm), eigh(C) (syevd), c = U^-1 y (trsm, first N columns).
+// Works on copies Sw, Hw; eigenvalues in s->evals, eigenvectors in Hw.  After
+// potrf the squared pivot ratio is written to s->svals[0] for the caller's
+// conditioning check; on potrf failure s->info > 0 and the rest is skipped by
+// the caller.
+template <bool CPLX>
+int chol_factor(qm_lobpcg *s, int n3, hipStream_t st) {
+    using SmT = typename Sm<CPLX>::t;
+    QM_HIP(hipMemcpyAsync(s->Sw, s->S, sizeof(SmT) * n3 * n3, hipMemcpyDeviceToDevice, st));
+    QM_HIP(hipMemcpyAsync(s->Hw, s->H, sizeof(SmT) * n3 * n3, hipMemcpyDeviceToDevice, st));
+    if constexpr (CPLX)
+        QM_BLAS(rocsolver_zpotrf(s->blas, rocblas_fill_upper, n3, (rocblas_double_complex *)s->Sw, n3, s->info));
+    else
+        QM_BLAS(rocsolver_dpotrf(s->blas, rocblas_fill_upper, n3, (double *)s->Sw, n3, s->info));
+    hipLaunchKernelGGL((pivot_ratio_kernel<CPLX>), dim3(1), dim3(256), 0, st, s->svals, (const SmT *)s->Sw, n3);
+    QM_HIP(hipGetLastError());
+    return QM_OK;
+}
+
+template <bool CPLX>
+int chol_solve(qm_lobpcg *s, int n3, hipStream_t st) {
+    const int N = s->N;
+    if constexpr (CPLX) {
+        const rocblas_double_complex one{1.0, 0.0};
+        auto *U = (rocblas_double_complex *)s->Sw, *C = (rocblas_double_complex *)s->Hw;
+        QM_BLAS(rocblas_ztrsm(s->blas, rocblas_side_left, rocblas_fill_upper, rocblas_operation_conjugate_transpose,
+                              rocblas_diagonal_non_unit, n3, n3, &one, U, n3, C, n3));
+        QM_BLAS(rocblas_ztrsm(s->blas, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                              rocblas_diagonal_non_unit, n3, n3, &one, U, n3, C, n3));
+        QM_BLAS(rocsolver_zheevd(s->blas, rocblas_evect_original, rocblas_fill_upper, n3, C, n3, s->evals, s->ework,
+                                 s->info));
+        QM_BLAS(rocblas_ztrsm(s->blas, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
+                              rocblas_diagonal_non_unit, n3, N, &one, U, n3, C, n3));
+    } else {
+        const double one = 1.0;
+        auto *U = (double *)s->Sw, *C = (double *)s->Hw;
+        QM_BLAS(rocblas_dtrsm(s->blas, rocblas_side_left, rocblas_fill_upper, rocblas_operation_transpose,
+                              rocblas_diagonal_non_unit, n3, n3, &one, U, n3, C, n3));
+        QM_BLAS(rocblas_dtrsm(s->blas, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                              rocblas_diagonal_non_unit, n3, n3, &one, U, n3, C, n3));
+        QM_BLAS(rocsolver_dsyevd(s->blas, rocblas_evect_original, rocblas_fill_upper, n3, C, n3, s->evals, s->ework,
+                                 s->info));
+        QM_BLAS(rocblas_dtrsm(s->blas, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
+                              rocblas_diagonal_non_unit, n3, N, &one, U, n3, C, n3));
+    }
+    return QM_OK;
+}
+
+// Fallback when S is not numerically positive definite: B = V s^-1/2 over the
+// kept S eigendirections, eigh(B^H H B), eigenvectors C = B y into s->Ev.
+template <bool CPLX>
+int svqb_eig(qm_lobpcg *s, int n3, double tau, hipStream_t st) {
+    using SmT = typename Sm<CPLX>::t;
+    QM_HIP(hipMemcpyAsync(s->Sw, s->S, sizeof(SmT) * n3 * n3, hipMemcpyDeviceToDevice, st));
+    if constexpr (CPLX) {
+        QM_BLAS(rocsolver_zheevd(s->blas, rocblas_evect_original, rocblas_fill_upper, n3,
+                                 (rocblas_double_complex *)s->Sw, n3, s->svals, s->ework, s->info));
+    } else {
+        QM_BLAS(rocsolver_dsyevd(s->blas, rocblas_evect_original, rocblas_fill_upper, n3, (double *)s->Sw, n3,
+                                 s->svals, s->ework, s->info));
+    }
+    hipLaunchKernelGGL((svqb_basis_kernel<CPLX>), dim3((unsigned)(((int64_t)n3 * n3 + 255) / 256)), dim3(256), 0, st,
+                       (SmT *)s->Sw, s->svw, s->svals, n3, tau);
+    QM_HIP(hipGetLastError());
+    if constexpr (CPLX) {
+        const rocblas_double_complex one{1.0, 0.0}, zero{0.0, 0.0};
+        auto *B = (rocblas_double_complex *)s->Sw, *T1 = (rocblas_double_complex *)s->T1,
+             *Hr = (rocblas_double_complex *)s->Hw;
+        QM_BLAS(rocblas_zgemm(s->blas, rocblas_operation_none, rocblas_operation_none, n3, n3, n3, &one,
+                              (rocblas_double_complex *)s->H, n3, B, n3, &zero, T1, n3));
+        QM_BLAS(rocblas_zgemm(s->blas, rocblas_operation_conjugate_transpose, rocblas_operation_none, n3, n3, n3,
+                              &one, B, n3, T1, n3, &zero, Hr, n3));
+        hipLaunchKernelGGL((svqb_penalty_kernel<CPLX>), dim3((n3 + 63) / 64), dim3(64), 0, st, (SmT *)s->Hw, s->svw,
+                           n3);
+        QM_HIP(hipGetLastError());
+        QM_BLAS(rocsolver_zheevd(s->blas, rocblas_evect_original, rocblas_fill_upper, n3, Hr, n3, s->evals,
+                                 s->ework, s->info));
+        QM_BLAS(rocblas_zgemm(s->blas, rocblas_operation_none, rocblas_operation_none, n3, n3, n3, &one, B, n3, Hr,
+                              n3, &zero, (rocblas_double_complex *)s->Ev, n3));
+    } else {
+        const double one = 1.0, zero = 0.0;
+        auto *B = (double *)s->Sw, *T1 = (double *)s->T1, *Hr = (double *)s->Hw;
+        QM_BLAS(rocblas_dgemm(s->blas, rocblas_operation_none, rocblas_operation_none, n3, n3, n3, &one,
+                              (double *)s->H, n3, B, n3, &zero, T1, n3));
+        QM_BLAS(rocblas_dgemm(s->blas, rocblas_operation_transpose, rocblas_operation_none, n3, n3, n3, &one, B, n3,
+                              T1, n3, &zero, Hr, n3));
+        hipLaunchKernelGGL((svqb_penalty_kernel<CPLX>), dim3((n3 + 63) / 64), dim3(64), 0, st, (SmT *)s->Hw, s->svw,
+                           n3);
+        QM_HIP(hipGetLastError());
+        QM_BLAS(rocsolver_dsyevd(s->blas, rocblas_evect_original, rocblas_fill_upper, n3, Hr, n3, s->evals, s->ework,
+                                 s->info));
+        QM_BLAS(rocblas_dgemm(s->blas, rocblas_operation_none, rocblas_operation_none, n3, n3, n3, &one, B, n3, Hr,
+                              n3, &zero, (double *)s->Ev, n3));
+    }
+    return QM_OK;
+}
+
+// SS = Cb^H M Cb for M = S, H (n3 x n3), Cb n3 x 2N.
+template <bool CPLX>
+int recurrence(qm_lobpcg *s, int n3) {
+    const int n2 = 2 * s->N;
+    if constexpr (CPLX) {
+        const rocblas_double_complex one{1.0, 0.0}, zero{0.0, 0.0};
+        auto *Cb = (rocblas_double_complex *)s->Cb, *T1 = (rocblas_double_complex *)s->T1;
+        QM_BLAS(rocblas_zgemm(s->blas, rocblas_operation_none, rocblas_operation_none, n3, n2, n3, &one,
+                              (rocblas_double_complex *)s->S, n3, Cb, n3, &zero, T1, n3));
+        QM_BLAS(rocblas_zgemm(s->blas, rocblas_operation_conjugate_transpose, rocblas_operation_none, n2, n2, n3,
+                              &one, Cb, n3, T1, n3, &zero, (rocblas_double_complex *)s->SS, n2));
+        QM_BLAS(rocblas_zgemm(s->blas, rocblas_operation_none, rocblas_operation_none, n3, n2, n3, &one,
+                              (rocblas_double_complex *)s->H, n3, Cb, n3, &zero, T1, n3));
+        QM_BLAS(rocblas_zgemm(s->blas, rocblas_operation_conjugate_transpose, rocblas_operation_none, n2, n2, n3,
+                              &one, Cb, n3, T1, n3, &zero, (rocblas_double_complex *)s->HH, n2));
+    } else {
+        const double one = 1.0, zero = 0.0;
+        auto *Cb = (double *)s->Cb, *T1 = (double *)s->T1;
+        QM_BLAS(rocblas_dgemm(s->blas, rocblas_operation_none, rocblas_operation_none, n3, n2, n3, &one,
+                              (double *)s->S, n3, Cb, n3, &zero, T1, n3));
+        QM_BLAS(rocblas_dgemm(s->blas, rocblas_operation_transpose, rocblas_operation_none, n2, n2, n3, &one, Cb, n3,
+                              T1, n3, &zero, (double *)s->SS, n2));
+        QM_BLAS(rocblas_dgemm(s->blas, rocblas_operation_none, rocblas_operation_none, n3, n2, n3, &one,
+                              (double *)s->H, n3, Cb, n3, &zero, T1, n3));
+        QM_BLAS(rocblas_dgemm(s->blas, rocblas_operation_transpose, rocblas_operation_none, n2, n2, n3, &one, Cb, n3,
+                              T1, n3, &zero, (double *)s->HH, n2));
+    }
+    return QM_OK;
+}
+
+inline unsigned nblk(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+template <typename T, bool CPLX>
+int solve_impl(qm_lobpcg *s, T *x_user, double *eigenvalues, double *residual_norms, int iter_max, int ik,
+               int first_time, int *iters_done, hipStream_t st) {
+    using SmT = typename Sm<CPLX>::t;
+    qm_ctx *ctx = s->ctx;
+    const int N = s->N, c = s->c, nc = c * N;
+    const int64_t ng = s->ng;
+    const double dV = ctx->hx * ctx->hy * ctx->hz;
+    QM_BLAS(rocblas_set_stream(s->blas, st));
+    for (double &v : s->t) v = 0.0;
+    s->fallbacks = 0;
+    const double t_start = now_ms();
+    auto sync_ms = [&](double &acc, double t0) -> int {
+        QM_HIP(hipStreamSynchronize(st));
+        acc += now_ms() - t0;
+        return QM_OK;
+    };
+    double t0 = now_ms();
+
+    T *X = (T *)s->X, *R = (T *)s->R, *P = (T *)s->P, *AX = (T *)s->AX, *AR = (T *)s->AR, *AP = (T *)s->AP;
+    T *W[4] = {(T *)s->W[0], (T *)s->W[1], (T *)s->W[2], (T *)s->W[3]};
+
+    // x := sqrt(dV) x (lobpcg_z_multi_mpi.h:1851); A x; eps = Re x^H A x; r = Ax - eps x
+    QM_TRY(scale_bands<T>(X, x_user, nullptr, std::sqrt(dV), N, c, ng, st));
+    QM_TRY(sync_ms(s->t[4], t0));
+    t0 = now_ms();
+    QM_TRY(hpsi<T>(s, AX, X, ik, st));
+    bool hp = false;
+    if (!first_time && s->have_p) {
+        QM_TRY(hpsi<T>(s, AP, P, ik, st));
+        hp = true;
+    }
+    QM_TRY(sync_ms(s->t[0], t0));
+    t0 = now_ms();
+    QM_TRY(band_reduce<T>(s->red_part, 0, s->eps, nullptr, nullptr, X, AX, nullptr, N, c, ng, st));
+    QM_TRY(band_reduce<T>(s->red_part, 1, s->rnorm, s->xnorm, R, X, AX, s->eps, N, c, ng, st));
+    hipLaunchKernelGGL(inv_sqrt_kernel, dim3(nblk(N, 64)), dim3(64), 0, st, s->dr, s->rnorm, N, 1.0);
+    QM_HIP(hipGetLastError());
+
+    std::vector<double> h_norm(N), h_eps(N);
+    QM_HIP(hipMemcpyAsync(h_eps.data(), s->eps, sizeof(double) * N, hipMemcpyDeviceToHost, st));
+    QM_TRY(sync_ms(s->t[4], t0));
+
+    int it_done = 0;
+    bool rolled_back = false;
+    for (int it = 0; it < iter_max; ++it) {
+        it_done = it + 1;
+        const int nbk = hp ? 3 : 2;
+        // A R
+        t0 = now_ms();
+        QM_TRY(hpsi<T>(s, AR, R, ik, st));
+        QM_TRY(sync_ms(s->t[0], t0));
+        // Gram products
+        t0 = now_ms();
+        {
+            ColBlocks A, B;
+            A.add(X, nc);
+            A.add(R, nc);
+            if (hp) A.add(P, nc);
+            B.add(R, nc);
+            B.add(AR, nc);
+            QM_TRY(gram<T>(s->gram_part, s->G1, A, B, A.total(), B.total(), ng, st));
+            if (it % kRestLimit == 0) {
+                ColBlocks A2, B2;
+                A2.add(X, nc);
+                B2.add(X, nc);
+                if (hp) {
+                    A2.add(P, nc);
+                    B2.add(P, nc);
+                }
+                B2.add(AX, nc);
+                if (hp) B2.add(AP, nc);
+                QM_TRY(gram<T>(s->gram_part, s->G2, A2, B2, A2.total(), B2.total(), ng, st));
+                hipLaunchKernelGGL((extract_blocks_kernel<CPLX>), dim3(nblk((int64_t)N * N)), dim3(256), 0, st,
+                                   (SmT *)s->blk, s->G2, N, c, hp);
+                QM_HIP(hipGetLastError());
+            }
+        }
+        QM_TRY(sync_ms(s->t[1], t0));
+        // subspace eigenproblem: Cholesky route; if S is not numerically positive
+        // definite or its pivots span more than cond_limit, retry without P
+        // ([X R], as on the first iteration), then fall back to the S eigenbasis
+        t0 = now_ms();
+        int nbk_used = nbk;
+        const void *evec = s->Hw;
+        const double cond_limit = sizeof(T) == 8 ? s->cond64 : s->cond32;
+        for (int attempt = 0;; ++attempt) {
+            const int n3a = nbk_used * N;
+            hipLaunchKernelGGL((assemble_kernel<CPLX>), dim3(nblk((int64_t)n3a * n3a)), dim3(256), 0, st,
+                               (SmT *)s->S, (SmT *)s->H, (const SmT *)s->blk, s->G1, s->dr, N, c, nbk_used,
+                               nbk * nc);
+            QM_HIP(hipGetLastError());
+            QM_TRY(chol_factor<CPLX>(s, n3a, st));
+            int h_info = 0;
+            double h_ratio = 0.0;
+            QM_HIP(hipMemcpyAsync(&h_info, s->info, sizeof(int), hipMemcpyDeviceToHost, st));
+            QM_HIP(hipMemcpyAsync(&h_ratio, s->svals, sizeof(double), hipMemcpyDeviceToHost, st));
+            QM_HIP(hipStreamSynchronize(st));
+            if (h_info == 0 && h_ratio >= cond_limit) {
+                QM_TRY(chol_solve<CPLX>(s, n3a, st));
+                QM_HIP(hipMemcpyAsync(&h_info, s->info, sizeof(int), hipMemcpyDeviceToHost, st));
+                QM_HIP(hipStreamSynchronize(st));
+                QM_CHECK(h_info == 0, QM_ERR_NUMERIC,
+                         "qm_lobpcg_solve: subspace eigensolve failed (rocSOLVER info " + std::to_string(h_info) + ")");
+                evec = s->Hw;
+                break;
+            }
+            if (nbk_used == 3) {  // drop P and retry
+                nbk_used = 2;
+                ++s->fallbacks;
+                continue;
+            }
+            QM_TRY(svqb_eig<CPLX>(s, n3a, sizeof(T) == 8 ? 1.0e-12 : cond_limit, st));
+            QM_HIP(hipMemcpyAsync(&h_info, s->info, sizeof(int), hipMemcpyDeviceToHost, st));
+            QM_HIP(hipStreamSynchronize(st));
+            QM_CHECK(h_info == 0, QM_ERR_NUMERIC,
+                     "qm_lobpcg_solve: subspace eigensolve failed (rocSOLVER info " + std::to_string(h_info) + ")");
+            evec = s->Ev;
+            ++s->fallbacks;
+            break;
+        }
+        const int n3 = nbk_used * N;
+        const bool hp_used = nbk_used == 3;
+        QM_TRY(sync_ms(s->t[2], t0));
+        // recurrence, update matrix, vector updates
+        t0 = now_ms();
+        hipLaunchKernelGGL((build_cb_kernel<CPLX>), dim3(nblk((int64_t)n3 * 2 * N)), dim3(256), 0, st, (SmT *)s->Cb,
+                           (const SmT *)evec, N, n3);
+        QM_HIP(hipGetLastError());
+        QM_TRY(recurrence<CPLX>(s, n3));
+        hipLaunchKernelGGL((next_blocks_kernel<CPLX, T>), dim3(nblk((int64_t)N * N)), dim3(256), 0, st, (SmT *)s->blk,
+                           s->dp, (const SmT *)s->SS, (const SmT *)s->HH, N);
+        QM_HIP(hipGetLastError());
+        hipLaunchKernelGGL((build_update_kernel<CPLX, T>), dim3(nblk((int64_t)n3 * 2 * N)), dim3(256), 0, st,
+                           (T *)s->U, (const SmT *)s->Cb, s->dr, s->dp, N, n3);
+        QM_HIP(hipGetLastError());
+        {
+            ColBlocks Z, Y, AZ, AY;
+            Z.add(X, nc);
+            Z.add(R, nc);
+            if (hp_used) Z.add(P, nc);
+            AZ.add(AX, nc);
+            AZ.add(AR, nc);
+            if (hp_used) AZ.add(AP, nc);
+            Y.add(W[0], nc);
+            Y.add(W[1], nc);
+            AY.add(W[2], nc);
+            AY.add(W[3], nc);
+            QM_TRY(update<T>(Z, Y, (const T *)s->U, c * n3, 2 * nc, ng, st));
+            QM_TRY(update<T>(AZ, AY, (const T *)s->U, c * n3, 2 * nc, ng, st));
+        }
+        // R' = AX' - X' L, |R'|^2, |X'|^2
+        QM_TRY(band_reduce<T>(s->red_part, 1, s->rnorm, s->xnorm, AR, W[0], W[2], s->evals, N, c, ng, st));
+        QM_HIP(hipMemcpyAsync(h_norm.data(), s->xnorm, sizeof(double) * N, hipMemcpyDeviceToHost, st));
+        QM_TRY(sync_ms(s->t[3], t0));
+        double maxdev = 0.0;
+        for (int n = 0; n < N; ++n) maxdev = std::max(maxdev, std::fabs(1.0 - h_norm[n]));
+        if (maxdev > kLimitNormX) {
+            // rollback and escape (lobpcg_z_multi_mpi.h:2611-2666): P = R and leave the
+            // loop.  The reference rescales the old x by the norms of the rejected
+            // update (2639); the old x is renormalised with its own norms here.
+            t0 = now_ms();
+            QM_HIP(hipMemcpyAsync(P, R, s->tsz * (size_t)nc * ng, hipMemcpyDeviceToDevice, st));
+            s->have_p = true;
+            QM_TRY(band_reduce<T>(s->red_part, 2, s->xnorm, nullptr, nullptr, X, nullptr, nullptr, N, c, ng, st));
+            hipLaunchKernelGGL(inv_sqrt_kernel, dim3(nblk(N, 64)), dim3(64), 0, st, s->dp, s->xnorm, N, dV);
+            QM_HIP(hipGetLastError());
+            QM_TRY(scale_bands<T>(x_user, X, s->dp, 0.0, N, c, ng, st));
+            QM_TRY(sync_ms(s->t[4], t0));
+            rolled_back = true;
+            break;
+        }
+        // the residual block written by band_reduce sits in AR (free after the update); rotate buffers
+        std::swap(X, W[0]);
+        std::swap(P, W[1]);
+        std::swap(AX, W[2]);
+        std::swap(AP, W[3]);
+        std::swap(R, AR);
+        hp = true;
+        QM_HIP(hipMemcpyAsync(h_eps.data(), s->evals, sizeof(double) * N, hipMemcpyDeviceToHost, st));
+        hipLaunchKernelGGL(inv_sqrt_kernel, dim3(nblk(N, 64)), dim3(64), 0, st, s->dr, s->rnorm, N, 1.0);
+        QM_HIP(hipGetLastError());
+    }
+    t0 = now_ms();
+    if (!rolled_back) {
+        // x_out = x / sqrt(|x|^2 dV)  (2796-2800)
+        if (iter_max > 0) {
+            hipLaunchKernelGGL(inv_sqrt_kernel, dim3(nblk(N, 64)), dim3(64), 0, st, s->dp, s->xnorm, N, dV);
+            QM_HIP(hipGetLastError());
+            QM_TRY(scale_bands<T>(x_user, X, s->dp, 0.0, N, c, ng, st));
+        } else {
+            QM_TRY(scale_bands<T>(x_user, X, nullptr, 1.0 / std::sqrt(dV), N, c, ng, st));
+        }
+        s->have_p = hp;
+    }
+    // keep the rotated buffers
+    s->X = X;
+    s->R = R;
+    s->P = P;
+    s->AX = AX;
+    s->AR = AR;
+    s->AP = AP;
+    for (int i = 0; i < 4; ++i) s->W[i] = W[i];
+    std::vector<double> h_rn(N);
+    QM_HIP(hipMemcpyAsync(h_rn.data(), s->rnorm, sizeof(double) * N, hipMemcpyDeviceToHost, st));
+    QM_TRY(sync_ms(s->t[4], t0));
+    for (int n = 0; n < N; ++n) {
+        eigenvalues[n] = h_eps[n];
+        if (residual_norms) residual_norms[n] = h_rn[n];
+    }
+    if (iters_done) *iters_done = it_done;
+    s->t[5] = now_ms() - t_start;
+    return QM_OK;
+}
+
+}  // namespace
+}  // namespace qm
+
+using namespace qm;
 
 extern "C" {
 
 int qm_lobpcg_create(qm_lobpcg **out, qm_ctx *ctx, int nbands, int ncomp, int precision) {
-    (void)ctx; (void)nbands; (void)ncomp; (void)precision;
-    if (out) *out = nullptr;
-    qm::set_error("qm_lobpcg_create: LOBPCG solver not available in this build");
-    return QM_ERR_STATE;
+    QM_CHECK(out && ctx, QM_ERR_ARG, "qm_lobpcg_create: NULL argument");
+    *out = nullptr;
+    QM_CHECK(nbands >= 1 && nbands <= 2048, QM_ERR_ARG, "qm_lobpcg_create: nbands must be in [1, 2048]");
+    QM_CHECK(ncomp == 1 || ncomp == 2, QM_ERR_ARG, "qm_lobpcg_create: ncomp must be 1 or 2");
+    QM_CHECK(precision == 32 || precision == 64, QM_ERR_ARG, "qm_lobpcg_create: precision must be 32 or 64");
+    QM_TRY(activate(ctx));
+    qm_lobpcg *s = new qm_lobpcg();
+    s->ctx = ctx;
+    s->N = nbands;
+    s->c = ncomp;
+    s->prec = precision;
+    s->ng = ctx->ng;
+    s->tsz = precision == 64 ? 8 : 4;
+    const int N = nbands, nc = ncomp * N, n3 = 3 * N;
+    const size_t blk = s->tsz * (size_t)nc * s->ng;
+    const size_t esz = ncomp == 2 ? 16 : 8;  // small-matrix element
+    int st = QM_OK;
+    void **vec[] = {&s->X, &s->R, &s->P, &s->AX, &s->AR, &s->AP, &s->W[0], &s->W[1], &s->W[2], &s->W[3]};
+    for (void **p : vec)
+        if (st == QM_OK) st = dalloc(p, blk);
+    if (st == QM_OK) st = dalloc((void **)&s->G1, sizeof(double) * (size_t)(3 * nc) * (2 * nc));
+    if (st == QM_OK) st = dalloc((void **)&s->G2, sizeof(double) * (size_t)(2 * nc) * (4 * nc));
+    void **sm[] = {&s->S, &s->H, &s->Sw, &s->Hw};
+    for (void **p : sm)
+        if (st == QM_OK) st = dalloc(p, esz * (size_t)n3 * n3);
+    if (st == QM_OK) st = dalloc(&s->blk, esz * (size_t)B_COUNT * N * N);
+    if (st == QM_OK) st = dalloc(&s->Cb, esz * (size_t)n3 * 2 * N);
+    if (st == QM_OK) st = dalloc(&s->T1, esz * (size_t)n3 * n3);
+    if (st == QM_OK) st = dalloc(&s->Ev, esz * (size_t)n3 * n3);
+    if (st == QM_OK) st = dalloc(&s->SS, esz * (size_t)4 * N * N);
+    if (st == QM_OK) st = dalloc(&s->HH, esz * (size_t)4 * N * N);
+    if (st == QM_OK) st = dalloc(&s->U, s->tsz * (size_t)(ncomp * n3) * (2 * nc));
+    double **dv[] = {&s->evals, &s->ework, &s->eps, &s->dr, &s->dp, &s->rnorm, &s->xnorm, &s->svals, &s->svw};
+    for (double **p : dv)
+        if (st == QM_OK) st = dalloc((void **)p, sizeof(double) * n3);
+    if (st == QM_OK) st = dalloc((void **)&s->info, sizeof(int));
+    if (st == QM_OK && rocblas_create_handle(&s->blas) != rocblas_status_success) {
+        set_error("qm_lobpcg_create: rocblas_create_handle failed");
+        st = QM_ERR_HIP;
+    }
+    if (st != QM_OK) {
+        lfree(s);
+        delete s;
+        return st;
+    }
+    *out = s;
+    return QM_OK;
 }
 
-void qm_lobpcg_destroy(qm_lobpcg *s) { (void)s; }
+void qm_lobpcg_destroy(qm_lobpcg *s) {
+    if (!s) return;
+    (void)hipSetDevice(s->ctx->device);
+    (void)hipDeviceSynchronize();
+    lfree(s);
+    delete s;
+}
 
 int qm_lobpcg_solve(qm_lobpcg *s, void *d_x, double *eigenvalues, double *residual_norms, int iter_max, int ik,
                     int first_time, int *iters_done, void *stream) {
-    (void)s; (void)d_x; (void)eigenvalues; (void)residual_norms; (void)iter_max; (void)ik; (void)first_time;
-    (void)iters_done; (void)stream;
-    qm::set_error("qm_lobpcg_solve: LOBPCG solver not available in this build");
-    return QM_ERR_STATE;
+    QM_CHECK(s && d_x && eigenvalues, QM_ERR_ARG, "qm_lobpcg_solve: NULL argument");
+    QM_CHECK(iter_max >= 0, QM_ERR_ARG, "qm_lobpcg_solve: iter_max < 0");
+    QM_CHECK(ik >= 0 && ik < (int)s->ctx->kpts.size(), QM_ERR_ARG, "qm_lobpcg_solve: k index out of range");
+    QM_CHECK(s->c == 2 || s->ctx->kpts[ik].gamma, QM_ERR_ARG, "qm_lobpcg_solve: real bands need a Gamma k-point");
+    QM_TRY(activate(s->ctx));
+    hipStream_t st = (hipStream_t)stream;
+    if (s->prec == 64) {
+        if (s->c == 2)
+            return solve_impl<double, true>(s, (double *)d_x, eigenvalues, residual_norms, iter_max, ik, first_time,
+                                            iters_done, st);
+        return solve_impl<double, false>(s, (double *)d_x, eigenvalues, residual_norms, iter_max, ik, first_time,
+                                         iters_done, st);
+    }
+    if (s->c == 2)
+        return solve_impl<float, true>(s, (float *)d_x, eigenvalues, residual_norms, iter_max, ik, first_time,
+                                       iters_done, st);
+    return solve_impl<float, false>(s, (float *)d_x, eigenvalues, residual_norms, iter_max, ik, first_time,
+                                    iters_done, st);
 }
 
+int qm_lobpcg_set_conditioning(qm_lobpcg *s, double min_pivot_ratio_fp64, double min_pivot_ratio_fp32) {
+    QM_CHECK(s, QM_ERR_ARG, "qm_lobpcg_set_conditioning: NULL solver");
+    QM_CHECK(min_pivot_ratio_fp64 >= 0.0 && min_pivot_ratio_fp32 >= 0.0 && min_pivot_ratio_fp64 < 1.0 &&
+                 min_pivot_ratio_fp32 < 1.0,
+             QM_ERR_ARG, "qm_lobpcg_set_conditioning: ratios must be in [0, 1)");
+    s->cond64 = min_pivot_ratio_fp64;
+    s->cond32 = min_pivot_ratio_fp32;
+    return QM_OK;
+}
+
+int qm_lobpcg_fallbacks(qm_lobpcg *s) { return s ? s->fallbacks : QM_ERR_ARG; }
+
 int qm_lobpcg_timings(qm_lobpcg *s, double *t6) {
-    (void)s; (void)t6;
-    return QM_ERR_STATE;
+    QM_CHECK(s && t6, QM_ERR_ARG, "qm_lobpcg_timings: NULL argument");
+    for (int i = 0; i < 6; ++i) t6[i] = s->t[i];
+    return QM_OK;
 }
 
 }  // extern "C"

@@ -103,6 +103,7 @@ struct qm_ctx {
     std::vector<qm::KPoint> kpts;
     qm::DevBuf coef;     // projector coefficients for the nonlocal apply
     qm::DevBuf fwork;    // force partials
+    qm::DevBuf gemm_part; // Gram partial sums of qm_gram_*
 };
 
 namespace qm {
@@ -124,6 +125,35 @@ int nonlocal_forces_f64(qm_ctx *ctx, const double *psi, const double *occ, int n
                         const KPoint &kp, double *forces, double *e_nl, hipStream_t st);
 int nonlocal_forces_f32(qm_ctx *ctx, const float *psi, const double *occ, int nb, int ncomp,
                         const KPoint &kp, double *forces, double *e_nl, hipStream_t st);
+
+// blas.hip: logical matrix made of up to kMaxColBlocks column blocks, each with
+// leading dimension Ng (column j of block b at ptr[b] + j*Ng)
+constexpr int kMaxColBlocks = 6;
+struct ColBlocks {
+    const void *ptr[kMaxColBlocks] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    int ncols[kMaxColBlocks] = {0, 0, 0, 0, 0, 0};
+    int nblk = 0;
+    void add(const void *p, int n) {
+        ptr[nblk] = p;
+        ncols[nblk] = n;
+        ++nblk;
+    }
+    int total() const {
+        int t = 0;
+        for (int b = 0; b < nblk; ++b) t += ncols[b];
+        return t;
+    }
+};
+template <typename T>
+int gram(DevBuf &partial, double *C, const ColBlocks &A, const ColBlocks &B, int m, int n, int64_t ng,
+         hipStream_t st);
+template <typename T>
+int update(const ColBlocks &Z, const ColBlocks &Y, const T *Cm, int m, int n, int64_t ng, hipStream_t st);
+template <typename T>
+int band_reduce(DevBuf &part, int mode, double *out0, double *out1, T *r, const T *x, const T *ax, const double *lam,
+                int nb, int ncomp, int64_t ng, hipStream_t st);
+template <typename T>
+int scale_bands(T *y, const T *x, const double *f, double fconst, int nb, int ncomp, int64_t ng, hipStream_t st);
 
 // adjoint.hip
 int launch_band_density_f64(const qm_ctx *ctx, double *rho, const double *psi, const double *d_occ,

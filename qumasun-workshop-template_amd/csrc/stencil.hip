@@ -32,7 +32,7 @@ constexpr int LY = TY + 2 * HW;
 
 template <typename T>
 struct FD8Coef {
-    T c0;        // 14350/5040 * 1/2 * sum_d 1/h_d^2 + |k|^2/2
+    T c0;        // FP64: 14350/5040 * 1/2 * sum_d 1/h_d^2 + |k|^2/2; FP32: |k|^2/2
     T cx[4], cy[4], cz[4];   // -1/2 * {8064, -1008, 128, -9}/5040 / h_d^2
     T ax[4], ay[4], az[4];   // {672, -168, 32, -3}/840 * k_d / h_d
 };
@@ -136,12 +136,28 @@ __global__ __launch_bounds__(TX * BY) void fd8_local_kernel(T *__restrict__ out,
 #pragma unroll
             for (int cc = 0; cc < NC; ++cc) {
                 const T p = q[cc][r][4];
-                T lap = c.c0 * p;
+                T lap;
+                if constexpr (sizeof(T) == 8) {
+                    // reference grouping: c0 p + sum_j c_j (p[+j] + p[-j])
+                    lap = c.c0 * p;
 #pragma unroll
-                for (int j = 1; j <= 4; ++j) {
-                    lap += c.cx[j - 1] * (sm[cc][ly][lx + j] + sm[cc][ly][lx - j]);
-                    lap += c.cy[j - 1] * (sm[cc][ly + j][lx] + sm[cc][ly - j][lx]);
-                    lap += c.cz[j - 1] * (q[cc][r][4 + j] + q[cc][r][4 - j]);
+                    for (int j = 1; j <= 4; ++j) {
+                        lap += c.cx[j - 1] * (sm[cc][ly][lx + j] + sm[cc][ly][lx - j]);
+                        lap += c.cy[j - 1] * (sm[cc][ly + j][lx] + sm[cc][ly - j][lx]);
+                        lap += c.cz[j - 1] * (q[cc][r][4 + j] + q[cc][r][4 - j]);
+                    }
+                } else {
+                    // FP32: difference form sum_j c_j ((p[+j] - p) + (p[-j] - p)).  The
+                    // kinetic diagonal (~14350/5040 / h^2 ~ 100 Ha) never appears, so
+                    // rounding is relative to the kinetic energy and constants are
+                    // annihilated exactly whatever the rounding of the taps.
+                    lap = c.c0 * p;  // |k|^2 / 2 only in this form
+#pragma unroll
+                    for (int j = 1; j <= 4; ++j) {
+                        lap += c.cx[j - 1] * ((sm[cc][ly][lx + j] - p) + (sm[cc][ly][lx - j] - p));
+                        lap += c.cy[j - 1] * ((sm[cc][ly + j][lx] - p) + (sm[cc][ly - j][lx] - p));
+                        lap += c.cz[j - 1] * ((q[cc][r][4 + j] - p) + (q[cc][r][4 - j] - p));
+                    }
                 }
                 res[cc] = lap + vv[r] * p;
             }
@@ -181,7 +197,11 @@ FD8Coef<T> make_coef(const qm_ctx *ctx, const double k[3]) {
     const double aj[4] = {672.0, -168.0, 32.0, -3.0};
     FD8Coef<T> c;
     const double k2 = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
-    c.c0 = T(14350.0 / 5040.0 * 0.5 * (1.0 / (h[0] * h[0]) + 1.0 / (h[1] * h[1]) + 1.0 / (h[2] * h[2])) + 0.5 * k2);
+    // FP64 keeps the reference's diagonal; the FP32 difference form folds the
+    // -2 sum_j c_j part of it into the taps and keeps only |k|^2 / 2
+    c.c0 = sizeof(T) == 8 ? T(14350.0 / 5040.0 * 0.5 * (1.0 / (h[0] * h[0]) + 1.0 / (h[1] * h[1]) + 1.0 / (h[2] * h[2])) +
+                             0.5 * k2)
+                          : T(0.5 * k2);
     T *cs[3] = {c.cx, c.cy, c.cz};
     T *as[3] = {c.ax, c.ay, c.az};
     for (int d = 0; d < 3; ++d)

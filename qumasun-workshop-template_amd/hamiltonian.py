@@ -69,14 +69,17 @@ class Hamiltonian:
         _lib.check(self.lib.qm_set_kpoints(self.ctx, k.shape[0], _ptr(k)))
 
     # -- operator ------------------------------------------------------------
-    def apply(self, hpsi, psi, nb=None, k_index=0, ncomp=None):
-        """hpsi <- H(k) psi on the current torch stream (hpsi overwritten)."""
+    def apply(self, hpsi, psi, nb=None, k_index=0, ncomp=None, part="full"):
+        """hpsi <- H(k) psi on the current torch stream (hpsi overwritten).
+        part="local": kinetic + V_loc only (overwrites); part="nonlocal": hpsi += Vnl psi."""
         import torch
         ncomp = self.sys.ncomp if ncomp is None else ncomp
         nb = psi.shape[0] // ncomp if nb is None else nb
         assert psi.is_cuda and hpsi.is_cuda and psi.is_contiguous() and hpsi.is_contiguous()
         assert psi.dtype == hpsi.dtype and psi.numel() >= nb * ncomp * self.sys.ng
-        fn = {torch.float64: self.lib.qm_hpsi_apply_f64, torch.float32: self.lib.qm_hpsi_apply_f32}[psi.dtype]
+        names = {"full": "qm_hpsi_apply", "local": "qm_local_apply", "nonlocal": "qm_nonlocal_add"}[part]
+        fn = {torch.float64: getattr(self.lib, names + "_f64"),
+              torch.float32: getattr(self.lib, names + "_f32")}[psi.dtype]
         _lib.check(fn(self.ctx, ctypes.c_void_p(hpsi.data_ptr()), ctypes.c_void_p(psi.data_ptr()), nb, ncomp,
                       k_index, _stream(torch, psi.device)))
         return hpsi
@@ -85,6 +88,32 @@ class Hamiltonian:
         import torch
         out = torch.empty_like(psi)
         return self.apply(out, psi, k_index=k_index, ncomp=ncomp)
+
+    # -- dense products (the blas_DGEMM_t / blas_DGEMM_n sub-boundary) ------
+    def gram(self, A, B):
+        """C = A^T B (FP64) for column-major A (k x m) and B (k x n) given as
+        torch tensors of shape (m, k) and (n, k) (one row per column)."""
+        import torch
+        assert A.dtype == B.dtype and A.is_contiguous() and B.is_contiguous() and A.shape[1] == B.shape[1]
+        m, k = A.shape
+        n = B.shape[0]
+        C = torch.empty((n, m), dtype=torch.float64, device=A.device)  # column-major m x n
+        fn = {torch.float64: self.lib.qm_gram_f64, torch.float32: self.lib.qm_gram_f32}[A.dtype]
+        _lib.check(fn(self.ctx, ctypes.c_void_p(A.data_ptr()), m, ctypes.c_void_p(B.data_ptr()), n, k,
+                      ctypes.c_void_p(C.data_ptr()), _stream(torch, A.device)))
+        return C.T  # (m, n) view
+
+    def update(self, Z, C):
+        """Y = Z C for Z as (m, k) rows-of-columns and C an (m, n) matrix; returns Y as (n, k)."""
+        import torch
+        m, k = Z.shape
+        n = C.shape[1]
+        Cc = C.T.contiguous().to(Z.dtype)  # column-major m x n
+        Y = torch.empty((n, k), dtype=Z.dtype, device=Z.device)
+        fn = {torch.float64: self.lib.qm_update_f64, torch.float32: self.lib.qm_update_f32}[Z.dtype]
+        _lib.check(fn(self.ctx, ctypes.c_void_p(Z.data_ptr()), m, ctypes.c_void_p(Cc.data_ptr()), n, k,
+                      ctypes.c_void_p(Y.data_ptr()), _stream(torch, Z.device)))
+        return Y
 
     # -- adjoint -------------------------------------------------------------
     def band_density(self, psi, occ, ncomp=None, rho=None):

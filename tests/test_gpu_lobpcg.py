@@ -1,0 +1,111 @@
+"""GPU LOBPCG (C ABI qm_lobpcg_*) against the FP64 LOBPCG oracle and the
+ARPACK eigenvalues of the 2-atom bcc system (tests/golden/c0_eigenvalues.json).
+
+Tolerances: FP64 path vs oracle after the same iterations from the same start
+<= 1e-12 relative in every eigenvalue; converged FP64 eigenvalues vs ARPACK
+<= 1e-10 Ha; mixed FP32/FP64 path vs ARPACK <= 1e-6 Ha on the converged bands.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = np.array(json.load(open(os.path.join(HERE, "golden", "c0_eigenvalues.json")))["eigenvalues_Ha"])
+
+
+def _oracle_run(qm, orc, s, X, iters, calls, ik=0):
+    import lobpcg_ref
+    k = s.kpoints[ik]
+    L = lobpcg_ref.LOBPCGRef(lambda a: orc.hpsi(s, a, k=k), s.ncomp, s.dV)
+    ev = None
+    for c in range(calls):
+        X, ev, rn, _ = L.solve(X, iters, first_time=(c == 0))
+    return X, ev
+
+
+def _gpu_run(qm, torch, s, X, iters, calls, prec, ik=0):
+    H = qm.hamiltonian.Hamiltonian(s)
+    L = qm.lobpcg.LOBPCG(H, s.nbands, precision=prec)
+    dtype = torch.float64 if prec == 64 else torch.float32
+    x = torch.from_numpy(X).to(device="cuda", dtype=dtype)
+    ev = rn = None
+    for c in range(calls):
+        ev, rn = L.solve(x, iters, k_index=ik, first_time=(c == 0))
+    t = L.timings()
+    out = x.double().cpu().numpy()
+    L.close()
+    H.close()
+    return out, ev, rn, t
+
+
+def test_lobpcg_fp64_matches_oracle_c0(qm, orc, torch_cuda):
+    """8 iterations: <= 1e-12 relative.  20 iterations: within 10x the
+    oracle's own spread under a 1e-15 relative perturbation of the start
+    vectors (once converged bands make S ill-conditioned the iteration
+    amplifies rounding; this bounds what any FP64 implementation can match)."""
+    s = qm.system.config(0)
+    X0 = s.initial_bands()
+    print("case C0 FP64 8 iterations vs oracle", flush=True)
+    _, ev_o = _oracle_run(qm, orc, s, X0, 8, 1)
+    X, ev_g, rn, t = _gpu_run(qm, torch_cuda, s, X0, 8, 1, 64)
+    err = np.abs(ev_g - ev_o).max() / np.abs(ev_o).max()
+    print("8 iterations: max rel eigenvalue diff", err)
+    assert err <= 1e-12
+    nrm = (X * X).sum(axis=1) * s.dV  # returned vectors are dV-normalised
+    assert np.abs(nrm - 1).max() < 1e-12
+    print("case C0 FP64 20 iterations vs oracle", flush=True)
+    _, ev_o = _oracle_run(qm, orc, s, X0, 20, 1)
+    Xp = X0 * (1 + 1e-15 * np.random.default_rng(1).standard_normal(X0.shape))
+    _, ev_p = _oracle_run(qm, orc, s, Xp, 20, 1)
+    spread = np.abs(ev_p - ev_o).max()
+    _, ev_g, rn, t = _gpu_run(qm, torch_cuda, s, X0, 20, 1, 64)
+    print("timings ms", t)
+    diff = np.abs(ev_g - ev_o).max()
+    print(f"20 iterations: max |gpu - oracle| {diff:.3e} Ha, oracle self-spread {spread:.3e} Ha")
+    assert diff <= 10 * spread + 1e-12
+
+
+@pytest.mark.parametrize("prec,tol", [(64, 1e-10), (32, 1e-6)])
+def test_lobpcg_converges_c0(qm, torch_cuda, prec, tol):
+    """Converged eigenvalues vs ARPACK.  Bands closer than 1e-5 Ha (the lowest
+    pair is split by 1.3e-6 Ha) are compared through their cluster mean."""
+    s = qm.system.config(0)
+    X0 = s.initial_bands()
+    print(f"case C0 precision {prec}: 25 calls x 20 iterations", flush=True)
+    _, ev, rn, t = _gpu_run(qm, torch_cuda, s, X0, 20, 25, prec)
+    err = qm.lobpcg.cluster_mean_errors(ev, GOLD)
+    print("per-band |error| (Ha):", np.array2string(np.abs(ev - GOLD[:24]), precision=2))
+    print("cluster-mean |error| (Ha):", np.array2string(err, precision=2))
+    print("timings ms (last call)", t)
+    # bands 0..20 lie below the slowly converging top cluster
+    assert err[:21].max() <= tol
+
+
+def test_lobpcg_complex_kpoint_matches_oracle(qm, orc, torch_cuda):
+    s = qm.system.config(2, nbands=8, grid=40)
+    X0 = s.initial_bands()
+    print("case C2 40^3 8 complex bands k=(1/4,1/4,1/4) FP64 2 x 6 iterations", flush=True)
+    _, ev_o = _oracle_run(qm, orc, s, X0, 6, 2, ik=3)
+    _, ev_g, rn, t = _gpu_run(qm, torch_cuda, s, X0, 6, 2, 64, ik=3)
+    err = np.abs(ev_g - ev_o).max() / np.abs(ev_o).max()
+    print("max rel eigenvalue diff", err)
+    assert err <= 1e-12
+
+
+def test_lobpcg_mixed_complex_close_to_fp64(qm, torch_cuda):
+    """Mixed FP32/FP64 vs FP64 on the bands the FP64 run has converged
+    (|H x - e x| < 1e-5): eigenvalues within 1e-6 Ha."""
+    s = qm.system.config(2, nbands=8, grid=40)
+    X0 = s.initial_bands()
+    print("case C2 40^3 complex mixed vs FP64, 10 x 20 iterations", flush=True)
+    _, ev64, rn64, _ = _gpu_run(qm, torch_cuda, s, X0, 20, 10, 64, ik=1)
+    _, ev32, rn32, _ = _gpu_run(qm, torch_cuda, s, X0, 20, 10, 32, ik=1)
+    conv = np.sqrt(rn64) < 1e-5
+    err = qm.lobpcg.cluster_mean_errors(ev32, ev64)
+    print("mixed - fp64 (Ha):", ev32 - ev64, "converged:", conv)
+    assert conv[:4].all()
+    assert err[conv].max() <= 1e-6
