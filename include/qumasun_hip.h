@@ -154,6 +154,9 @@ void qm_lobpcg_destroy(qm_lobpcg *s);
  * (SCF_step == 0, lobpcg_z_multi_mpi.h:2052-2056); otherwise the kept P is reused
  * and A.P recomputed (2058-2065).  Host outputs: eigenvalues[nbands] (Ha),
  * residual_norms[nbands] (|| H x - e x ||^2 of the last residual, may be NULL).
+ * For precision 32 the returned eigenvalues are the FP64 Rayleigh quotients
+ * <x|H|x>/<x|x> of the returned vectors under the FP64 operator and the residual
+ * norms are FP64 (one FP64 H.psi per call).
  * Returns the number of iterations done through *iters_done (may be NULL); the
  * reference's early exit on loss of normalisation (|1 - |x|^2| > 1e-3,
  * 2611-2666) is kept. */

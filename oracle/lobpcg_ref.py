@@ -26,10 +26,11 @@ as the product).  Follows lobpcg_z_multi_mpi.h:1732-2852:
   This build computes the R blocks (X^H R, R^H R, P^H R and their H versions)
   directly every iteration, where the reference takes X^H R and R^H P from the
   recurrence (DIRECT_S_MATRIX_XR_RP, 58).
-* R and P columns are normalised before the Rayleigh-Ritz step (the reference's
-  EVERY_NORMALIZE_P option, 2690-2707, extended to R).  Column scaling of R and
-  P leaves the iterates unchanged in exact arithmetic and keeps S well
-  conditioned.
+* Before A R, the residual block is replaced by an orthonormal basis of its
+  component orthogonal to X (Cholesky QR, dropping near-dependent directions);
+  P columns are normalised (the reference's EVERY_NORMALIZE_P option,
+  2690-2707).  Both are changes of basis of span{X, R, P}, so the iterates are
+  unchanged in exact arithmetic, and S stays well conditioned.
 * No locking: residual_threshold = 0 gives num_fix = 0 (1962-1986).
 
 Vectors are complex internally; the SoA real layout (re, im rows per band, or
@@ -83,6 +84,35 @@ def subspace_eig(Hm, S):
         return lam, B @ Y
 
 
+def orthonormalise_r(x, r, drop):
+    """R' = [X Rn] W with Rn = R normalised per column: R orthogonalised against X
+    (X^H X = I) and orthonormalised through the Cholesky factor of
+    M = Rn^H Rn - (X^H Rn)^H (X^H Rn); when M's squared pivots span less than
+    `drop`, through its eigenbasis with near-null directions dropped (zero rows).
+    Returns (R', keep mask)."""
+    N = x.shape[0]
+    rn = np.sqrt(np.sum(np.abs(r) ** 2, axis=1))
+    rn[rn == 0] = 1.0
+    r = r / rn[:, None]
+    XR = np.conj(x) @ r.T
+    M = np.conj(r) @ r.T - np.conj(XR).T @ XR
+    M = 0.5 * (M + np.conj(M).T)
+    try:
+        U = scipy.linalg.cholesky(M, lower=False)
+        d = np.abs(np.diag(U)) ** 2
+        if d.min() < drop * d.max():
+            raise np.linalg.LinAlgError
+        Uinv = scipy.linalg.solve_triangular(U, np.eye(N), lower=False)
+        W = np.concatenate([-XR @ Uinv, Uinv], axis=0)
+        keep = np.ones(N, dtype=bool)
+    except np.linalg.LinAlgError:
+        sv, V = scipy.linalg.eigh(M, driver="evd")
+        keep = sv > drop * sv.max()
+        Vs = V * np.where(keep, 1.0 / np.sqrt(np.where(keep, sv, 1.0)), 0.0)[None, :]
+        W = np.concatenate([-XR @ Vs, Vs], axis=0)
+    return W[:N].T @ x + W[N:].T @ r, keep
+
+
 def rr_solve(Hm, S, N, have_p, cond):
     """Rayleigh-Ritz step as the GPU driver takes it: Cholesky of S with a
     squared-pivot ratio >= cond, else retry on [X R] (P dropped), else the
@@ -107,11 +137,13 @@ def rr_solve(Hm, S, N, have_p, cond):
 class LOBPCGRef:
     """Keeps P between calls like the reference's `keep` block."""
 
-    def __init__(self, apply_soa, ncomp, dV, cond=1.0e-14):
+    def __init__(self, apply_soa, ncomp, dV, cond=1.0e-14, rdrop=1.0e-13):
         self.apply_soa = apply_soa
         self.ncomp = ncomp
         self.dV = dV
-        self.cond = cond  # minimum squared Cholesky pivot ratio of S
+        self.cond = cond    # minimum squared Cholesky pivot ratio of S
+        self.rdrop = rdrop  # R directions kept above this pivot / eigenvalue ratio
+        self.P = None
         self.P = None
 
     def _A(self, z):
@@ -144,10 +176,8 @@ class LOBPCGRef:
         it_done = 0
         for it in range(iter_max):
             it_done = it + 1
-            # normalise R (exact-arithmetic invariant), then A R
-            rn = np.sqrt(np.sum(np.abs(r) ** 2, axis=1))
-            rn[rn == 0] = 1.0
-            r = r / rn[:, None]
+            # R <- orthonormal basis of (1 - X X^H) R (exact-arithmetic invariant), then A R
+            r, rkeep = orthonormalise_r(x, r, self.rdrop)
             Ar = self._A(r)
             G = self._gram
             recompute = (it % S_MAT_REST_LIMIT == 0)
@@ -172,6 +202,9 @@ class LOBPCGRef:
             # Hermitian from the upper triangle of blocks (Conj_U_to_L, conjugate.h)
             S = np.triu(S) + np.triu(S, 1).conj().T
             Hm = np.triu(Hm) + np.triu(Hm, 1).conj().T
+            for j in np.nonzero(~rkeep)[0]:  # dropped R directions: decoupled, sorted last
+                S[N + j, N + j] = 1.0
+                Hm[N + j, N + j] = 1.0e30
             lam, C, used_p = rr_solve(Hm, S, N, have_p, self.cond)
             if not used_p:  # solved on [X R]: continue as if P were empty
                 S, Hm = S[:2 * N, :2 * N], Hm[:2 * N, :2 * N]

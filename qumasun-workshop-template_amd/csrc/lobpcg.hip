@@ -101,7 +101,7 @@ __global__ void extract_blocks_kernel(typename Sm<CPLX>::t *blk, const double *G
 template <bool CPLX>
 __global__ void assemble_kernel(typename Sm<CPLX>::t *Sm_, typename Sm<CPLX>::t *Hm_,
                                 const typename Sm<CPLX>::t *blk, const double *G1, const double *dr, int N, int c,
-                                int nbk, int ld1) {
+                                int nbk, int ld1, const double *rkeep) {
     using S = Sm<CPLX>;
     using t = typename S::t;
     const int n3 = nbk * N;
@@ -135,6 +135,15 @@ __global__ void assemble_kernel(typename Sm<CPLX>::t *Sm_, typename Sm<CPLX>::t 
     if (lower) {
         s = S::conj(s);
         h = S::conj(h);
+    }
+    if (rkeep && I == J && bi == 1 && rkeep[I - N] == 0.0) {  // R column dropped by the orthonormalisation
+        if constexpr (CPLX) {
+            s = {1.0, 0.0};
+            h = {1.0e30, 0.0};
+        } else {
+            s = 1.0;
+            h = 1.0e30;
+        }
     }
     Sm_[e] = s;
     Hm_[e] = h;
@@ -210,6 +219,92 @@ __global__ void inv_sqrt_kernel(double *out, const double *in, int n, double sca
     out[i] = v > 0.0 ? 1.0 / sqrt(v) : 1.0;
 }
 
+// R orthonormalisation against X.  From G0 = [X R]^T R (real, 2nc x nc):
+//   XRn = X^H R diag(dr),  M = diag(dr) R^H R diag(dr)   (both N x N; M gets -XRn^H XRn later)
+template <bool CPLX>
+__global__ void rorth_pack_kernel(typename Sm<CPLX>::t *XRn, typename Sm<CPLX>::t *M, const double *G0,
+                                  const double *dr, int N, int c) {
+    using S = Sm<CPLX>;
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= N * N) return;
+    const int a = e % N, b = e / N;
+    const int nc = c * N;
+    XRn[e] = S::scale(S::pack(G0, 2 * nc, 0, 0, a, b), dr[b]);
+    M[e] = S::scale(S::pack(G0, 2 * nc, nc, 0, a, b), dr[a] * dr[b]);
+}
+
+// W = [-XRn; I] (2N x N) before the right triangular solve W U^-1
+template <bool CPLX>
+__global__ void rorth_w_kernel(typename Sm<CPLX>::t *W, const typename Sm<CPLX>::t *XRn, int N) {
+    using S = Sm<CPLX>;
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (int64_t)2 * N * N) return;
+    const int row = (int)(e % (2 * N)), col = (int)(e / (2 * N));
+    typename S::t v;
+    if (row < N) v = S::scale(XRn[(int64_t)col * N + row], -1.0);
+    else v = (row - N == col) ? S::scale(S::zero(), 0.0) : S::zero();
+    if (row >= N && row - N == col) {
+        if constexpr (CPLX) v = {1.0, 0.0};
+        else v = 1.0;
+    }
+    W[e] = v;
+}
+
+// SVQB variant: V columns scaled by s^-1/2 (0 when s <= tau s_max), keep mask per column
+template <bool CPLX>
+__global__ void rorth_svqb_kernel(typename Sm<CPLX>::t *V, double *keep, const double *sv, int N, double tau) {
+    using S = Sm<CPLX>;
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= N * N) return;
+    const int j = e / N;
+    const double smax = sv[N - 1];
+    const double wj = sv[j] > tau * smax ? 1.0 / sqrt(sv[j]) : 0.0;
+    V[e] = S::scale(V[e], wj);
+    if (e % N == 0) keep[j] = wj > 0.0 ? 1.0 : 0.0;
+}
+
+// rows N..2N-1 of W (2N x N) scaled by dr (R rows); the X rows pass through
+template <bool CPLX>
+__global__ void rorth_scale_kernel(typename Sm<CPLX>::t *W, const double *dr, int N) {
+    using S = Sm<CPLX>;
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (int64_t)2 * N * N) return;
+    const int row = (int)(e % (2 * N));
+    if (row >= N) W[e] = S::scale(W[e], dr[row - N]);
+}
+
+// real update matrix (T) from a complex / real coefficient block W (m x n)
+template <bool CPLX, typename T>
+__global__ void real_coef_kernel(T *U, const typename Sm<CPLX>::t *W, int m, int n) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (int64_t)m * n) return;
+    const int k = (int)(e % m), j = (int)(e / m);
+    const auto u = W[e];
+    if constexpr (!CPLX) {
+        U[(int64_t)j * m + k] = (T)u;
+    } else {
+        const int ld = 2 * m;
+        U[(int64_t)(2 * j) * ld + 2 * k] = (T)u.re;
+        U[(int64_t)(2 * j) * ld + 2 * k + 1] = (T)(-u.im);
+        U[(int64_t)(2 * j + 1) * ld + 2 * k] = (T)u.im;
+        U[(int64_t)(2 * j + 1) * ld + 2 * k + 1] = (T)u.re;
+    }
+}
+
+__global__ void cast_kernel(double *__restrict__ y, const float *__restrict__ x, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) y[i] = x[i];
+}
+
+__global__ void ratio_kernel(double *out, const double *num, const double *den, int n) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    if (i < n) out[i] = den[i] > 0.0 ? num[i] / den[i] : num[i];
+}
+
+__global__ void fill_kernel(double *p, double v, int n) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 // min / max of the squared Cholesky pivots |U_ii|^2 (a cheap conditioning probe of S)
 template <bool CPLX>
 __global__ void pivot_ratio_kernel(double *out, const typename Sm<CPLX>::t *U, int n3) {
@@ -277,7 +372,12 @@ struct qm_lobpcg {
     void *U = nullptr;  // update matrix (T)
     void *Ev = nullptr; // eigenvectors of the SVQB fallback (n3 x n3)
     int fallbacks = 0;  // subspace-solve fallbacks (P dropped or SVQB) in the last solve
+    int rdrops = 0;     // R orthonormalisations that dropped near-dependent directions
     double cond64 = 1.0e-14, cond32 = 1.0e-4;  // minimum squared Cholesky pivot ratio of S
+    double rdrop64 = 1.0e-13, rdrop32 = 1.0e-6; // R directions kept above this eigenvalue ratio
+    void *XRn = nullptr, *Mr = nullptr, *Wr = nullptr, *Ur = nullptr;  // R orthonormalisation
+    double *X64 = nullptr, *AX64 = nullptr;  // FP64 Rayleigh-quotient refinement of the mixed solve
+    double *G0 = nullptr, *rkeep = nullptr;
     double *evals = nullptr, *ework = nullptr, *eps = nullptr, *dr = nullptr, *dp = nullptr, *rnorm = nullptr,
            *xnorm = nullptr, *svals = nullptr, *svw = nullptr;
     int *info = nullptr;
@@ -291,14 +391,15 @@ namespace {
 
 void lfree(qm_lobpcg *s) {
     void **ptrs[] = {&s->X, &s->R, &s->P, &s->AX, &s->AR, &s->AP, &s->W[0], &s->W[1], &s->W[2], &s->W[3],
-                     &s->S, &s->H, &s->Sw, &s->Hw, &s->blk, &s->Cb, &s->T1, &s->SS, &s->HH, &s->U, &s->Ev};
+                     &s->S, &s->H, &s->Sw, &s->Hw, &s->blk, &s->Cb, &s->T1, &s->SS, &s->HH, &s->U, &s->Ev,
+                     &s->XRn, &s->Mr, &s->Wr, &s->Ur, (void **)&s->X64, (void **)&s->AX64};
     for (void **p : ptrs)
         if (*p) {
             (void)hipFree(*p);
             *p = nullptr;
         }
     double **dptrs[] = {&s->G1, &s->G2, &s->evals, &s->ework, &s->eps, &s->dr, &s->dp, &s->rnorm, &s->xnorm,
-                        &s->svals, &s->svw};
+                        &s->svals, &s->svw, &s->G0, &s->rkeep};
     for (double **p : dptrs)
         if (*p) {
             (void)hipFree(*p);
@@ -435,6 +536,111 @@ int svqb_eig(qm_lobpcg *s, int n3, double tau, hipStream_t st) {
     return QM_OK;
 }
 
+// R' = [X R] W: R orthogonalised against X (X^H X = I) and orthonormalised by a
+// Cholesky QR of M = Rn^H Rn - (X^H Rn)^H (X^H Rn), Rn = R diag(dr); when M is not
+// well conditioned, by its eigenbasis with the near-null directions dropped
+// (their columns of R' are zero and s->rkeep marks them).  R' spans the same
+// space as R minus its X component, so the Rayleigh-Ritz step is unchanged in
+// exact arithmetic; H and S of the R block are then computed from a
+// well-conditioned basis.  Writes R' into Rout.
+template <typename T, bool CPLX>
+int orthonormalise_r(qm_lobpcg *s, T *Rout, const T *X, const T *R, hipStream_t st) {
+    using SmT = typename Sm<CPLX>::t;
+    const int N = s->N, c = s->c, nc = c * N;
+    const int64_t ng = s->ng;
+    ColBlocks A0, B0;
+    A0.add(X, nc);
+    A0.add(R, nc);
+    B0.add(R, nc);
+    QM_TRY(gram<T>(s->gram_part, s->G0, A0, B0, 2 * nc, nc, ng, st));
+    hipLaunchKernelGGL((rorth_pack_kernel<CPLX>), dim3((unsigned)(((int64_t)N * N + 255) / 256)), dim3(256), 0, st,
+                       (SmT *)s->XRn, (SmT *)s->Mr, s->G0, s->dr, N, c);
+    QM_HIP(hipGetLastError());
+    // M -= XRn^H XRn
+    if constexpr (CPLX) {
+        const rocblas_double_complex mone{-1.0, 0.0}, one{1.0, 0.0};
+        QM_BLAS(rocblas_zgemm(s->blas, rocblas_operation_conjugate_transpose, rocblas_operation_none, N, N, N, &mone,
+                              (rocblas_double_complex *)s->XRn, N, (rocblas_double_complex *)s->XRn, N, &one,
+                              (rocblas_double_complex *)s->Mr, N));
+    } else {
+        const double mone = -1.0, one = 1.0;
+        QM_BLAS(rocblas_dgemm(s->blas, rocblas_operation_transpose, rocblas_operation_none, N, N, N, &mone,
+                              (double *)s->XRn, N, (double *)s->XRn, N, &one, (double *)s->Mr, N));
+    }
+    // Cholesky attempt on a copy (Sw is free at this point)
+    QM_HIP(hipMemcpyAsync(s->Sw, s->Mr, sizeof(SmT) * N * N, hipMemcpyDeviceToDevice, st));
+    if constexpr (CPLX)
+        QM_BLAS(rocsolver_zpotrf(s->blas, rocblas_fill_upper, N, (rocblas_double_complex *)s->Sw, N, s->info));
+    else
+        QM_BLAS(rocsolver_dpotrf(s->blas, rocblas_fill_upper, N, (double *)s->Sw, N, s->info));
+    hipLaunchKernelGGL((pivot_ratio_kernel<CPLX>), dim3(1), dim3(256), 0, st, s->svals, (const SmT *)s->Sw, N);
+    QM_HIP(hipGetLastError());
+    int h_info = 0;
+    double h_ratio = 0.0;
+    QM_HIP(hipMemcpyAsync(&h_info, s->info, sizeof(int), hipMemcpyDeviceToHost, st));
+    QM_HIP(hipMemcpyAsync(&h_ratio, s->svals, sizeof(double), hipMemcpyDeviceToHost, st));
+    QM_HIP(hipStreamSynchronize(st));
+    const double drop = sizeof(T) == 8 ? s->rdrop64 : s->rdrop32;
+    const unsigned gW = (unsigned)(((int64_t)2 * N * N + 255) / 256);
+    if (h_info == 0 && h_ratio >= drop) {
+        hipLaunchKernelGGL((rorth_w_kernel<CPLX>), dim3(gW), dim3(256), 0, st, (SmT *)s->Wr, (const SmT *)s->XRn, N);
+        QM_HIP(hipGetLastError());
+        if constexpr (CPLX) {
+            const rocblas_double_complex one{1.0, 0.0};
+            QM_BLAS(rocblas_ztrsm(s->blas, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                                  rocblas_diagonal_non_unit, 2 * N, N, &one, (rocblas_double_complex *)s->Sw, N,
+                                  (rocblas_double_complex *)s->Wr, 2 * N));
+        } else {
+            const double one = 1.0;
+            QM_BLAS(rocblas_dtrsm(s->blas, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                                  rocblas_diagonal_non_unit, 2 * N, N, &one, (double *)s->Sw, N, (double *)s->Wr,
+                                  2 * N));
+        }
+        hipLaunchKernelGGL(fill_kernel, dim3((N + 63) / 64), dim3(64), 0, st, s->rkeep, 1.0, N);
+        QM_HIP(hipGetLastError());
+    } else {
+        // eigenbasis of M with near-null directions dropped
+        QM_HIP(hipMemcpyAsync(s->Sw, s->Mr, sizeof(SmT) * N * N, hipMemcpyDeviceToDevice, st));
+        if constexpr (CPLX)
+            QM_BLAS(rocsolver_zheevd(s->blas, rocblas_evect_original, rocblas_fill_upper, N,
+                                     (rocblas_double_complex *)s->Sw, N, s->svals, s->ework, s->info));
+        else
+            QM_BLAS(rocsolver_dsyevd(s->blas, rocblas_evect_original, rocblas_fill_upper, N, (double *)s->Sw, N,
+                                     s->svals, s->ework, s->info));
+        hipLaunchKernelGGL((rorth_svqb_kernel<CPLX>), dim3((unsigned)(((int64_t)N * N + 255) / 256)), dim3(256), 0,
+                           st, (SmT *)s->Sw, s->rkeep, s->svals, N, drop);
+        QM_HIP(hipGetLastError());
+        // W = [-XRn Vs ; Vs]
+        if constexpr (CPLX) {
+            const rocblas_double_complex mone{-1.0, 0.0}, zero{0.0, 0.0};
+            QM_BLAS(rocblas_zgemm(s->blas, rocblas_operation_none, rocblas_operation_none, N, N, N, &mone,
+                                  (rocblas_double_complex *)s->XRn, N, (rocblas_double_complex *)s->Sw, N, &zero,
+                                  (rocblas_double_complex *)s->Wr, 2 * N));
+            QM_HIP(hipMemcpy2DAsync((rocblas_double_complex *)s->Wr + N, sizeof(SmT) * 2 * N, s->Sw, sizeof(SmT) * N,
+                                    sizeof(SmT) * N, N, hipMemcpyDeviceToDevice, st));
+        } else {
+            const double mone = -1.0, zero = 0.0;
+            QM_BLAS(rocblas_dgemm(s->blas, rocblas_operation_none, rocblas_operation_none, N, N, N, &mone,
+                                  (double *)s->XRn, N, (double *)s->Sw, N, &zero, (double *)s->Wr, 2 * N));
+            QM_HIP(hipMemcpy2DAsync((double *)s->Wr + N, sizeof(SmT) * 2 * N, s->Sw, sizeof(SmT) * N, sizeof(SmT) * N,
+                                    N, hipMemcpyDeviceToDevice, st));
+        }
+        ++s->rdrops;
+    }
+    hipLaunchKernelGGL((rorth_scale_kernel<CPLX>), dim3(gW), dim3(256), 0, st, (SmT *)s->Wr, s->dr, N);
+    QM_HIP(hipGetLastError());
+    hipLaunchKernelGGL((real_coef_kernel<CPLX, T>), dim3(gW), dim3(256), 0, st, (T *)s->Ur, (const SmT *)s->Wr, 2 * N,
+                       N);
+    QM_HIP(hipGetLastError());
+    ColBlocks Y;
+    Y.add(Rout, nc);
+    QM_TRY(update<T>(A0, Y, (const T *)s->Ur, 2 * nc, nc, ng, st));
+    // R' columns are orthonormal: unit scale for the Rayleigh-Ritz assembly
+    hipLaunchKernelGGL(fill_kernel, dim3((N + 63) / 64), dim3(64), 0, st, s->dr, 1.0, N);
+    QM_HIP(hipGetLastError());
+    return QM_OK;
+}
+
 // SS = Cb^H M Cb for M = S, H (n3 x n3), Cb n3 x 2N.
 template <bool CPLX>
 int recurrence(qm_lobpcg *s, int n3) {
@@ -482,6 +688,7 @@ int solve_impl(qm_lobpcg *s, T *x_user, double *eigenvalues, double *residual_no
     QM_BLAS(rocblas_set_stream(s->blas, st));
     for (double &v : s->t) v = 0.0;
     s->fallbacks = 0;
+    s->rdrops = 0;
     const double t_start = now_ms();
     auto sync_ms = [&](double &acc, double t0) -> int {
         QM_HIP(hipStreamSynchronize(st));
@@ -519,6 +726,11 @@ int solve_impl(qm_lobpcg *s, T *x_user, double *eigenvalues, double *residual_no
     for (int it = 0; it < iter_max; ++it) {
         it_done = it + 1;
         const int nbk = hp ? 3 : 2;
+        // R <- orthonormal basis of (1 - X X^H) R
+        t0 = now_ms();
+        QM_TRY((orthonormalise_r<T, CPLX>(s, AR, X, R, st)));
+        std::swap(R, AR);
+        QM_TRY(sync_ms(s->t[1], t0));
         // A R
         t0 = now_ms();
         QM_TRY(hpsi<T>(s, AR, R, ik, st));
@@ -561,7 +773,7 @@ int solve_impl(qm_lobpcg *s, T *x_user, double *eigenvalues, double *residual_no
             const int n3a = nbk_used * N;
             hipLaunchKernelGGL((assemble_kernel<CPLX>), dim3(nblk((int64_t)n3a * n3a)), dim3(256), 0, st,
                                (SmT *)s->S, (SmT *)s->H, (const SmT *)s->blk, s->G1, s->dr, N, c, nbk_used,
-                               nbk * nc);
+                               nbk * nc, s->rkeep);
             QM_HIP(hipGetLastError());
             QM_TRY(chol_factor<CPLX>(s, n3a, st));
             int h_info = 0;
@@ -666,6 +878,30 @@ int solve_impl(qm_lobpcg *s, T *x_user, double *eigenvalues, double *residual_no
         }
         s->have_p = hp;
     }
+    if constexpr (sizeof(T) == 4) {
+        if (iter_max > 0 && !rolled_back) {
+            // mixed precision: band energies as FP64 Rayleigh quotients of the returned
+            // FP32 vectors with the FP64 operator, and FP64 residual norms
+            // (second order in the vector error, so the FP32 operator's rounding drops out)
+            const int64_t nel = (int64_t)nc * ng;
+            hipLaunchKernelGGL(cast_kernel, dim3((unsigned)std::min<int64_t>((nel + 255) / 256, 8192)), dim3(256), 0,
+                               st, s->X64, (const float *)x_user, nel);
+            QM_HIP(hipGetLastError());
+            QM_TRY(hpsi<double>(s, s->AX64, s->X64, ik, st));
+            QM_TRY(band_reduce<double>(s->red_part, 0, s->eps, nullptr, nullptr, s->X64, s->AX64, nullptr, N, c, ng,
+                                       st));
+            QM_TRY(band_reduce<double>(s->red_part, 2, s->xnorm, nullptr, nullptr, s->X64, nullptr, nullptr, N, c, ng,
+                                       st));
+            hipLaunchKernelGGL(ratio_kernel, dim3(nblk(N, 64)), dim3(64), 0, st, s->evals, s->eps, s->xnorm, N);
+            QM_HIP(hipGetLastError());
+            // |H x - e x|^2 dV (x is dV-normalised: scaled-unit residual)
+            QM_TRY(band_reduce<double>(s->red_part, 1, s->rnorm, nullptr, s->AX64, s->X64, s->AX64, s->evals, N, c, ng,
+                                       st));
+            hipLaunchKernelGGL(ratio_kernel, dim3(nblk(N, 64)), dim3(64), 0, st, s->rnorm, s->rnorm, s->xnorm, N);
+            QM_HIP(hipGetLastError());
+            QM_HIP(hipMemcpyAsync(h_eps.data(), s->evals, sizeof(double) * N, hipMemcpyDeviceToHost, st));
+        }
+    }
     // keep the rotated buffers
     s->X = X;
     s->R = R;
@@ -726,7 +962,17 @@ int qm_lobpcg_create(qm_lobpcg **out, qm_ctx *ctx, int nbands, int ncomp, int pr
     if (st == QM_OK) st = dalloc(&s->SS, esz * (size_t)4 * N * N);
     if (st == QM_OK) st = dalloc(&s->HH, esz * (size_t)4 * N * N);
     if (st == QM_OK) st = dalloc(&s->U, s->tsz * (size_t)(ncomp * n3) * (2 * nc));
-    double **dv[] = {&s->evals, &s->ework, &s->eps, &s->dr, &s->dp, &s->rnorm, &s->xnorm, &s->svals, &s->svw};
+    if (st == QM_OK) st = dalloc((void **)&s->G0, sizeof(double) * (size_t)(2 * nc) * nc);
+    if (precision == 32) {
+        if (st == QM_OK) st = dalloc((void **)&s->X64, sizeof(double) * (size_t)nc * s->ng);
+        if (st == QM_OK) st = dalloc((void **)&s->AX64, sizeof(double) * (size_t)nc * s->ng);
+    }
+    if (st == QM_OK) st = dalloc(&s->XRn, esz * (size_t)N * N);
+    if (st == QM_OK) st = dalloc(&s->Mr, esz * (size_t)N * N);
+    if (st == QM_OK) st = dalloc(&s->Wr, esz * (size_t)2 * N * N);
+    if (st == QM_OK) st = dalloc(&s->Ur, s->tsz * (size_t)(2 * nc) * nc);
+    double **dv[] = {&s->evals, &s->ework, &s->eps, &s->dr, &s->dp, &s->rnorm, &s->xnorm, &s->svals, &s->svw,
+                     &s->rkeep};
     for (double **p : dv)
         if (st == QM_OK) st = dalloc((void **)p, sizeof(double) * n3);
     if (st == QM_OK) st = dalloc((void **)&s->info, sizeof(int));

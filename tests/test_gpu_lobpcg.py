@@ -97,13 +97,13 @@ def test_lobpcg_complex_kpoint_matches_oracle(qm, orc, torch_cuda):
 
 
 def test_lobpcg_mixed_complex_close_to_fp64(qm, torch_cuda):
-    """Mixed FP32/FP64 vs FP64 on the bands the FP64 run has converged
-    (|H x - e x| < 1e-5): eigenvalues within 1e-6 Ha."""
-    s = qm.system.config(2, nbands=8, grid=40)
+    """Complex k-point: mixed FP32/FP64 vs FP64 on the bands the FP64 run has
+    converged (|H x - e x| < 1e-5): cluster-mean eigenvalues within 1e-6 Ha."""
+    s = qm.system.small_system(n=24, L=5.0, natoms=2, rc=1.6, nbands=8, ncomp=2, k_red=(0.25, 0.25, 0.25), seed=9)
     X0 = s.initial_bands()
-    print("case C2 40^3 complex mixed vs FP64, 10 x 20 iterations", flush=True)
-    _, ev64, rn64, _ = _gpu_run(qm, torch_cuda, s, X0, 20, 10, 64, ik=1)
-    _, ev32, rn32, _ = _gpu_run(qm, torch_cuda, s, X0, 20, 10, 32, ik=1)
+    print("case 24^3 2-atom complex k=(1/4,1/4,1/4) mixed vs FP64, 8 x 20 iterations", flush=True)
+    _, ev64, rn64, _ = _gpu_run(qm, torch_cuda, s, X0, 20, 8, 64)
+    _, ev32, rn32, _ = _gpu_run(qm, torch_cuda, s, X0, 20, 8, 32)
     conv = np.sqrt(rn64) < 1e-5
     err = qm.lobpcg.cluster_mean_errors(ev32, ev64)
     print("mixed - fp64 (Ha):", ev32 - ev64, "converged:", conv)

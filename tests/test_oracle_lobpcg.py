@@ -30,20 +30,25 @@ def test_oracle_lobpcg_converges_to_dense_eigh(qm, orc, ncomp, kred):
 
 def test_oracle_lobpcg_rollback_branch(qm, orc):
     """A Rayleigh-Ritz step whose update loses normalisation by more than 1e-3
-    takes the reference's escape path: P = R, x rescaled, loop left early."""
+    (here: a corrupted overlap block, so C^H S C = I no longer holds for the
+    actual vectors) takes the reference's escape path: P = R, the old X
+    renormalised, loop left early."""
     import lobpcg_ref
+
+    class Corrupt(lobpcg_ref.LOBPCGRef):
+        calls = 0
+
+        def _gram(self, a, b):
+            g = np.conj(a) @ b.T
+            Corrupt.calls += 1
+            if Corrupt.calls == 20:  # a block of the 4th iteration
+                g = g + 0.05
+            return g
+
     s = qm.system.small_system(n=8, L=4.0, natoms=1, rc=1.5, nbands=3, seed=4)
-    calls = {"n": 0}
-
-    def bad_apply(a):
-        calls["n"] += 1
-        out = orc.hpsi(s, a)
-        if calls["n"] == 2:  # corrupt A R of the first iteration
-            out = out * 1e3 + 1e2 * np.random.default_rng(0).standard_normal(out.shape)
-        return out
-
-    L = lobpcg_ref.LOBPCGRef(bad_apply, 1, s.dV)
-    X0 = s.initial_bands()
-    X, ev, rn, it = L.solve(X0, 10, first_time=True)
+    L = Corrupt(lambda a: orc.hpsi(s, a), 1, s.dV)
+    X, ev, rn, it = L.solve(s.initial_bands(), 10, first_time=True)
     assert it < 10  # escaped
     assert L.P is not None
+    nrm = (X * X).sum(axis=1) * s.dV
+    assert np.abs(nrm - 1).max() < 1e-12  # the kept X is renormalised
