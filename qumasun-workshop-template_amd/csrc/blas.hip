@@ -17,34 +17,10 @@
 #include <algorithm>
 
 #include "qm_internal.h"
+#include "qm_mfma.h"
 
 namespace qm {
 namespace {
-
-template <typename T>
-struct Mfma;
-template <>
-struct Mfma<float> {
-    typedef float acc_t __attribute__((ext_vector_type(4)));
-    typedef float vec_t __attribute__((ext_vector_type(4)));
-    static constexpr int VEC = 4;
-    static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
-        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-    }
-    // output row of item r in lane group g (lane l: g = l / 16, column l % 16)
-    static __device__ __forceinline__ int row(int g, int r) { return 4 * g + r; }
-};
-template <>
-struct Mfma<double> {
-    typedef double acc_t __attribute__((ext_vector_type(4)));
-    typedef double vec_t __attribute__((ext_vector_type(2)));
-    static constexpr int VEC = 2;
-    static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
-        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-    }
-    // the FP64 16x16x4 result interleaves rows across lane groups
-    static __device__ __forceinline__ int row(int g, int r) { return g + 4 * r; }
-};
 
 template <typename T>
 __device__ __forceinline__ const T *col_ptr(const ColBlocks &c, int j, int64_t ng) {

@@ -80,6 +80,15 @@ static void free_projectors(Projectors &p) {
     dfree(p.d_cov_off);
     dfree(p.d_cov_vox);
     dfree(p.d_atom_order);
+    dfree(p.d_tab16_64);
+    dfree(p.d_tab16_32);
+    dfree(p.d_chunk_atom);
+    dfree(p.d_chunk_v0);
+    dfree(p.d_atom_chunk_off);
+    dfree(p.d_proj_atom);
+    dfree(p.d_block_atom_off);
+    dfree(p.d_block_atoms);
+    dfree(p.d_cov_slot);
     p = Projectors();
 }
 
@@ -244,6 +253,7 @@ void qm_destroy(qm_ctx *ctx) {
     ctx->coef.release();
     ctx->fwork.release();
     ctx->gemm_part.release();
+    ctx->nl_part.release();
     delete ctx;
 }
 
@@ -346,6 +356,63 @@ int qm_set_projectors(qm_ctx *ctx, int natoms, const int *nproj, const int64_t *
     QM_TRY(upload(&p.d_cov_off, cov_off.data(), cov_off.size()));
     QM_TRY(upload(&p.d_cov_vox, order.data(), order.size()));
     QM_TRY(upload(&p.d_atom_order, aord.data(), aord.size()));
+    // voxel-major, 16-padded tables and the projection work list
+    {
+        std::vector<double> t16((size_t)p.vtot * kMaxProj, 0.0);
+        std::vector<float> t16f((size_t)p.vtot * kMaxProj, 0.0f);
+        std::vector<int> catom, cv0, acoff(natoms + 1, 0), patom(p.ptot);
+        for (int a = 0; a < natoms; ++a) {
+            const int64_t V = vox_off[a + 1] - vox_off[a];
+            for (int w = 0; w < nproj[a]; ++w) {
+                patom[proj_off[a] + w] = a;
+                for (int64_t v = 0; v < V; ++v) {
+                    const double val = table[tab_off[a] + w * V + v];
+                    t16[(size_t)(vox_off[a] + v) * kMaxProj + w] = val;
+                    t16f[(size_t)(vox_off[a] + v) * kMaxProj + w] = (float)val;
+                }
+            }
+            for (int64_t v0 = 0; v0 < V; v0 += kProjChunk) {
+                catom.push_back(a);
+                cv0.push_back((int)v0);
+            }
+            acoff[a + 1] = (int)catom.size();
+        }
+        p.nchunk = (int)catom.size();
+        QM_TRY(upload(&p.d_tab16_64, t16.data(), t16.size()));
+        QM_TRY(upload(&p.d_tab16_32, t16f.data(), t16f.size()));
+        QM_TRY(upload(&p.d_chunk_atom, catom.data(), catom.size()));
+        QM_TRY(upload(&p.d_chunk_v0, cv0.data(), cv0.size()));
+        QM_TRY(upload(&p.d_atom_chunk_off, acoff.data(), acoff.size()));
+        QM_TRY(upload(&p.d_proj_atom, patom.data(), patom.size()));
+    }
+    // per scatter workgroup: distinct atoms of its entries, and every entry's slot
+    {
+        const int nb = (p.ncov + kScatterBlock - 1) / kScatterBlock;
+        std::vector<int> boff(nb + 1, 0), batoms, slot(p.vtot, -1);
+        for (int b = 0; b < nb; ++b) {
+            std::vector<int> list;
+            const int i0 = b * kScatterBlock, i1 = std::min(p.ncov, i0 + kScatterBlock);
+            for (int i = i0; i < i1; ++i)
+                for (int e = cov_off[i]; e < cov_off[i + 1]; ++e) {
+                    const int a = atom[order[e]];
+                    int s = -1;
+                    for (int q = 0; q < (int)list.size(); ++q)
+                        if (list[q] == a) s = q;
+                    if (s < 0 && (int)list.size() < kScatterSlots) {
+                        list.push_back(a);
+                        s = (int)list.size() - 1;
+                    }
+                    slot[e] = s;
+                }
+            batoms.insert(batoms.end(), list.begin(), list.end());
+            boff[b + 1] = (int)batoms.size();
+        }
+        if (batoms.empty()) batoms.push_back(0);
+        p.nsblock = nb;
+        QM_TRY(upload(&p.d_block_atom_off, boff.data(), boff.size()));
+        QM_TRY(upload(&p.d_block_atoms, batoms.data(), batoms.size()));
+        QM_TRY(upload(&p.d_cov_slot, slot.data(), slot.size()));
+    }
     for (auto &k : ctx->kpts) QM_TRY(build_phase(ctx, k));
     return QM_OK;
 }

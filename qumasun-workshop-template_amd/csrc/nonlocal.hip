@@ -9,20 +9,24 @@
 //
 // GPU structure (replaces per-atom DGEMM_t / DGEMM_n calls, host Bloch loops,
 // host-side MPI reductions and one AddSubgrid launch per atom colour):
-//   1. project: one launch covers every atom (uneven sphere sizes, heaviest
-//      atoms scheduled first) x every band group.  Each block gathers the
-//      atom's voxels of NBG bands straight from the grid, applies e^{+ik.r},
-//      accumulates <p_w|psi_n> per thread, and reduces across the block in
-//      FP64; E_w dV is folded in before the coefficient is stored.
+//   1. project: one launch over (voxel chunk, 16-column band tile) work items
+//      of every atom, so spheres of any size balance by construction.  Each
+//      wave gathers 256 voxels x 16 SoA columns straight from the grid, applies
+//      e^{+ik.r}, and multiplies by the voxel-major projector table on MFMA
+//      (16 projectors x 16 columns, K = voxels); the four waves of a chunk and
+//      then the chunks of an atom are summed in FP64 in a fixed order, and
+//      E_w dV is folded in (coef).
 //   2. scatter: owner-computes over the grid points covered by any sphere.
 //      Each covered point sums the contributions of all atoms whose sphere
-//      contains it, so overlapping spheres need no atomics or colouring and
-//      the result is deterministic.
+//      contains it for a tile of columns, so overlapping spheres need no
+//      atomics or colouring, tables are read once per tile, and the result is
+//      deterministic.
 #include <algorithm>
 #include <cmath>
 #include <numeric>
 
 #include "qm_internal.h"
+#include "qm_mfma.h"
 
 namespace qm {
 namespace {
@@ -36,129 +40,183 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// coef[(proj_off[a] + w) * ncol + n*NC + c] = E_w dV <p_w | e^{ik.r} psi_n>_c
-template <typename T, int MM, int NBG, int NC, bool PH>
-__global__ __launch_bounds__(NL_THREADS) void nl_project_kernel(
-    T *__restrict__ coef, const T *__restrict__ psi, int64_t ng, int nb, const int *__restrict__ atom_order,
-    const int *__restrict__ nproj, const int *__restrict__ proj_off, const int64_t *__restrict__ vox_off,
-    const int64_t *__restrict__ tab_off, const int *__restrict__ vox_grid, const T *__restrict__ table,
-    const T *__restrict__ phase, const double *__restrict__ energy, double dV) {
-    constexpr int NV = MM * NBG * NC;
-    __shared__ double red[NL_THREADS / 64][NV];
-    const int a = atom_order[blockIdx.x];
-    const int n0 = blockIdx.y * NBG;
-    const int64_t v0 = vox_off[a];
-    const int V = (int)(vox_off[a + 1] - v0);
-    const int M = nproj[a];
-    const T *tab = table + tab_off[a];
-    const int ncol = nb * NC;
-
-    T acc[MM][NBG][NC];
-#pragma unroll
-    for (int w = 0; w < MM; ++w)
-#pragma unroll
-        for (int b = 0; b < NBG; ++b)
-#pragma unroll
-            for (int c = 0; c < NC; ++c) acc[w][b][c] = T(0);
-
-    const T *col[NBG][NC];
-#pragma unroll
-    for (int b = 0; b < NBG; ++b)
-#pragma unroll
-        for (int c = 0; c < NC; ++c) col[b][c] = psi + ((int64_t)min(n0 + b, nb - 1) * NC + c) * ng;
-
-    for (int v = threadIdx.x; v < V; v += NL_THREADS) {
-        const int g = vox_grid[v0 + v];
-        T pw[MM];
-#pragma unroll
-        for (int w = 0; w < MM; ++w) pw[w] = (w < M) ? tab[(int64_t)w * V + v] : T(0);
-        T cs = T(1), sn = T(0);
-        if constexpr (PH) {
-            cs = phase[2 * (v0 + v)];
-            sn = phase[2 * (v0 + v) + 1];
-        }
-#pragma unroll
-        for (int b = 0; b < NBG; ++b) {
-            T re = col[b][0][g];
-            T im = T(0);
-            if constexpr (NC == 2) im = col[b][1][g];
+// part[(item * 16 + w) * ncol + col] = <p_w | e^{ik.r} psi>_col summed over the
+// item's voxels (col = real SoA column n * NC + c).  Each wave takes 256 voxels
+// in sub-chunks of 64: the 64 x 16 block of (phase-rotated) psi values is
+// gathered with lanes along voxels (coalesced along x runs) into LDS, then fed to
+// MFMA as the B operand against the voxel-major projector table (A operand).
+constexpr int PJ_VPW = kProjChunk / 4;  // voxels per wave
+constexpr int PJ_SUB = 64;              // voxels per LDS sub-chunk
+template <typename T, int NC, bool PH>
+__global__ __launch_bounds__(256) void nl_project_kernel(double *__restrict__ part, const T *__restrict__ psi,
+                                                         int64_t ng, int ncol, const int *__restrict__ chunk_atom,
+                                                         const int *__restrict__ chunk_v0,
+                                                         const int64_t *__restrict__ vox_off,
+                                                         const int *__restrict__ vox_grid, const T *__restrict__ tab16,
+                                                         const T *__restrict__ phase) {
+    using M = Mfma<T>;
+    using acc_t = typename M::acc_t;
+    __shared__ T stage[4][PJ_SUB][17];
+    __shared__ double red[4][256];
+    const int item = blockIdx.x;
+    const int tile = blockIdx.y;
+    const int a = chunk_atom[item];
+    const int64_t vg0 = vox_off[a];
+    const int V = (int)(vox_off[a + 1] - vg0);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int j = lane & 15, g = lane >> 4;
+    const int vb = chunk_v0[item] + wave * PJ_VPW;
+    acc_t acc = acc_t{0, 0, 0, 0};
+    for (int sc = 0; sc < PJ_VPW / PJ_SUB; ++sc) {
+        // gather: lane = voxel, 16 columns
+        {
+            const int v = vb + sc * PJ_SUB + lane;
+            const bool in = v < V;
+            const int64_t vid = vg0 + (in ? v : 0);
+            const int gi = in ? vox_grid[vid] : 0;
+            T cs = T(1), sn = T(0);
             if constexpr (PH) {
-                const T br = cs * re - sn * im;
-                const T bi = cs * im + sn * re;
-                re = br;
-                im = bi;
+                cs = phase[2 * vid];
+                sn = phase[2 * vid + 1];
             }
 #pragma unroll
-            for (int w = 0; w < MM; ++w) {
-                acc[w][b][0] += pw[w] * re;
-                if constexpr (NC == 2) acc[w][b][1] += pw[w] * im;
+            for (int jj = 0; jj < 16; jj += NC) {
+                const int col = tile * 16 + jj;
+                T re = T(0), im = T(0);
+                if (in && col < ncol) {
+                    const T *p = psi + (int64_t)col * ng;  // column col is band col / NC, component 0
+                    re = p[gi];
+                    if constexpr (NC == 2) im = p[ng + gi];
+                }
+                if constexpr (NC == 2) {
+                    if constexpr (PH) {
+                        stage[wave][lane][jj] = cs * re - sn * im;
+                        stage[wave][lane][jj + 1] = cs * im + sn * re;
+                    } else {
+                        stage[wave][lane][jj] = re;
+                        stage[wave][lane][jj + 1] = im;
+                    }
+                } else {
+                    stage[wave][lane][jj] = re;
+                }
             }
         }
+        __syncthreads();
+#pragma unroll 4
+        for (int s = 0; s < PJ_SUB / 4; ++s) {
+            const int vl = 4 * s + g;
+            const int v = vb + sc * PJ_SUB + vl;
+            const T av = v < V ? tab16[(vg0 + v) * kMaxProj + j] : T(0);  // A[w = j][k = g]
+            acc = M::mma(av, stage[wave][vl][j], acc);                    // B[k = g][col = j]
+        }
+        __syncthreads();
     }
-
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
-    for (int w = 0; w < MM; ++w)
-#pragma unroll
-        for (int b = 0; b < NBG; ++b)
-#pragma unroll
-            for (int c = 0; c < NC; ++c) {
-                const double s = wave_sum<T>((double)acc[w][b][c]);
-                if (lane == 0) red[wid][(w * NBG + b) * NC + c] = s;
-            }
+    for (int r = 0; r < 4; ++r) red[wave][M::row(g, r) * 16 + j] = (double)acc[r];
     __syncthreads();
-    for (int i = threadIdx.x; i < NV; i += NL_THREADS) {
-        const int w = i / (NBG * NC);
-        const int b = (i / NC) % NBG;
-        const int c = i % NC;
-        if (w < M && n0 + b < nb) {
-            double s = 0.0;
-#pragma unroll
-            for (int k = 0; k < NL_THREADS / 64; ++k) s += red[k][i];
-            coef[(int64_t)(proj_off[a] + w) * ncol + (n0 + b) * NC + c] = (T)(s * dV * energy[proj_off[a] + w]);
-        }
+    const int t = threadIdx.x;
+    const int w = t >> 4, jj = t & 15;
+    if (tile * 16 + jj < ncol) {
+        const double sum = (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
+        part[((int64_t)item * kMaxProj + w) * ncol + tile * 16 + jj] = sum;
     }
 }
 
-// Hpsi[g] += sum over (atom, voxel) entries covering g of e^{-ik.r} sum_w p_w c_w
-template <typename T, int NC, bool PH>
-__global__ __launch_bounds__(NL_THREADS) void nl_scatter_kernel(
-    T *__restrict__ hpsi, const T *__restrict__ coef, int64_t ng, int nb, int ncov, const int *__restrict__ cov_grid,
-    const int *__restrict__ cov_off, const int *__restrict__ cov_vox, const int *__restrict__ vox_atom,
-    const int *__restrict__ nproj, const int *__restrict__ proj_off, const int64_t *__restrict__ vox_off,
-    const int64_t *__restrict__ tab_off, const T *__restrict__ table, const T *__restrict__ phase) {
-    const int i = blockIdx.x * NL_THREADS + threadIdx.x;
-    const int n = blockIdx.y;
-    if (i >= ncov) return;
-    const int ncol = nb * NC;
-    const int g = cov_grid[i];
-    T sr = T(0), si = T(0);
-    for (int e = cov_off[i]; e < cov_off[i + 1]; ++e) {
-        const int vid = cov_vox[e];
-        const int a = vox_atom[vid];
-        const int64_t va = vox_off[a];
-        const int V = (int)(vox_off[a + 1] - va);
-        const int v = (int)(vid - va);
-        const int M = nproj[a];
-        const T *tab = table + tab_off[a] + v;
-        const T *cf = coef + (int64_t)proj_off[a] * ncol + n * NC;
-        T tr = T(0), ti = T(0);
-        for (int w = 0; w < M; ++w) {
-            const T pv = tab[(int64_t)w * V];
-            tr += pv * cf[(int64_t)w * ncol];
-            if constexpr (NC == 2) ti += pv * cf[(int64_t)w * ncol + 1];
-        }
-        if constexpr (PH) {
-            const T cs = phase[2 * (int64_t)vid], sn = phase[2 * (int64_t)vid + 1];
-            sr += cs * tr + sn * ti;
-            si += cs * ti - sn * tr;
-        } else {
-            sr += tr;
-            si += ti;
+// coef[p * ncol + col] = E_p dV sum over the atom's chunks (fixed order, FP64)
+template <typename T>
+__global__ void nl_coef_kernel(T *__restrict__ coef, const double *__restrict__ part, int ncol, int ptot,
+                               const int *__restrict__ proj_atom, const int *__restrict__ proj_off,
+                               const int *__restrict__ atom_chunk_off, const double *__restrict__ energy, double dV) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (int64_t)ptot * ncol) return;
+    const int p = (int)(e / ncol), col = (int)(e % ncol);
+    const int a = proj_atom[p];
+    const int w = p - proj_off[a];
+    double s = 0.0;
+    for (int it = atom_chunk_off[a]; it < atom_chunk_off[a + 1]; ++it)
+        s += part[((int64_t)it * kMaxProj + w) * ncol + col];
+    coef[e] = (T)(s * dV * energy[p]);
+}
+
+// Hpsi[col][g] += sum over (atom, voxel) entries covering g of (e^{-ik.r} sum_w p_w c_w)_col
+// for NBS real columns per thread (NBS a multiple of NC).  The coefficients of
+// the (up to kScatterSlots) atoms a workgroup touches are staged in LDS.
+template <typename T, int NC, bool PH, int NBS>
+__global__ __launch_bounds__(kScatterBlock) void nl_scatter_kernel(
+    T *__restrict__ hpsi, const T *__restrict__ coef, int64_t ng, int ncol, int ncov,
+    const int *__restrict__ cov_grid, const int *__restrict__ cov_off, const int *__restrict__ cov_vox,
+    const int *__restrict__ cov_slot, const int *__restrict__ block_atom_off, const int *__restrict__ block_atoms,
+    const int *__restrict__ vox_atom, const int *__restrict__ nproj, const int *__restrict__ proj_off,
+    const T *__restrict__ tab16, const T *__restrict__ phase) {
+    __shared__ T cf_s[kScatterSlots][kMaxProj][NBS];
+    const int c0 = blockIdx.y * NBS;
+    {
+        const int b0 = block_atom_off[blockIdx.x], nat = block_atom_off[blockIdx.x + 1] - b0;
+        for (int e = threadIdx.x; e < nat * kMaxProj * NBS; e += kScatterBlock) {
+            const int sl = e / (kMaxProj * NBS), w = (e / NBS) % kMaxProj, q = e % NBS;
+            const int a = block_atoms[b0 + sl];
+            cf_s[sl][w][q] = (w < nproj[a] && c0 + q < ncol) ? coef[(int64_t)(proj_off[a] + w) * ncol + c0 + q] : T(0);
         }
     }
-    hpsi[(int64_t)n * NC * ng + g] += sr;
-    if constexpr (NC == 2) hpsi[((int64_t)n * NC + 1) * ng + g] += si;
+    __syncthreads();
+    const int i = blockIdx.x * kScatterBlock + threadIdx.x;
+    if (i >= ncov) return;
+    const int g = cov_grid[i];
+    T acc[NBS];
+#pragma unroll
+    for (int q = 0; q < NBS; ++q) acc[q] = T(0);
+    for (int e = cov_off[i]; e < cov_off[i + 1]; ++e) {
+        const int vid = cov_vox[e];
+        const int sl = cov_slot[e];
+        T pw[kMaxProj];
+#pragma unroll
+        for (int w = 0; w < kMaxProj; ++w) pw[w] = tab16[(int64_t)vid * kMaxProj + w];
+        T cs = T(1), sn = T(0);
+        if constexpr (PH) {
+            cs = phase[2 * (int64_t)vid];
+            sn = phase[2 * (int64_t)vid + 1];
+        }
+        if (sl >= 0) {
+#pragma unroll
+            for (int q = 0; q < NBS; q += NC) {
+                T tr = T(0), ti = T(0);
+#pragma unroll
+                for (int w = 0; w < kMaxProj; ++w) {
+                    tr += pw[w] * cf_s[sl][w][q];
+                    if constexpr (NC == 2) ti += pw[w] * cf_s[sl][w][q + 1];
+                }
+                if constexpr (PH) {
+                    acc[q] += cs * tr + sn * ti;
+                    acc[q + 1] += cs * ti - sn * tr;
+                } else {
+                    acc[q] += tr;
+                    if constexpr (NC == 2) acc[q + 1] += ti;
+                }
+            }
+        } else {  // more atoms in this workgroup than LDS slots: coefficients from global memory
+            const int a = vox_atom[vid];
+            const int M = nproj[a];
+            const T *cf = coef + (int64_t)proj_off[a] * ncol + c0;
+            for (int q = 0; q < NBS; q += NC) {
+                if (c0 + q >= ncol) break;
+                T tr = T(0), ti = T(0);
+                for (int w = 0; w < M; ++w) {
+                    tr += pw[w] * cf[(int64_t)w * ncol + q];
+                    if constexpr (NC == 2) ti += pw[w] * cf[(int64_t)w * ncol + q + 1];
+                }
+                if constexpr (PH) {
+                    acc[q] += cs * tr + sn * ti;
+                    acc[q + 1] += cs * ti - sn * tr;
+                } else {
+                    acc[q] += tr;
+                    if constexpr (NC == 2) acc[q + 1] += ti;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NBS; ++q)
+        if (c0 + q < ncol) hpsi[(int64_t)(c0 + q) * ng + g] += acc[q];
 }
 
 // Per (atom, band group): E_nl and F = 2 sum occ E Re(conj(<p|psi>) <dp/dr|psi>).
@@ -278,54 +336,51 @@ const double *phase_of<double>(const KPoint &k) { return k.d_phase64; }
 template <>
 const float *phase_of<float>(const KPoint &k) { return k.d_phase32; }
 
-template <typename T, int MM, int NC, bool PH>
-void launch_project(qm_ctx *ctx, T *coef, const T *psi, int nb, const T *phase, hipStream_t st) {
-    constexpr int NBG = (NC == 1) ? 8 : 4;
-    const Projectors &p = ctx->proj;
-    dim3 grid(p.natoms, (nb + NBG - 1) / NBG);
-    hipLaunchKernelGGL((nl_project_kernel<T, MM, NBG, NC, PH>), grid, dim3(NL_THREADS), 0, st, coef, psi, ctx->ng,
-                       nb, p.d_atom_order, p.d_nproj, p.d_proj_off, p.d_vox_off, p.d_tab_off, p.d_vox_grid,
-                       table_of<T>(p), phase, p.d_energy, ctx->hx * ctx->hy * ctx->hz);
-}
+template <typename T>
+const T *tab16_of(const Projectors &p);
+template <>
+const double *tab16_of<double>(const Projectors &p) { return p.d_tab16_64; }
+template <>
+const float *tab16_of<float>(const Projectors &p) { return p.d_tab16_32; }
 
 template <typename T, int NC, bool PH>
-void launch_scatter(qm_ctx *ctx, T *hpsi, const T *coef, int nb, const T *phase, hipStream_t st) {
+int nonlocal_apply(qm_ctx *ctx, T *hpsi, const T *psi, int nb, const T *phase, hipStream_t st) {
     const Projectors &p = ctx->proj;
-    dim3 grid((p.ncov + NL_THREADS - 1) / NL_THREADS, nb);
-    hipLaunchKernelGGL((nl_scatter_kernel<T, NC, PH>), grid, dim3(NL_THREADS), 0, st, hpsi, coef, ctx->ng, nb, p.ncov,
-                       p.d_cov_grid, p.d_cov_off, p.d_cov_vox, p.d_vox_atom, p.d_nproj, p.d_proj_off, p.d_vox_off,
-                       p.d_tab_off, table_of<T>(p), phase);
+    const int ncol = nb * NC;
+    QM_TRY(ctx->coef.reserve(sizeof(T) * (size_t)p.ptot * ncol));
+    QM_TRY(ctx->nl_part.reserve(sizeof(double) * (size_t)p.nchunk * kMaxProj * ncol));
+    T *coef = ctx->coef.as<T>();
+    const int tiles = (ncol + 15) / 16;
+    QM_CHECK(tiles <= 65535, QM_ERR_ARG, "nonlocal apply: too many bands per call");
+    hipLaunchKernelGGL((nl_project_kernel<T, NC, PH>), dim3(p.nchunk, tiles), dim3(256), 0, st,
+                       ctx->nl_part.as<double>(), psi, ctx->ng, ncol, p.d_chunk_atom, p.d_chunk_v0, p.d_vox_off,
+                       p.d_vox_grid, tab16_of<T>(p), phase);
+    QM_HIP(hipGetLastError());
+    const int64_t ncoef = (int64_t)p.ptot * ncol;
+    hipLaunchKernelGGL((nl_coef_kernel<T>), dim3((unsigned)((ncoef + 255) / 256)), dim3(256), 0, st, coef,
+                       ctx->nl_part.as<double>(), ncol, p.ptot, p.d_proj_atom, p.d_proj_off, p.d_atom_chunk_off,
+                       p.d_energy, ctx->hx * ctx->hy * ctx->hz);
+    QM_HIP(hipGetLastError());
+    constexpr int NBS = 16;
+    const int stiles = (ncol + NBS - 1) / NBS;
+    hipLaunchKernelGGL((nl_scatter_kernel<T, NC, PH, NBS>), dim3(p.nsblock, stiles), dim3(kScatterBlock), 0, st, hpsi,
+                       coef, ctx->ng, ncol, p.ncov, p.d_cov_grid, p.d_cov_off, p.d_cov_vox, p.d_cov_slot,
+                       p.d_block_atom_off, p.d_block_atoms, p.d_vox_atom, p.d_nproj, p.d_proj_off, tab16_of<T>(p),
+                       phase);
+    QM_HIP(hipGetLastError());
+    return QM_OK;
 }
 
 template <typename T>
 int launch_nonlocal(qm_ctx *ctx, T *hpsi, const T *psi, int nb, int ncomp, const KPoint &kp, hipStream_t st) {
     const Projectors &p = ctx->proj;
     if (p.natoms == 0 || p.ptot == 0) return QM_OK;
-    QM_CHECK(nb <= 65535, QM_ERR_ARG, "nonlocal apply: at most 65535 bands per call");
-    QM_TRY(ctx->coef.reserve(sizeof(T) * (size_t)p.ptot * nb * ncomp));
-    T *coef = ctx->coef.as<T>();
     const bool ph = !kp.gamma;
     QM_CHECK(ncomp == 2 || !ph, QM_ERR_ARG, "real (ncomp=1) bands need a Gamma k-point");
     const T *phase = phase_of<T>(kp);
-    const bool small = p.max_m <= 9;
-    if (ncomp == 1) {
-        if (small) launch_project<T, 9, 1, false>(ctx, coef, psi, nb, phase, st);
-        else launch_project<T, 16, 1, false>(ctx, coef, psi, nb, phase, st);
-        QM_HIP(hipGetLastError());
-        launch_scatter<T, 1, false>(ctx, hpsi, coef, nb, phase, st);
-    } else if (!ph) {
-        if (small) launch_project<T, 9, 2, false>(ctx, coef, psi, nb, phase, st);
-        else launch_project<T, 16, 2, false>(ctx, coef, psi, nb, phase, st);
-        QM_HIP(hipGetLastError());
-        launch_scatter<T, 2, false>(ctx, hpsi, coef, nb, phase, st);
-    } else {
-        if (small) launch_project<T, 9, 2, true>(ctx, coef, psi, nb, phase, st);
-        else launch_project<T, 16, 2, true>(ctx, coef, psi, nb, phase, st);
-        QM_HIP(hipGetLastError());
-        launch_scatter<T, 2, true>(ctx, hpsi, coef, nb, phase, st);
-    }
-    QM_HIP(hipGetLastError());
-    return QM_OK;
+    if (ncomp == 1) return nonlocal_apply<T, 1, false>(ctx, hpsi, psi, nb, phase, st);
+    if (!ph) return nonlocal_apply<T, 2, false>(ctx, hpsi, psi, nb, phase, st);
+    return nonlocal_apply<T, 2, true>(ctx, hpsi, psi, nb, phase, st);
 }
 
 template <typename T, int MM, int NC, bool PH>

@@ -80,7 +80,24 @@ struct Projectors {
     int *d_cov_vox = nullptr;            // [vtot] global voxel ids grouped by covered point
     // atom work list for the projection launch, longest atoms first
     int *d_atom_order = nullptr;
+    // voxel-major tables padded to 16 projectors: tab16[v * 16 + w] (MFMA operand layout)
+    double *d_tab16_64 = nullptr;
+    float *d_tab16_32 = nullptr;
+    // projection work items: chunks of kProjChunk voxels of one atom
+    int nchunk = 0;
+    int *d_chunk_atom = nullptr, *d_chunk_v0 = nullptr;
+    int *d_atom_chunk_off = nullptr;  // [natoms + 1]
+    int *d_proj_atom = nullptr;       // [ptot] owning atom of each projector
+    // scatter blocks of kScatterBlock covered points: the atoms they touch
+    // (block_atom_off / block_atoms) and each entry's slot in that list (-1 if
+    // the block touches more than kScatterSlots atoms)
+    int nsblock = 0;
+    int *d_block_atom_off = nullptr, *d_block_atoms = nullptr, *d_cov_slot = nullptr;
 };
+
+constexpr int kProjChunk = 1024;  // voxels per projection work item (4 waves x 256)
+constexpr int kScatterBlock = 256;  // covered points per scatter workgroup
+constexpr int kScatterSlots = 8;    // atoms whose coefficients a scatter workgroup stages in LDS
 
 struct KPoint {
     double k[3] = {0, 0, 0};
@@ -104,6 +121,7 @@ struct qm_ctx {
     qm::DevBuf coef;     // projector coefficients for the nonlocal apply
     qm::DevBuf fwork;    // force partials
     qm::DevBuf gemm_part; // Gram partial sums of qm_gram_*
+    qm::DevBuf nl_part;   // per-chunk projection partials (FP64)
 };
 
 namespace qm {

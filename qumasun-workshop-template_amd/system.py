@@ -225,6 +225,23 @@ def build_projectors(s: System):
     return s
 
 
+def retabulate(s: System, positions) -> System:
+    """Copy of `s` with the projector (and gradient) tables evaluated at new atom
+    positions on the SAME voxel sets (sphere membership frozen), as a central
+    finite difference of the projector position derivative requires."""
+    t = dataclasses.replace(s, positions=np.array(positions, dtype=np.float64))
+    h = s.h
+    tabs, dtabs = [], []
+    for a in range(s.natoms):
+        vox = s.vox_xyz[s.vox_off[a]:s.vox_off[a + 1]]
+        tab, dtab = projector_values(t.positions[a], int(s.lmax[a]), vox, h)
+        tabs.append(tab.ravel())
+        dtabs.append(dtab.reshape(3, -1))
+    t.table = np.ascontiguousarray(np.concatenate(tabs)) if s.natoms else s.table
+    t.dtable = np.ascontiguousarray(np.concatenate(dtabs, axis=1)) if s.natoms else s.dtable
+    return t
+
+
 # --------------------------------------------------------------------------
 # The five benchmark systems
 

@@ -1,0 +1,45 @@
+"""The C-ABI library loads without a GPU and exports every entry point that
+include/qumasun_hip.h declares (no compute calls here)."""
+import ctypes
+import os
+import re
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            txt = open(os.path.join(ROOT, "include", fn)).read()
+            txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+            names |= set(re.findall(r"\b(qm_[a-z0-9_]+)\s*\(", txt))
+    return names
+
+
+def test_library_exports_every_declared_symbol(qm):
+    lib = ctypes.CDLL(qm._lib.LIB_PATH)
+    declared = _declared()
+    assert declared, "no declarations found"
+    missing = [n for n in sorted(declared) if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_binding_covers_the_header(qm):
+    assert set(qm._lib.EXPORTS) == _declared()
+
+
+def test_load_sets_signatures(qm):
+    L = qm._lib.load()
+    assert L.qm_hpsi_apply_f64.restype == ctypes.c_int
+    assert L.qm_last_error.restype == ctypes.c_char_p
+    assert L.qm_last_error() in (b"", None) or isinstance(L.qm_last_error(), bytes)
+
+
+def test_errors_without_device(qm):
+    """Argument checks run before any device call and report through qm_last_error."""
+    L = qm._lib.load()
+    ctx = ctypes.c_void_p()
+    st = L.qm_create(ctypes.byref(ctx), 2, 8, 8, 0.1, 0.1, 0.1, 0)
+    assert st == -1  # QM_ERR_ARG: grid dimensions must be >= 4
+    assert b"grid" in L.qm_last_error()
