@@ -121,6 +121,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain 512 MiB device copy (the scale for rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--lobpcg-calls", type=int, default=10,
                     help="LOBPCG calls of 20 iterations per precision (0 skips the LOBPCG leg)")
     args = ap.parse_args()
@@ -151,6 +153,14 @@ def main():
     def nonlocal_():
         H.apply(y, x, part="nonlocal")
 
+    if args.copy_calibration:
+        # 128 Mi floats: 512 MiB read + 512 MiB written by one copy kernel
+        a = torch.empty(128 << 20, dtype=torch.float32, device="cuda:0").fill_(1.0)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     for _ in range(args.warmup):
         full()
     torch.cuda.synchronize()
