@@ -175,6 +175,8 @@ def main():
     ms64 = _time_applies(torch, lambda: H.apply(y64, x64), max(5, args.steps // 5))
     del x64, y64
 
+    cand = H.stencil_choice(32, nb)
+    kernel_name = H.stencil_candidates()[cand] if cand >= 0 else "untuned"
     bytes_local = local_bytes(s, nb, 4)
     achieved = bytes_local / (ms_local * 1e-3) / 1e9
     out = {
@@ -194,7 +196,7 @@ def main():
                                f"real bands, FD8 + Gaussian V_loc + KB l<=2 projectors (r_c 2.5)",
                    "grid": [s.nx, s.ny, s.nz], "bands": nb, "atoms": s.natoms,
                    "projector_voxels": int(s.vox_off[-1])},
-        "roofline": {"bound": "hbm", "kernel": "fd8_local_kernel<float,1,false>",
+        "roofline": {"bound": "hbm", "kernel": kernel_name,
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None, "bytes_per_launch": bytes_local, "ms_per_launch": ms_local},
         "parts_ms": {"local_fd8_v": ms_local, "nonlocal": ms_nl, "full": ms},

@@ -96,6 +96,25 @@ int qm_local_apply_f32(qm_ctx *ctx, float *d_out, const float *d_psi, int nb, in
 int qm_nonlocal_add_f64(qm_ctx *ctx, double *d_out, const double *d_psi, int nb, int ncomp, int ik, void *stream);
 int qm_nonlocal_add_f32(qm_ctx *ctx, float *d_out, const float *d_psi, int nb, int ncomp, int ik, void *stream);
 
+/* ------------------------------------------------------ stencil autotuner */
+
+/* Tile-shape autotuner of the fused stencil (no counterpart in the reference,
+ * whose CPU loops have one shape).  The first qm_hpsi_apply_* / qm_local_apply_*
+ * of a (precision, ncomp, k type, nb) shape on a context times every candidate
+ * on the caller's buffers and keeps the fastest; it synchronises the stream
+ * once.  Every candidate computes each point with the same arithmetic, so the
+ * choice never changes a result.  Under stream capture an untuned shape uses
+ * candidate 0.
+ * qm_stencil_num_candidates  number of candidates;
+ * qm_stencil_candidate_name  its description (NUL-terminated, truncated to buflen);
+ * qm_stencil_force           use `candidate` for every later apply, -1 = autotune;
+ * qm_stencil_choice          candidate in use for that shape, -1 if not tuned yet
+ *                            (kt = 1 for a k-point with |k_d| >= 1e-8 in some d). */
+int qm_stencil_num_candidates(void);
+int qm_stencil_candidate_name(int candidate, char *buf, int buflen);
+int qm_stencil_force(qm_ctx *ctx, int candidate);
+int qm_stencil_choice(qm_ctx *ctx, int precision, int ncomp, int kt, int nb, int *candidate);
+
 /* ---------------------------------------------------------------- adjoint */
 
 /* d_rho[i] += sum_n occ[n] |psi_n(i)|^2 (FP64 accumulation).  dE_band/dV_loc(i) is
@@ -135,6 +154,17 @@ int qm_update_f64(qm_ctx *ctx, const double *d_Z, int m, const double *d_C, int 
                   void *stream);
 int qm_update_f32(qm_ctx *ctx, const float *d_Z, int m, const float *d_C, int n, int64_t k, float *d_Y,
                   void *stream);
+
+/* Generalized symmetric eigenproblem A z = l B z on the GPU in FP64, in place of
+ * lapack_DSYGVD(UPLO='U', A, B, eigen_values, eigen_vectors, triangle_factor, N)
+ * (w_dsygvd.h:14-72; the Cholesky route of lapack_ZHEGVD_emu, w_zhegvd.h:227-272).
+ * d_A, d_B: n x n column-major, upper triangles referenced, not modified.
+ * d_w: all n eigenvalues, ascending.  d_Z: n x nev, eigenvectors of the nev
+ * lowest, B-orthonormal.  d_U: n x n, upper triangle receives the Cholesky
+ * factor of B (strictly lower part unspecified).  n <= 1024.  Synchronises the
+ * stream.  QM_ERR_NUMERIC when B is not positive definite. */
+int qm_sygvd_f64(qm_ctx *ctx, int n, const double *d_A, const double *d_B, double *d_w, double *d_Z, double *d_U,
+                 int nev, void *stream);
 
 /* ---------------------------------------------------------------- LOBPCG */
 

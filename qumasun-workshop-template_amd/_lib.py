@@ -19,6 +19,8 @@ EXPORTS = (
     "qm_nonlocal_forces_f64", "qm_nonlocal_forces_f32", "qm_gram_f64", "qm_gram_f32", "qm_update_f64",
     "qm_update_f32", "qm_lobpcg_create", "qm_lobpcg_destroy",
     "qm_lobpcg_solve", "qm_lobpcg_set_conditioning", "qm_lobpcg_fallbacks", "qm_lobpcg_timings",
+    "qm_stencil_num_candidates", "qm_stencil_candidate_name", "qm_stencil_force", "qm_stencil_choice",
+    "qm_sygvd_f64",
 )
 
 QM_OK = 0
@@ -68,6 +70,11 @@ def load():
     L.qm_lobpcg_timings.argtypes = [P, P]
     L.qm_lobpcg_set_conditioning.argtypes = [P, D, D]
     L.qm_lobpcg_fallbacks.argtypes = [P]
+    L.qm_stencil_num_candidates.argtypes = []
+    L.qm_stencil_candidate_name.argtypes = [I, ctypes.c_char_p, I]
+    L.qm_stencil_force.argtypes = [P, I]
+    L.qm_stencil_choice.argtypes = [P, I, I, I, I, ctypes.POINTER(I)]
+    L.qm_sygvd_f64.argtypes = [P, I, P, P, P, P, P, I, P]
     for n in EXPORTS:
         if n not in ("qm_last_error", "qm_destroy", "qm_lobpcg_destroy"):
             getattr(L, n).restype = I

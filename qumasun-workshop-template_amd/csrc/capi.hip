@@ -5,6 +5,8 @@
 #include <cstring>
 #include <numeric>
 
+#include <rocblas/rocblas.h>
+
 #include "qm_internal.h"
 
 namespace qm {
@@ -254,6 +256,8 @@ void qm_destroy(qm_ctx *ctx) {
     ctx->fwork.release();
     ctx->gemm_part.release();
     ctx->nl_part.release();
+    ctx->eig_work.release();
+    if (ctx->blas) rocblas_destroy_handle((rocblas_handle)ctx->blas);
     delete ctx;
 }
 
@@ -496,6 +500,35 @@ int qm_nonlocal_forces_f64(qm_ctx *ctx, const double *d_psi, const double *occ, 
 int qm_nonlocal_forces_f32(qm_ctx *ctx, const float *d_psi, const double *occ, int nb, int ncomp, int ik,
                            double *forces_out, double *e_nl, void *stream) {
     return forces<float>(ctx, d_psi, occ, nb, ncomp, ik, forces_out, e_nl, stream);
+}
+
+int qm_stencil_num_candidates(void) { return stencil_num_candidates(); }
+
+int qm_stencil_candidate_name(int candidate, char *buf, int buflen) {
+    QM_CHECK(candidate >= 0 && candidate < stencil_num_candidates(), QM_ERR_ARG,
+             "qm_stencil_candidate_name: candidate out of range");
+    QM_CHECK(buf && buflen > 0, QM_ERR_ARG, "qm_stencil_candidate_name: NULL or empty buffer");
+    const std::string name = stencil_candidate_name(candidate);
+    const size_t n = std::min(name.size(), (size_t)buflen - 1);
+    name.copy(buf, n);
+    buf[n] = '\0';
+    return QM_OK;
+}
+
+int qm_stencil_force(qm_ctx *ctx, int candidate) {
+    QM_CHECK(ctx, QM_ERR_ARG, "qm_stencil_force: NULL context");
+    QM_CHECK(candidate >= -1 && candidate < stencil_num_candidates(), QM_ERR_ARG,
+             "qm_stencil_force: candidate must be -1 (autotune) or a candidate index");
+    ctx->stencil_force = candidate;
+    return QM_OK;
+}
+
+int qm_stencil_choice(qm_ctx *ctx, int precision, int ncomp, int kt, int nb, int *candidate) {
+    QM_CHECK(ctx && candidate, QM_ERR_ARG, "qm_stencil_choice: NULL argument");
+    QM_CHECK(precision == 32 || precision == 64, QM_ERR_ARG, "qm_stencil_choice: precision must be 32 or 64");
+    QM_CHECK(ncomp == 1 || ncomp == 2, QM_ERR_ARG, "qm_stencil_choice: ncomp must be 1 or 2");
+    *candidate = stencil_choice(ctx, precision, ncomp, kt, nb);
+    return QM_OK;
 }
 
 }  // extern "C"

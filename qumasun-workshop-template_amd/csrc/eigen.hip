@@ -1,0 +1,343 @@
+// Dense FP64 symmetric eigensolver for the LOBPCG subspace problem.
+//
+// The generalized problem A z = l B z of lapack_DSYGVD (w_dsygvd.h:14-72) and
+// lapack_ZHEGVD_emu (w_zhegvd.h:227-272) is solved by the Cholesky route:
+// B = U^T U, C = U^-T A U^-1, C = Q T Q^T (tridiagonal), T = Y L Y^T, z = U^-1 Q Y.
+//
+// The tridiagonal reduction is this file's own: rocSOLVER's sytrd launches
+// about ten small kernels per column, which at the subspace sizes of LOBPCG
+// (n = 2N or 3N, a few hundred) costs ~15 us per column in launch latency
+// alone.  Here one workgroup of 1024 threads reduces a panel of 32 columns in
+// one launch (the blocked scheme of LAPACK's dlatrd: each column is updated
+// with the panel's earlier reflectors, its Householder vector is formed, and
+// w = tau (A22 v - V W^T v - W V^T v) is accumulated against the trailing
+// matrix as it stood at the panel's start), one rocBLAS GEMM applies the
+// rank-64 update A22 -= V W^T + W V^T, and the last <= 88 columns are reduced
+// unblocked with the trailing block held in LDS.  Output follows LAPACK's
+// lower-storage convention (d, e, tau, reflectors below the subdiagonal), so
+// rocSOLVER's stedc (divide and conquer on T) and ormtr (apply Q) consume it.
+// The matrix is kept symmetric in both triangles, so the matrix-vector
+// product is a plain column sweep.
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <cmath>
+
+#include "qm_internal.h"
+
+namespace qm {
+namespace {
+
+constexpr int kNB = 32;         // panel width
+constexpr int kTailMax = 88;    // columns left for the LDS kernel (88^2 doubles = 61952 bytes)
+constexpr int kThreads = 1024;
+constexpr int kWaves = kThreads / 64;
+constexpr int kMaxN = 1024;     // one thread per row of the matrix-vector product
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Sum over the block in a fixed order; every thread receives the total.
+__device__ double block_sum(double v, double *red) {
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int i = 0; i < kWaves; ++i) t += red[i];
+    __syncthreads();
+    return t;
+}
+
+// Householder reflector of LAPACK's dlarfg for x = xs[0..m-1]: H x = beta e_1,
+// H = I - tau v v^T, v_0 = 1.  Returns tau and beta, overwrites xs with v.
+__device__ void householder(double *xs, int m, double *red, double &tau, double &beta) {
+    const double alpha = xs[0];
+    double loc = 0.0;
+    for (int k = 1 + threadIdx.x; k < m; k += kThreads) loc += xs[k] * xs[k];
+    const double xn2 = block_sum(loc, red);  // all reads of xs[0] happen before its sync
+    double scal = 1.0;
+    if (xn2 == 0.0) {
+        tau = 0.0;
+        beta = alpha;
+    } else {
+        beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+        tau = (beta - alpha) / beta;
+        scal = 1.0 / (alpha - beta);
+    }
+    for (int k = threadIdx.x; k < m; k += kThreads) xs[k] = k == 0 ? 1.0 : xs[k] * scal;
+    __syncthreads();
+}
+
+// Columns i0 .. i0+nb-1 of the symmetric n x n matrix A (column-major, both
+// triangles valid on entry).  VW holds three n x nb blocks [V | W | V]; rows
+// below each reflector's start are written.
+__global__ __launch_bounds__(kThreads) void trd_panel_kernel(double *__restrict__ A, int lda, int n, int i0, int nb,
+                                                             double *__restrict__ d, double *__restrict__ e,
+                                                             double *__restrict__ tau, double *__restrict__ VW,
+                                                             int ldp) {
+    __shared__ double vs[kMaxN];
+    __shared__ double part[kThreads];
+    __shared__ double vrow[2 * kNB];
+    __shared__ double tt[2 * kNB];
+    __shared__ double red[kWaves];
+    const int tid = threadIdx.x;
+    double *V = VW, *W = VW + (size_t)nb * ldp, *V2 = VW + (size_t)2 * nb * ldp;
+    for (int j = 0; j < nb; ++j) {
+        const int c = i0 + j;
+        const int m = n - c - 1;
+        // A(c:n, c) -= V(c:n, :j) W(c, :j)^T + W(c:n, :j) V(c, :j)^T
+        if (tid < j) {
+            vrow[tid] = V[c + (size_t)tid * ldp];
+            vrow[kNB + tid] = W[c + (size_t)tid * ldp];
+        }
+        __syncthreads();
+        for (int r = c + tid; r < n; r += kThreads) {
+            double a = A[r + (size_t)c * lda];
+            for (int q = 0; q < j; ++q) {
+                a = fma(-V[r + (size_t)q * ldp], vrow[kNB + q], a);
+                a = fma(-W[r + (size_t)q * ldp], vrow[q], a);
+            }
+            if (r == c) d[c] = a;
+            else vs[r - c - 1] = a;
+        }
+        __syncthreads();
+        double t, beta;
+        householder(vs, m, red, t, beta);
+        for (int k = tid; k < m; k += kThreads) {
+            const double v = vs[k];
+            V[c + 1 + k + (size_t)j * ldp] = v;
+            V2[c + 1 + k + (size_t)j * ldp] = v;
+            A[c + 1 + k + (size_t)c * lda] = k == 0 ? beta : v;
+        }
+        if (tid == 0) {
+            e[c] = beta;
+            tau[c] = t;
+        }
+        // w = A22 v over G column groups, A22 = A(c+1:n, c+1:n) as at the panel start
+        const int mp = (m + 63) & ~63;
+        const int G = max(1, min(16, kThreads / mp));
+        {
+            const int r = tid % mp, g = tid / mp;
+            if (g < G && r < m) {
+                const int k0 = (int)((int64_t)m * g / G), k1 = (int)((int64_t)m * (g + 1) / G);
+                const double *col = A + (c + 1 + r) + (size_t)(c + 1) * lda;
+                double acc = 0.0;
+                int k = k0;
+                for (; k + 3 < k1; k += 4) {
+                    const double a0 = col[(size_t)k * lda], a1 = col[(size_t)(k + 1) * lda];
+                    const double a2 = col[(size_t)(k + 2) * lda], a3 = col[(size_t)(k + 3) * lda];
+                    acc = fma(a0, vs[k], acc);
+                    acc = fma(a1, vs[k + 1], acc);
+                    acc = fma(a2, vs[k + 2], acc);
+                    acc = fma(a3, vs[k + 3], acc);
+                }
+                for (; k < k1; ++k) acc = fma(col[(size_t)k * lda], vs[k], acc);
+                part[g * mp + r] = acc;
+            }
+        }
+        // t1 = W(c+1:n, :j)^T v, t2 = V(c+1:n, :j)^T v: one wave per dot product
+        {
+            const int wv = tid >> 6, ln = tid & 63;
+            for (int q2 = wv; q2 < 2 * j; q2 += kWaves) {
+                const double *col = (q2 < j ? W + (size_t)q2 * ldp : V + (size_t)(q2 - j) * ldp) + c + 1;
+                double acc = 0.0;
+                for (int k = ln; k < m; k += 64) acc = fma(col[k], vs[k], acc);
+                acc = wave_sum(acc);
+                if (ln == 0) tt[q2] = acc;
+            }
+        }
+        __syncthreads();
+        double loc = 0.0;
+        for (int r = tid; r < m; r += kThreads) {
+            double w = 0.0;
+            for (int g = 0; g < G; ++g) w += part[g * mp + r];
+            for (int q = 0; q < j; ++q) {
+                w = fma(-V[c + 1 + r + (size_t)q * ldp], tt[q], w);
+                w = fma(-W[c + 1 + r + (size_t)q * ldp], tt[j + q], w);
+            }
+            w *= t;
+            part[r] = w;  // only this thread reads part[g * mp + r]
+            loc = fma(w, vs[r], loc);
+        }
+        const double alpha = -0.5 * t * block_sum(loc, red);
+        for (int r = tid; r < m; r += kThreads) W[c + 1 + r + (size_t)j * ldp] = fma(alpha, vs[r], part[r]);
+        __syncthreads();
+    }
+}
+
+// Unblocked reduction of the trailing block A(it:n, it:n), nt = n - it <= kTailMax,
+// held in LDS (LAPACK's dsytd2, lower).
+__global__ __launch_bounds__(kThreads) void trd_tail_kernel(double *__restrict__ A, int lda, int n, int it,
+                                                            double *__restrict__ d, double *__restrict__ e,
+                                                            double *__restrict__ tau) {
+    __shared__ double L[kTailMax * kTailMax];
+    __shared__ double vs[kTailMax], ws[kTailMax];
+    __shared__ double red[kWaves];
+    const int tid = threadIdx.x;
+    const int nt = n - it;
+    for (int idx = tid; idx < nt * nt; idx += kThreads) {
+        const int r = idx % nt, k = idx / nt;
+        L[idx] = A[(it + r) + (size_t)(it + k) * lda];
+    }
+    __syncthreads();
+    for (int j = 0; j < nt - 1; ++j) {
+        const int c = it + j;
+        const int m = nt - j - 1;
+        for (int k = tid; k < m; k += kThreads) vs[k] = L[(j + 1 + k) + j * nt];
+        if (tid == 0) d[c] = L[j + j * nt];
+        __syncthreads();
+        double t, beta;
+        householder(vs, m, red, t, beta);
+        for (int k = tid; k < m; k += kThreads) A[c + 1 + k + (size_t)c * lda] = k == 0 ? beta : vs[k];
+        if (tid == 0) {
+            e[c] = beta;
+            tau[c] = t;
+        }
+        // w = tau A22 v, alpha = -1/2 tau w.v, w += alpha v, A22 -= v w^T + w v^T
+        double loc = 0.0;
+        for (int r = tid; r < m; r += kThreads) {
+            double acc = 0.0;
+            for (int k = 0; k < m; ++k) acc = fma(L[(j + 1 + r) + (j + 1 + k) * nt], vs[k], acc);
+            acc *= t;
+            ws[r] = acc;
+            loc = fma(acc, vs[r], loc);
+        }
+        const double alpha = -0.5 * t * block_sum(loc, red);
+        for (int r = tid; r < m; r += kThreads) ws[r] = fma(alpha, vs[r], ws[r]);
+        __syncthreads();
+        for (int idx = tid; idx < m * m; idx += kThreads) {
+            const int r = idx % m, k = idx / m;
+            double &a = L[(j + 1 + r) + (j + 1 + k) * nt];
+            a = fma(-vs[r], ws[k], a);
+            a = fma(-ws[r], vs[k], a);
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        d[n - 1] = L[(nt - 1) + (nt - 1) * nt];
+        tau[n - 1] = 0.0;
+        e[n - 1] = 0.0;
+    }
+}
+
+#define QM_EIG_BLAS(call)                                                                                   \
+    do {                                                                                                    \
+        rocblas_status s_ = (call);                                                                         \
+        if (s_ != rocblas_status_success) {                                                                 \
+            ::qm::set_error(std::string(#call) + " failed with rocblas status " + std::to_string((int)s_)); \
+            return QM_ERR_HIP;                                                                              \
+        }                                                                                                   \
+    } while (0)
+
+}  // namespace
+
+// C = Q T Q^T for the symmetric n x n matrix C (column-major, lda >= n, both
+// triangles; overwritten with the reflectors).  trd holds d, e, tau (3 n);
+// VW is n x 3*32 workspace.  Runs on the handle's stream.
+int sytrd_lower(void *blas, int n, double *C, int lda, double *trd, double *VW, hipStream_t st) {
+    rocblas_handle h = (rocblas_handle)blas;
+    static_assert(3 * kNB == kTrdPanelCols, "VW workspace width");
+    QM_CHECK(n >= 1 && n <= kMaxN, QM_ERR_ARG, "tridiagonal reduction: n must be in [1, 1024]");
+    double *d = trd, *e = trd + n, *tau = trd + 2 * n;
+    int i0 = 0;
+    const double minus1 = -1.0, one = 1.0;
+    while (n - i0 > kTailMax) {
+        hipLaunchKernelGGL(trd_panel_kernel, dim3(1), dim3(kThreads), 0, st, C, lda, n, i0, kNB, d, e, tau, VW, n);
+        QM_HIP(hipGetLastError());
+        const int m2 = n - i0 - kNB;
+        QM_EIG_BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, m2, m2, 2 * kNB, &minus1,
+                                  VW + i0 + kNB, n, VW + (size_t)kNB * n + i0 + kNB, n, &one,
+                                  C + (i0 + kNB) + (size_t)(i0 + kNB) * lda, lda));
+        i0 += kNB;
+    }
+    hipLaunchKernelGGL(trd_tail_kernel, dim3(1), dim3(kThreads), 0, st, C, lda, n, i0, d, e, tau);
+    QM_HIP(hipGetLastError());
+    return QM_OK;
+}
+
+// Eigenvalues (all, ascending, into w) and the eigenvectors of the nev lowest
+// into Z[:, :nev] (ldz >= n) of the symmetric C (overwritten).  Z must hold
+// n x n for stedc.  info receives rocSOLVER's stedc status.
+int syev_lower(void *blas, int n, int nev, double *C, int lda, double *w, double *Z, int ldz, double *trd,
+               double *VW, int *info, hipStream_t st) {
+    rocblas_handle h = (rocblas_handle)blas;
+    QM_TRY(sytrd_lower(blas, n, C, lda, trd, VW, st));
+    QM_HIP(hipMemcpyAsync(w, trd, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    QM_EIG_BLAS(rocsolver_dstedc(h, rocblas_evect_tridiagonal, n, w, trd + n, Z, ldz, info));
+    QM_EIG_BLAS(rocsolver_dormtr(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, n, nev, C, lda,
+                                 trd + 2 * n, Z, ldz));
+    return QM_OK;
+}
+
+int own_eig_max_n() { return kMaxN; }
+
+}  // namespace qm
+
+namespace qm {
+namespace {
+
+// C = A with the upper triangle mirrored into the lower one
+__global__ void symmetrize_upper_kernel(double *__restrict__ C, const double *__restrict__ A, int n) {
+    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= (int64_t)n * n) return;
+    const int r = (int)(idx % n), k = (int)(idx / n);
+    C[idx] = r <= k ? A[idx] : A[k + (int64_t)r * n];
+}
+
+}  // namespace
+}  // namespace qm
+
+using namespace qm;
+
+extern "C" {
+
+int qm_sygvd_f64(qm_ctx *ctx, int n, const double *d_A, const double *d_B, double *d_w, double *d_Z, double *d_U,
+                 int nev, void *stream) {
+    QM_CHECK(ctx && d_A && d_B && d_w && d_Z && d_U, QM_ERR_ARG, "qm_sygvd_f64: NULL argument");
+    QM_CHECK(n >= 1 && n <= kMaxN, QM_ERR_ARG, "qm_sygvd_f64: n must be in [1, 1024]");
+    QM_CHECK(nev >= 1 && nev <= n, QM_ERR_ARG, "qm_sygvd_f64: nev must be in [1, n]");
+    QM_TRY(activate(ctx));
+    hipStream_t st = (hipStream_t)stream;
+    if (!ctx->blas) {
+        rocblas_handle h = nullptr;
+        QM_CHECK(rocblas_create_handle(&h) == rocblas_status_success, QM_ERR_HIP,
+                 "qm_sygvd_f64: rocblas_create_handle failed");
+        ctx->blas = h;
+    }
+    rocblas_handle h = (rocblas_handle)ctx->blas;
+    QM_EIG_BLAS(rocblas_set_stream(h, st));
+    // workspace: C (n^2), Z (n^2), d e tau (3n), VW (96 n), info
+    const size_t nn = (size_t)n * n;
+    const size_t need = sizeof(double) * (2 * nn + 3 * (size_t)n + (size_t)kTrdPanelCols * n) + 256;
+    QM_TRY(ctx->eig_work.reserve(need));
+    double *C = ctx->eig_work.as<double>(), *Z = C + nn, *trd = Z + nn, *VW = trd + 3 * n;
+    int *info = (int *)(VW + (size_t)kTrdPanelCols * n);
+    hipLaunchKernelGGL(symmetrize_upper_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, C, d_A, n);
+    QM_HIP(hipGetLastError());
+    QM_HIP(hipMemcpyAsync(d_U, d_B, sizeof(double) * nn, hipMemcpyDeviceToDevice, st));
+    QM_EIG_BLAS(rocsolver_dpotrf(h, rocblas_fill_upper, n, d_U, n, info));
+    int h_info = 0;
+    QM_HIP(hipMemcpyAsync(&h_info, info, sizeof(int), hipMemcpyDeviceToHost, st));
+    QM_HIP(hipStreamSynchronize(st));
+    QM_CHECK(h_info == 0, QM_ERR_NUMERIC,
+             "qm_sygvd_f64: B is not positive definite (leading minor " + std::to_string(h_info) + ")");
+    const double one = 1.0;
+    QM_EIG_BLAS(rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_transpose,
+                              rocblas_diagonal_non_unit, n, n, &one, d_U, n, C, n));
+    QM_EIG_BLAS(rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                              rocblas_diagonal_non_unit, n, n, &one, d_U, n, C, n));
+    QM_TRY(syev_lower(h, n, nev, C, n, d_w, Z, n, trd, VW, info, st));
+    QM_EIG_BLAS(rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
+                              rocblas_diagonal_non_unit, n, nev, &one, d_U, n, Z, n));
+    QM_HIP(hipMemcpyAsync(d_Z, Z, sizeof(double) * (size_t)n * nev, hipMemcpyDeviceToDevice, st));
+    QM_HIP(hipMemcpyAsync(&h_info, info, sizeof(int), hipMemcpyDeviceToHost, st));
+    QM_HIP(hipStreamSynchronize(st));
+    QM_CHECK(h_info == 0, QM_ERR_NUMERIC,
+             "qm_sygvd_f64: tridiagonal eigensolve failed (rocSOLVER stedc info " + std::to_string(h_info) + ")");
+    return QM_OK;
+}
+
+}  // extern "C"

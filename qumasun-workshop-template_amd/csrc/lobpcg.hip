@@ -381,6 +381,8 @@ struct qm_lobpcg {
     double *evals = nullptr, *ework = nullptr, *eps = nullptr, *dr = nullptr, *dp = nullptr, *rnorm = nullptr,
            *xnorm = nullptr, *svals = nullptr, *svw = nullptr;
     int *info = nullptr;
+    double *trd = nullptr, *VW = nullptr;  // tridiagonal reduction workspace (real subspace problems)
+    bool evec_t1 = false;                  // last chol_solve left the eigenvectors in T1 (else Hw)
     qm::DevBuf gram_part, red_part;
     rocblas_handle blas = nullptr;
     double t[6] = {0, 0, 0, 0, 0, 0};
@@ -399,7 +401,7 @@ void lfree(qm_lobpcg *s) {
             *p = nullptr;
         }
     double **dptrs[] = {&s->G1, &s->G2, &s->evals, &s->ework, &s->eps, &s->dr, &s->dp, &s->rnorm, &s->xnorm,
-                        &s->svals, &s->svw, &s->G0, &s->rkeep};
+                        &s->svals, &s->svw, &s->G0, &s->rkeep, &s->trd, &s->VW};
     for (double **p : dptrs)
         if (*p) {
             (void)hipFree(*p);
@@ -479,11 +481,21 @@ int chol_solve(qm_lobpcg *s, int n3, hipStream_t st) {
                               rocblas_diagonal_non_unit, n3, n3, &one, U, n3, C, n3));
         QM_BLAS(rocblas_dtrsm(s->blas, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
                               rocblas_diagonal_non_unit, n3, n3, &one, U, n3, C, n3));
+        if (n3 <= own_eig_max_n()) {
+            // own tridiagonal reduction + rocSOLVER stedc / ormtr; eigenvectors into T1
+            auto *Z = (double *)s->T1;
+            QM_TRY(syev_lower(s->blas, n3, N, C, n3, s->evals, Z, n3, s->trd, s->VW, s->info, st));
+            QM_BLAS(rocblas_dtrsm(s->blas, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
+                                  rocblas_diagonal_non_unit, n3, N, &one, U, n3, Z, n3));
+            s->evec_t1 = true;
+            return QM_OK;
+        }
         QM_BLAS(rocsolver_dsyevd(s->blas, rocblas_evect_original, rocblas_fill_upper, n3, C, n3, s->evals, s->ework,
                                  s->info));
         QM_BLAS(rocblas_dtrsm(s->blas, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
                               rocblas_diagonal_non_unit, n3, N, &one, U, n3, C, n3));
     }
+    s->evec_t1 = false;
     return QM_OK;
 }
 
@@ -787,7 +799,7 @@ int solve_impl(qm_lobpcg *s, T *x_user, double *eigenvalues, double *residual_no
                 QM_HIP(hipStreamSynchronize(st));
                 QM_CHECK(h_info == 0, QM_ERR_NUMERIC,
                          "qm_lobpcg_solve: subspace eigensolve failed (rocSOLVER info " + std::to_string(h_info) + ")");
-                evec = s->Hw;
+                evec = s->evec_t1 ? s->T1 : s->Hw;
                 break;
             }
             if (nbk_used == 3) {  // drop P and retry
@@ -976,6 +988,8 @@ int qm_lobpcg_create(qm_lobpcg **out, qm_ctx *ctx, int nbands, int ncomp, int pr
     for (double **p : dv)
         if (st == QM_OK) st = dalloc((void **)p, sizeof(double) * n3);
     if (st == QM_OK) st = dalloc((void **)&s->info, sizeof(int));
+    if (st == QM_OK) st = dalloc((void **)&s->trd, sizeof(double) * 3 * (size_t)n3);
+    if (st == QM_OK) st = dalloc((void **)&s->VW, sizeof(double) * (size_t)kTrdPanelCols * n3);
     if (st == QM_OK && rocblas_create_handle(&s->blas) != rocblas_status_success) {
         set_error("qm_lobpcg_create: rocblas_create_handle failed");
         st = QM_ERR_HIP;

@@ -5,6 +5,7 @@
 
 #include <array>
 #include <cstdint>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -122,6 +123,10 @@ struct qm_ctx {
     qm::DevBuf fwork;    // force partials
     qm::DevBuf gemm_part; // Gram partial sums of qm_gram_*
     qm::DevBuf nl_part;   // per-chunk projection partials (FP64)
+    std::map<int64_t, int> stencil_choice;  // autotuned stencil candidate per shape
+    int stencil_force = -1;                 // >= 0: use this candidate, no tuning
+    void *blas = nullptr;                   // rocblas_handle of qm_sygvd_f64, created on first use
+    qm::DevBuf eig_work;                    // qm_sygvd_f64 workspace
 };
 
 namespace qm {
@@ -129,10 +134,20 @@ namespace qm {
 int activate(const qm_ctx *ctx);
 
 // stencil.hip
-int launch_fd8_local_f64(const qm_ctx *ctx, double *out, const double *in, int nb, int ncomp,
+int launch_fd8_local_f64(qm_ctx *ctx, double *out, const double *in, int nb, int ncomp,
                          const double k[3], hipStream_t st);
-int launch_fd8_local_f32(const qm_ctx *ctx, float *out, const float *in, int nb, int ncomp,
+int launch_fd8_local_f32(qm_ctx *ctx, float *out, const float *in, int nb, int ncomp,
                          const double k[3], hipStream_t st);
+int stencil_num_candidates();
+std::string stencil_candidate_name(int cand);
+int stencil_choice(const qm_ctx *ctx, int precision, int ncomp, int kt, int nb);
+
+// eigen.hip: FP64 symmetric eigensolver pieces (rocblas_handle passed as void *)
+int sytrd_lower(void *blas, int n, double *C, int lda, double *trd, double *VW, hipStream_t st);
+int syev_lower(void *blas, int n, int nev, double *C, int lda, double *w, double *Z, int ldz, double *trd,
+               double *VW, int *info, hipStream_t st);
+int own_eig_max_n();
+constexpr int kTrdPanelCols = 96;  // VW workspace columns (n x 96) of sytrd_lower
 
 // nonlocal.hip
 int launch_nonlocal_f64(qm_ctx *ctx, double *hpsi, const double *psi, int nb, int ncomp,

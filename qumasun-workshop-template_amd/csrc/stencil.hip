@@ -1,4 +1,5 @@
-// Fused FD8 kinetic + local potential kernel for gfx950.
+// Fused FD8 kinetic + local potential kernel for gfx950, with an autotuner
+// over its tile shapes.
 //
 // out = -1/2 (Delta_FD8 + 2 i k.grad_FD8 - |k|^2) psi + V_loc psi   (overwrites out)
 //
@@ -6,15 +7,31 @@
 // (GridDifferenceKpoint8th.h:17-151) with coef = -1/2
 // (qumasun_base1.matrix.h:298-317) plus mPotentialMatrixAdd_ddm_bundle
 // (qumasun_base1.matrix.h:329-337).  The periodic halo of PasteHalo_f4
-// (GridDifference8th.h:18-78) becomes wrapped loads into LDS.
+// (GridDifference8th.h:18-78) becomes wrapped loads.
 //
-// Layout on the GPU: a block owns a TX x TY tile of the x-y plane of one band
-// (x coalesced: one wave reads 64 consecutive x) and streams a chunk of z
-// planes.  The +-4 z neighbours of each point live in a 9-deep register queue,
-// so each grid value is read from HBM once per chunk; the x and y neighbours
-// come from an LDS copy of the current plane with a 4-wide halo.  The local
-// potential and the diagonal are folded into the same pass, so one H.psi
-// reads psi once, V once per tile, and writes Hpsi once.
+// Arithmetic: along each axis the 9-point second difference and the 8-point
+// first difference are written in forward differences D(x) = p(x+1) - p(x):
+//   sum_j c_j (p(x+j) + p(x-j) - 2 p(x)) = sum_{i=0..3} C_i (D(x+i) - D(x-1-i)),
+//   sum_j a_j (p(x+j) - p(x-j))          = sum_{i=0..3} A_i (D(x+i) + D(x-1-i)),
+// with C_i = sum_{j>i} c_j and A_i = sum_{j>i} a_j.  Each point computes one D
+// per axis and reads the other seven, so a point costs ~30 VALU operations
+// instead of ~50 for the term-by-term form, and the kinetic diagonal
+// (14350/5040 / h^2 ~ 100 Ha) never appears: rounding stays relative to the
+// kinetic energy in FP32 as well as FP64.
+//
+// Layout: each 64-lane wave is independent (no LDS tile, no barrier).  Lanes
+// run along x; each lane owns RY consecutive y rows and streams a chunk of z
+// planes.  Per row it keeps the next 8 planes of psi (prefetched 4 planes
+// ahead of use) and the 8 z differences around the current plane in
+// registers; the 8-step unrolled loop rotates both queues by renaming
+// registers, not by moves.  The y differences come from the lane's rows plus
+// 8 halo rows, the x differences from other lanes through ds_bpermute.  When
+// nx divides 64 the wave holds whole periodic rows (64/nx side by side) and
+// the x neighbours are a rotation inside each row; otherwise waves cover
+// overlapping 64-wide x windows and write the middle 56 points.  Loads and
+// stores are buffer operations whose per-plane offset is a wave-uniform
+// scalar, so addressing costs no VALU.  Blocks are mapped so that consecutive
+// tiles run on one XCD and share its L2 for the halos.
 #include <cmath>
 
 #include "qm_internal.h"
@@ -22,171 +39,314 @@
 namespace qm {
 namespace {
 
-constexpr int TX = 64;   // tile width in x (= wave width)
-constexpr int BY = 4;    // thread rows per block
-constexpr int RY = 4;    // points per thread in y
-constexpr int TY = BY * RY;
-constexpr int HW = 4;    // FD8 half width
-constexpr int LX = TX + 2 * HW;
-constexpr int LY = TY + 2 * HW;
+constexpr int HW = 4;              // FD8 half width
+constexpr int WAVE = 64;
+constexpr int XW = WAVE - 2 * HW;  // useful width of an overlapping x window
+constexpr unsigned kRsrcWord3 = 0x00020000u;  // raw 32-bit buffer resource, gfx9 family
 
 template <typename T>
 struct FD8Coef {
-    T c0;        // FP64: 14350/5040 * 1/2 * sum_d 1/h_d^2 + |k|^2/2; FP32: |k|^2/2
-    T cx[4], cy[4], cz[4];   // -1/2 * {8064, -1008, 128, -9}/5040 / h_d^2
-    T ax[4], ay[4], az[4];   // {672, -168, 32, -3}/840 * k_d / h_d
+    T c0;         // |k|^2 / 2
+    T lw[3][4];   // C_i = sum_{j>i} c_j, c_j = -1/2 {8064, -1008, 128, -9} / (5040 h_d^2)
+    T gw[3][4];   // A_i = sum_{j>i} a_j, a_j = {672, -168, 32, -3} k_d / (840 h_d)
 };
+
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ rsrc_t make_rsrc(const void *base, int64_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, (int)bytes, (int)kRsrcWord3);
+}
+__device__ __forceinline__ void bload(float &v, rsrc_t r, int voff, int soff) {
+    v = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+__device__ __forceinline__ void bload(double &v, rsrc_t r, int voff, int soff) {
+    v = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+__device__ __forceinline__ void bstore(float v, rsrc_t r, int voff, int soff) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
+}
+__device__ __forceinline__ void bstore(double v, rsrc_t r, int voff, int soff) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, 0);
+}
 
 __device__ __forceinline__ int wrapi(int i, int n) {
     int r = i % n;
     return r < 0 ? r + n : r;
 }
 
-template <typename T, int NC, bool KT>
-__global__ __launch_bounds__(TX * BY) void fd8_local_kernel(T *__restrict__ out, const T *__restrict__ in,
-                                                            const T *__restrict__ vloc, int nx, int ny, int nz,
-                                                            int zchunk, int nzc, FD8Coef<T> c) {
-    __shared__ T sm[NC][LY][LX];
-    const int tx = threadIdx.x, ty = threadIdx.y;
-    const int tid = ty * TX + tx;
-    const int zc = blockIdx.x % nzc;
-    const int band = blockIdx.x / nzc;
-    const int y0 = blockIdx.y * TY;
-    const int x0 = blockIdx.z * TX;
-    const int zb = zc * zchunk;
-    const int ze = min(zb + zchunk, nz);
+__device__ __forceinline__ float fmadd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmadd(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// value of v held by the lane whose byte address (4 * lane) is addr
+__device__ __forceinline__ float lane_read(float v, int addr) {
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(v)));
+}
+__device__ __forceinline__ double lane_read(double v, int addr) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_ds_bpermute(addr, (int)(unsigned)(b & 0xffffffffull));
+    const unsigned hi = (unsigned)__builtin_amdgcn_ds_bpermute(addr, (int)(unsigned)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// SEG: nx divides 64 and the wave holds 64/nx whole rows side by side.
+// A block is WY waves stacked in y over one (band, z chunk, x window): wave k
+// owns block rows (k*S + seg)*RY .. +RY (S = 64/nx row segments per wave in
+// SEG mode, else 1).  Each plane, every wave puts its rows' centre values into
+// LDS (double-buffered by plane parity, one barrier per plane); y halo rows
+// inside the block come from there, only rows outside it from global memory.
+// Queue slots: at step phase P (plane w = zz + P), logical slot m of pq holds
+// plane w + m and logical slot m of dz holds D(w - 4 + m); physical slot is
+// (m + P) & 7.
+template <typename T, int NC, bool KT, int RY, bool SEG, int WY>
+__global__ __launch_bounds__(64 * WY) void fd8_wave_kernel(T *__restrict__ out, const T *__restrict__ in,
+                                                           const T *__restrict__ vloc, int nx, int ny, int nz,
+                                                           int zchunk, int nzc, int nyb, int nxt, int nblk,
+                                                           int per_xcd, FD8Coef<T> c) {
+    constexpr int BRMAX = WY * RY * 64;  // block rows x LDS row width <= this
+    __shared__ T sm[2][NC][BRMAX];
+    // hardware block b runs on XCD b % 8: give each XCD a contiguous range of tiles
+    const int b = blockIdx.x;
+    const int lb = (b & 7) * per_xcd + (b >> 3);
+    if (lb >= nblk) return;  // block-uniform
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int k = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));  // wave in block
+    int t = lb;
+    const int yb = t % nyb;
+    t /= nyb;
+    const int xt = t % nxt;
+    t /= nxt;
+    const int zc = t % nzc;
+    const int band = t / nzc;
+
+    // lanes holding x+1, x+2, x+3 (la[0..2]) and x-1 .. x-4 (lb_[0..3])
+    int x, xl, b0, S, LW, la[3], lb_[4];
+    bool xout;
+    if constexpr (SEG) {
+        S = WAVE / nx;
+        const int seg = lane / nx;
+        x = lane - seg * nx;
+        xl = x;
+        LW = nx;
+        b0 = (k * S + seg) * RY;
+        xout = true;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) la[i] = (seg * nx + wrapi(x + i + 1, nx)) * 4;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) lb_[i] = (seg * nx + wrapi(x - i - 1, nx)) * 4;
+    } else {
+        S = 1;
+        const int x0 = xt * XW;
+        x = wrapi(x0 - HW + lane, nx);
+        xl = lane;
+        LW = WAVE;
+        b0 = k * RY;
+        xout = lane >= HW && lane < WAVE - HW && x0 + lane - HW < nx;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) la[i] = min(lane + i + 1, WAVE - 1) * 4;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) lb_[i] = max(lane - i - 1, 0) * 4;
+    }
+    const int BR = WY * S * RY;  // rows of the block
+    const int ybase = yb * BR;
+    const int y0 = ybase + b0;
     const int64_t plane = (int64_t)nx * ny;
     const int64_t ng = plane * nz;
-    const int w = min(TX, nx - x0);   // valid tile width
-    const int hgt = min(TY, ny - y0); // valid tile height
+    const int pbytes = (int)(plane * sizeof(T));
+    const int zb = zc * zchunk;
+    const int ze = min(zb + zchunk, nz);
 
-    const T *src[NC];
-    T *dst[NC];
+    rsrc_t rin[NC], rout[NC];
 #pragma unroll
     for (int cc = 0; cc < NC; ++cc) {
-        src[cc] = in + ((int64_t)band * NC + cc) * ng;
-        dst[cc] = out + ((int64_t)band * NC + cc) * ng;
+        rin[cc] = make_rsrc(in + ((int64_t)band * NC + cc) * ng, ng * (int64_t)sizeof(T));
+        rout[cc] = make_rsrc(out + ((int64_t)band * NC + cc) * ng, ng * (int64_t)sizeof(T));
     }
+    const rsrc_t rv = make_rsrc(vloc, ng * (int64_t)sizeof(T));
 
-    const bool xin = tx < w;
+    // byte offsets inside a plane; every load is wrapped into the grid, only
+    // the stores are masked
     int rowoff[RY];
-    bool pin[RY];
+    bool rowok[RY];
 #pragma unroll
     for (int r = 0; r < RY; ++r) {
-        const int ly = ty + BY * r;
-        pin[r] = xin && (ly < hgt);
-        rowoff[r] = pin[r] ? (x0 + tx) + nx * (y0 + ly) : 0;
+        rowoff[r] = (x + nx * wrapi(y0 + r, ny)) * (int)sizeof(T);
+        rowok[r] = xout && (y0 + r < ny);
     }
-    // y halo: thread row ty loads halo rows ty and ty+BY (8 rows in total);
-    // rows 0..3 sit above the tile, rows 4..7 below the last valid row.
-    int yh_off[2];
-    int yh_lds[2];
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-        const int hr = ty + BY * j;
-        const int ly = hr < HW ? hr - HW : hgt + (hr - HW);
-        yh_lds[j] = ly + HW;
-        yh_off[j] = xin ? (x0 + tx) + nx * wrapi(y0 + ly, ny) : 0;
-    }
-    // x halo: 8 columns x TY rows, threads 0..8*TY-1
-    const bool xh_on = tid < 8 * TY;
-    const int xh_col = tid & 7, xh_row = tid >> 3;
-    const int xh_lx = xh_col < HW ? xh_col - HW : w + (xh_col - HW);
-    const bool xh_valid = xh_on && xh_row < hgt;
-    const int xh_off = xh_valid ? wrapi(x0 + xh_lx, nx) + nx * (y0 + xh_row) : 0;
-
-    T q[NC][RY][9];
+    // halo rows y0-4..y0-1 and y0+RY..y0+RY+3: LDS slot inside the block, or a
+    // global byte offset (hsl < 0) outside it
+    int hoff[8], hsl[8];
+    bool hglob = false;
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-        const int64_t zo = (int64_t)wrapi(zb - HW + j, nz) * plane;
+        const int g = wrapi(j < 4 ? y0 - 4 + j : y0 + RY + (j - 4), ny);
+        hoff[j] = (x + nx * g) * (int)sizeof(T);
+        hsl[j] = (g - ybase >= 0 && g - ybase < BR) ? (g - ybase) * LW + xl : -1;
+        hglob = hglob || hsl[j] < 0;
+    }
+    const bool wglob = __any(hglob);  // some lane of the wave needs global halo rows
+
+    T pq[NC][RY][8], dz[NC][RY][8];
+    // planes zb .. zb+7 into pq; D(zb-4) .. D(zb+2) into dz
+    {
+        T lo[NC][RY][4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const int so = wrapi(zb - 4 + m, nz) * pbytes;
+#pragma unroll
+            for (int cc = 0; cc < NC; ++cc)
+#pragma unroll
+                for (int r = 0; r < RY; ++r) bload(lo[cc][r][m], rin[cc], rowoff[r], so);
+        }
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            const int so = wrapi(zb + m, nz) * pbytes;
+#pragma unroll
+            for (int cc = 0; cc < NC; ++cc)
+#pragma unroll
+                for (int r = 0; r < RY; ++r) bload(pq[cc][r][m], rin[cc], rowoff[r], so);
+        }
 #pragma unroll
         for (int cc = 0; cc < NC; ++cc)
 #pragma unroll
-            for (int r = 0; r < RY; ++r) q[cc][r][j] = pin[r] ? src[cc][zo + rowoff[r]] : T(0);
+            for (int r = 0; r < RY; ++r) {
+#pragma unroll
+                for (int m = 0; m < 3; ++m) dz[cc][r][m] = lo[cc][r][m + 1] - lo[cc][r][m];
+                dz[cc][r][3] = pq[cc][r][0] - lo[cc][r][3];
+#pragma unroll
+                for (int m = 4; m < 7; ++m) dz[cc][r][m] = pq[cc][r][m - 3] - pq[cc][r][m - 4];
+                dz[cc][r][7] = T(0);
+            }
+    }
+    T gh[NC][8], vv[RY];  // global halo rows (edge waves only) and V of the current plane
+    {
+        const int so = zb * pbytes;
+        if (wglob) {
+#pragma unroll
+            for (int cc = 0; cc < NC; ++cc)
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    gh[cc][j] = T(0);
+                    if (hsl[j] < 0) bload(gh[cc][j], rin[cc], hoff[j], so);
+                }
+        }
+#pragma unroll
+        for (int r = 0; r < RY; ++r) bload(vv[r], rv, rowoff[r], so);
     }
 
-    for (int z = zb; z < ze; ++z) {
-        const int64_t zo = (int64_t)z * plane;
-        const int64_t zf = (int64_t)wrapi(z + HW, nz) * plane;
+    for (int zz = zb; zz < ze; zz += 8) {
 #pragma unroll
-        for (int cc = 0; cc < NC; ++cc)
+        for (int P = 0; P < 8; ++P) {
+            const int w = zz + P;
+            if (w < ze) {  // block-uniform
+                // share this plane's centre values with the other waves of the block
 #pragma unroll
-            for (int r = 0; r < RY; ++r) q[cc][r][8] = pin[r] ? src[cc][zf + rowoff[r]] : T(0);
-        T vv[RY];
+                for (int cc = 0; cc < NC; ++cc)
 #pragma unroll
-        for (int r = 0; r < RY; ++r) vv[r] = pin[r] ? vloc[zo + rowoff[r]] : T(0);
-
-        // plane z into LDS: centre from the register queue, halos from global
+                    for (int r = 0; r < RY; ++r) sm[w & 1][cc][(b0 + r) * LW + xl] = pq[cc][r][P & 7];
+                // global halo rows and V of plane w + 1
+                T nh[NC][8], nv[RY];
+                if (w + 1 < ze) {
+                    const int so = (w + 1) * pbytes;
+                    if (wglob) {
 #pragma unroll
-        for (int cc = 0; cc < NC; ++cc) {
+                        for (int cc = 0; cc < NC; ++cc)
 #pragma unroll
-            for (int r = 0; r < RY; ++r)
-                if (pin[r]) sm[cc][ty + BY * r + HW][tx + HW] = q[cc][r][4];
-#pragma unroll
-            for (int j = 0; j < 2; ++j)
-                if (xin) sm[cc][yh_lds[j]][tx + HW] = src[cc][zo + yh_off[j]];
-            if (xh_valid) sm[cc][xh_row + HW][xh_lx + HW] = src[cc][zo + xh_off];
-        }
-        __syncthreads();
-
-#pragma unroll
-        for (int r = 0; r < RY; ++r) {
-            const int lx = tx + HW, ly = ty + BY * r + HW;
-            T res[NC];
-#pragma unroll
-            for (int cc = 0; cc < NC; ++cc) {
-                const T p = q[cc][r][4];
-                T lap;
-                if constexpr (sizeof(T) == 8) {
-                    // reference grouping: c0 p + sum_j c_j (p[+j] + p[-j])
-                    lap = c.c0 * p;
-#pragma unroll
-                    for (int j = 1; j <= 4; ++j) {
-                        lap += c.cx[j - 1] * (sm[cc][ly][lx + j] + sm[cc][ly][lx - j]);
-                        lap += c.cy[j - 1] * (sm[cc][ly + j][lx] + sm[cc][ly - j][lx]);
-                        lap += c.cz[j - 1] * (q[cc][r][4 + j] + q[cc][r][4 - j]);
+                            for (int j = 0; j < 8; ++j) {
+                                nh[cc][j] = T(0);
+                                if (hsl[j] < 0) bload(nh[cc][j], rin[cc], hoff[j], so);
+                            }
                     }
-                } else {
-                    // FP32: difference form sum_j c_j ((p[+j] - p) + (p[-j] - p)).  The
-                    // kinetic diagonal (~14350/5040 / h^2 ~ 100 Ha) never appears, so
-                    // rounding is relative to the kinetic energy and constants are
-                    // annihilated exactly whatever the rounding of the taps.
-                    lap = c.c0 * p;  // |k|^2 / 2 only in this form
 #pragma unroll
-                    for (int j = 1; j <= 4; ++j) {
-                        lap += c.cx[j - 1] * ((sm[cc][ly][lx + j] - p) + (sm[cc][ly][lx - j] - p));
-                        lap += c.cy[j - 1] * ((sm[cc][ly + j][lx] - p) + (sm[cc][ly - j][lx] - p));
-                        lap += c.cz[j - 1] * ((q[cc][r][4 + j] - p) + (q[cc][r][4 - j] - p));
+                    for (int r = 0; r < RY; ++r) bload(nv[r], rv, rowoff[r], so);
+                }
+                // newest z difference D(w + 3) = p(w + 4) - p(w + 3)
+#pragma unroll
+                for (int cc = 0; cc < NC; ++cc)
+#pragma unroll
+                    for (int r = 0; r < RY; ++r)
+                        dz[cc][r][(7 + P) & 7] = pq[cc][r][(4 + P) & 7] - pq[cc][r][(3 + P) & 7];
+                __syncthreads();
+                // y differences of the lane's column: rows y0-4 .. y0+RY+3
+                T dy[NC][RY + 7];
+#pragma unroll
+                for (int cc = 0; cc < NC; ++cc) {
+                    T e[RY + 8];
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        const T hv = hsl[j] >= 0 ? sm[w & 1][cc][hsl[j] >= 0 ? hsl[j] : 0] : gh[cc][j];
+                        e[j < 4 ? j : RY + j] = hv;
+                    }
+#pragma unroll
+                    for (int r = 0; r < RY; ++r) e[4 + r] = pq[cc][r][P & 7];
+#pragma unroll
+                    for (int q = 0; q < RY + 7; ++q) dy[cc][q] = e[q + 1] - e[q];
+                }
+                const int so = w * pbytes;
+#pragma unroll
+                for (int r = 0; r < RY; ++r) {
+                    T lap[NC], grd[NC];
+#pragma unroll
+                    for (int cc = 0; cc < NC; ++cc) {
+                        const T p = pq[cc][r][P & 7];
+                        // x differences: own D(x), then D(x+1..x+3) and D(x-1..x-4) from lanes
+                        const T dx0 = lane_read(p, la[0]) - p;
+                        T dxp[4], dxm[4];
+                        dxp[0] = dx0;
+#pragma unroll
+                        for (int i = 0; i < 3; ++i) dxp[i + 1] = lane_read(dx0, la[i]);
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) dxm[i] = lane_read(dx0, lb_[i]);
+                        T l = KT ? c.c0 * p : T(0);
+                        T g = T(0);
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) {
+                            const T zp = dz[cc][r][(4 + i + P) & 7], zm = dz[cc][r][(3 - i + P) & 7];
+                            const T yp = dy[cc][r + 4 + i], ym = dy[cc][r + 3 - i];
+                            l = fmadd(c.lw[0][i], dxp[i] - dxm[i], l);
+                            l = fmadd(c.lw[1][i], yp - ym, l);
+                            l = fmadd(c.lw[2][i], zp - zm, l);
+                            if constexpr (KT) {
+                                g = fmadd(c.gw[0][i], dxp[i] + dxm[i], g);
+                                g = fmadd(c.gw[1][i], yp + ym, g);
+                                g = fmadd(c.gw[2][i], zp + zm, g);
+                            }
+                        }
+                        lap[cc] = fmadd(vv[r], p, l);
+                        grd[cc] = g;
+                    }
+                    if constexpr (KT && NC == 2) {
+                        // re += k.grad im, im -= k.grad re
+                        lap[0] += grd[1];
+                        lap[1] -= grd[0];
+                    }
+                    if (rowok[r]) {
+#pragma unroll
+                        for (int cc = 0; cc < NC; ++cc) bstore(lap[cc], rout[cc], rowoff[r], so);
                     }
                 }
-                res[cc] = lap + vv[r] * p;
-            }
-            if constexpr (KT && NC == 2) {
-                T d_re = T(0), d_im = T(0);
+                // plane w + 8 into the slot plane w leaves
+                {
+                    int pf = w + 8;
+                    while (pf >= nz) pf -= nz;
+                    const int sf = pf * pbytes;
 #pragma unroll
-                for (int j = 1; j <= 4; ++j) {
-                    d_re += c.ax[j - 1] * (sm[0][ly][lx + j] - sm[0][ly][lx - j]);
-                    d_re += c.ay[j - 1] * (sm[0][ly + j][lx] - sm[0][ly - j][lx]);
-                    d_re += c.az[j - 1] * (q[0][r][4 + j] - q[0][r][4 - j]);
-                    d_im += c.ax[j - 1] * (sm[1][ly][lx + j] - sm[1][ly][lx - j]);
-                    d_im += c.ay[j - 1] * (sm[1][ly + j][lx] - sm[1][ly - j][lx]);
-                    d_im += c.az[j - 1] * (q[1][r][4 + j] - q[1][r][4 - j]);
+                    for (int cc = 0; cc < NC; ++cc)
+#pragma unroll
+                        for (int r = 0; r < RY; ++r) bload(pq[cc][r][P & 7], rin[cc], rowoff[r], sf);
                 }
-                res[0] += d_im;
-                res[1] -= d_re;
-            }
-            if (pin[r]) {
+                if (w + 1 < ze) {
+                    if (wglob) {
 #pragma unroll
-                for (int cc = 0; cc < NC; ++cc) dst[cc][zo + rowoff[r]] = res[cc];
+                        for (int cc = 0; cc < NC; ++cc)
+#pragma unroll
+                            for (int j = 0; j < 8; ++j) gh[cc][j] = nh[cc][j];
+                    }
+#pragma unroll
+                    for (int r = 0; r < RY; ++r) vv[r] = nv[r];
+                }
             }
         }
-        __syncthreads();
-#pragma unroll
-        for (int cc = 0; cc < NC; ++cc)
-#pragma unroll
-            for (int r = 0; r < RY; ++r)
-#pragma unroll
-                for (int j = 0; j < 8; ++j) q[cc][r][j] = q[cc][r][j + 1];
     }
 }
 
@@ -196,62 +356,178 @@ FD8Coef<T> make_coef(const qm_ctx *ctx, const double k[3]) {
     const double cj[4] = {8064.0, -1008.0, 128.0, -9.0};
     const double aj[4] = {672.0, -168.0, 32.0, -3.0};
     FD8Coef<T> c;
-    const double k2 = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
-    // FP64 keeps the reference's diagonal; the FP32 difference form folds the
-    // -2 sum_j c_j part of it into the taps and keeps only |k|^2 / 2
-    c.c0 = sizeof(T) == 8 ? T(14350.0 / 5040.0 * 0.5 * (1.0 / (h[0] * h[0]) + 1.0 / (h[1] * h[1]) + 1.0 / (h[2] * h[2])) +
-                             0.5 * k2)
-                          : T(0.5 * k2);
-    T *cs[3] = {c.cx, c.cy, c.cz};
-    T *as[3] = {c.ax, c.ay, c.az};
-    for (int d = 0; d < 3; ++d)
-        for (int j = 0; j < 4; ++j) {
-            cs[d][j] = T(-0.5 * cj[j] / (5040.0 * h[d] * h[d]));
-            // the reference drops the first-derivative term when |k_d| < 1e-8
-            // (GridDifferenceKpoint8th.h:330-365)
-            as[d][j] = std::fabs(k[d]) < 1.0e-8 ? T(0) : T(aj[j] * k[d] / (840.0 * h[d]));
+    c.c0 = T(0.5 * (k[0] * k[0] + k[1] * k[1] + k[2] * k[2]));
+    for (int d = 0; d < 3; ++d) {
+        // the reference drops the first-derivative term when |k_d| < 1e-8
+        // (GridDifferenceKpoint8th.h:330-365)
+        const bool grad = !(std::fabs(k[d]) < 1.0e-8);
+        for (int i = 0; i < 4; ++i) {
+            double cs = 0.0, as = 0.0;
+            for (int j = i; j < 4; ++j) {
+                cs += -0.5 * cj[j] / (5040.0 * h[d] * h[d]);
+                as += aj[j] * k[d] / (840.0 * h[d]);
+            }
+            c.lw[d][i] = T(cs);
+            c.gw[d][i] = grad ? T(as) : T(0);
         }
+    }
     return c;
 }
 
-template <typename T>
-int launch_fd8(const qm_ctx *ctx, T *out, const T *in, const T *vloc, int nb, int ncomp, const double k[3],
-               hipStream_t st) {
-    QM_CHECK(ctx->nx >= HW && ctx->ny >= HW && ctx->nz >= HW, QM_ERR_ARG, "grid dimensions must be >= 4");
-    const bool kt = !(std::fabs(k[0]) < 1.0e-8 && std::fabs(k[1]) < 1.0e-8 && std::fabs(k[2]) < 1.0e-8);
-    QM_CHECK(ncomp == 2 || !kt, QM_ERR_ARG, "real (ncomp=1) bands need a Gamma k-point");
-    const FD8Coef<T> c = make_coef<T>(ctx, k);
-    const int ntx = (ctx->nx + TX - 1) / TX;
-    const int nty = (ctx->ny + TY - 1) / TY;
-    // z chunk: long enough to amortise the 8-plane queue warm-up, short enough
-    // to put >= 4 blocks on each of the 256 CUs
-    int zchunk = 32;
-    while (zchunk > 8 && (int64_t)ntx * nty * nb * ((ctx->nz + zchunk - 1) / zchunk) < 1024) zchunk /= 2;
-    const int nzc = (ctx->nz + zchunk - 1) / zchunk;
-    QM_CHECK((int64_t)nb * nzc < (int64_t)INT32_MAX && nty <= 65535 && ntx <= 65535, QM_ERR_ARG,
-             "grid too large for one launch");
-    dim3 grid(nb * nzc, nty, ntx), block(TX, BY);
-    if (ncomp == 1)
-        hipLaunchKernelGGL((fd8_local_kernel<T, 1, false>), grid, block, 0, st, out, in, vloc, ctx->nx, ctx->ny,
-                           ctx->nz, zchunk, nzc, c);
-    else if (!kt)
-        hipLaunchKernelGGL((fd8_local_kernel<T, 2, false>), grid, block, 0, st, out, in, vloc, ctx->nx, ctx->ny,
-                           ctx->nz, zchunk, nzc, c);
+// ---------------------------------------------------------------------------
+// candidates, autotuner and launch
+// ---------------------------------------------------------------------------
+
+struct Candidate {
+    int ry;      // rows per lane
+    int wy;      // waves stacked in y per block
+    int zchunk;  // z planes per block (a multiple of 8)
+};
+constexpr Candidate kCandidates[] = {
+    {4, 8, 32}, {4, 8, 16}, {4, 8, 64}, {2, 16, 32}, {2, 16, 64}, {4, 4, 32}, {8, 4, 32}, {8, 8, 64},
+};
+constexpr int kNumCandidates = sizeof(kCandidates) / sizeof(kCandidates[0]);
+
+template <typename T, int NC, bool KT, int RY, int WY>
+int launch_wave(const qm_ctx *ctx, T *out, const T *in, const T *vloc, int nb, int zchunk, const FD8Coef<T> &c,
+                hipStream_t st) {
+    const int nx = ctx->nx, ny = ctx->ny, nz = ctx->nz;
+    const bool seg = WAVE % nx == 0;
+    const int rows = (seg ? WAVE / nx : 1) * RY * WY;  // y rows per block
+    const int nxt = seg ? 1 : (nx + XW - 1) / XW;
+    const int nyb = (ny + rows - 1) / rows;
+    const int nzc = (nz + zchunk - 1) / zchunk;
+    const int64_t nblk = (int64_t)nb * nzc * nyb * nxt;
+    QM_CHECK(nblk < (int64_t)INT32_MAX / 2, QM_ERR_ARG, "grid too large for one stencil launch");
+    const int per_xcd = (int)((nblk + 7) / 8);
+    if (seg)
+        hipLaunchKernelGGL((fd8_wave_kernel<T, NC, KT, RY, true, WY>), dim3(8 * per_xcd), dim3(64 * WY), 0, st, out,
+                           in, vloc, nx, ny, nz, zchunk, nzc, nyb, nxt, (int)nblk, per_xcd, c);
     else
-        hipLaunchKernelGGL((fd8_local_kernel<T, 2, true>), grid, block, 0, st, out, in, vloc, ctx->nx, ctx->ny,
-                           ctx->nz, zchunk, nzc, c);
+        hipLaunchKernelGGL((fd8_wave_kernel<T, NC, KT, RY, false, WY>), dim3(8 * per_xcd), dim3(64 * WY), 0, st, out,
+                           in, vloc, nx, ny, nz, zchunk, nzc, nyb, nxt, (int)nblk, per_xcd, c);
     QM_HIP(hipGetLastError());
     return QM_OK;
 }
 
+template <typename T, int NC, bool KT>
+int launch_candidate(const qm_ctx *ctx, int cand, T *out, const T *in, const T *vloc, int nb, const FD8Coef<T> &c,
+                     hipStream_t st) {
+    const Candidate cd = kCandidates[cand];
+    if constexpr (sizeof(T) == 8 || NC == 2) {
+        // FP64 or complex bands need twice the registers per row: 2 or 4 rows
+        // per lane, 4 waves per block (larger shapes spill)
+        if (cd.ry == 2) return launch_wave<T, NC, KT, 2, 4>(ctx, out, in, vloc, nb, cd.zchunk, c, st);
+        return launch_wave<T, NC, KT, 4, 4>(ctx, out, in, vloc, nb, cd.zchunk, c, st);
+    } else {
+        if (cd.ry == 2) return launch_wave<T, NC, KT, 2, 16>(ctx, out, in, vloc, nb, cd.zchunk, c, st);
+        if (cd.ry == 4 && cd.wy == 8) return launch_wave<T, NC, KT, 4, 8>(ctx, out, in, vloc, nb, cd.zchunk, c, st);
+        if (cd.ry == 4) return launch_wave<T, NC, KT, 4, 4>(ctx, out, in, vloc, nb, cd.zchunk, c, st);
+        if (cd.wy == 8) return launch_wave<T, NC, KT, 8, 8>(ctx, out, in, vloc, nb, cd.zchunk, c, st);
+        return launch_wave<T, NC, KT, 8, 4>(ctx, out, in, vloc, nb, cd.zchunk, c, st);
+    }
+}
+
+template <typename T>
+int launch_any(const qm_ctx *ctx, int cand, int ncomp, bool kt, T *out, const T *in, const T *vloc, int nb,
+               const FD8Coef<T> &c, hipStream_t st) {
+    if (ncomp == 1) return launch_candidate<T, 1, false>(ctx, cand, out, in, vloc, nb, c, st);
+    if (!kt) return launch_candidate<T, 2, false>(ctx, cand, out, in, vloc, nb, c, st);
+    return launch_candidate<T, 2, true>(ctx, cand, out, in, vloc, nb, c, st);
+}
+
+int64_t tune_key(int precision, int ncomp, bool kt, int nb) {
+    return ((((int64_t)nb * 2 + (ncomp - 1)) * 2 + (kt ? 1 : 0)) << 8) | precision;
+}
+
+// Times every candidate on the caller's buffers (each run overwrites out with
+// the same values) and returns the fastest in *best.
+template <typename T>
+int autotune(const qm_ctx *ctx, int ncomp, bool kt, T *out, const T *in, const T *vloc, int nb, const FD8Coef<T> &c,
+             hipStream_t st, int *best) {
+    hipEvent_t ev[2];
+    QM_HIP(hipEventCreate(&ev[0]));
+    if (hipEventCreate(&ev[1]) != hipSuccess) {
+        (void)hipEventDestroy(ev[0]);
+        QM_CHECK(false, QM_ERR_HIP, "stencil autotune: hipEventCreate failed");
+    }
+    int status = QM_OK;
+    float best_ms = 1e30f;
+    *best = 0;
+    for (int cand = 0; cand < kNumCandidates && status == QM_OK; ++cand) {
+        float t_min = 1e30f;
+        for (int rep = 0; rep < 3 && status == QM_OK; ++rep) {  // the first run warms up
+            float ms = 0.0f;
+            if (hipEventRecord(ev[0], st) != hipSuccess) status = QM_ERR_HIP;
+            if (status == QM_OK) status = launch_any<T>(ctx, cand, ncomp, kt, out, in, vloc, nb, c, st);
+            if (status == QM_OK && hipEventRecord(ev[1], st) != hipSuccess) status = QM_ERR_HIP;
+            if (status == QM_OK && hipEventSynchronize(ev[1]) != hipSuccess) status = QM_ERR_HIP;
+            if (status == QM_OK && hipEventElapsedTime(&ms, ev[0], ev[1]) != hipSuccess) status = QM_ERR_HIP;
+            if (status == QM_OK && rep > 0) t_min = std::min(t_min, ms);
+        }
+        if (status == QM_OK && t_min < best_ms) {
+            best_ms = t_min;
+            *best = cand;
+        }
+    }
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    if (status == QM_ERR_HIP) set_error("stencil autotune: HIP event failure");
+    return status;
+}
+
+template <typename T>
+int launch_fd8(qm_ctx *ctx, T *out, const T *in, const T *vloc, int nb, int ncomp, const double k[3],
+               hipStream_t st) {
+    QM_CHECK(ctx->nx >= HW && ctx->ny >= HW && ctx->nz >= HW, QM_ERR_ARG, "grid dimensions must be >= 4");
+    QM_CHECK(ctx->ng * (int64_t)sizeof(T) < (int64_t)INT32_MAX, QM_ERR_ARG,
+             "grid too large for 32-bit buffer offsets (Ng * sizeof(T) must be < 2 GiB)");
+    const bool kt = !(std::fabs(k[0]) < 1.0e-8 && std::fabs(k[1]) < 1.0e-8 && std::fabs(k[2]) < 1.0e-8);
+    QM_CHECK(ncomp == 2 || !kt, QM_ERR_ARG, "real (ncomp=1) bands need a Gamma k-point");
+    const FD8Coef<T> c = make_coef<T>(ctx, k);
+    int cand = ctx->stencil_force;
+    if (cand < 0) {
+        const int64_t key = tune_key(sizeof(T) == 8 ? 64 : 32, ncomp, kt, nb);
+        auto it = ctx->stencil_choice.find(key);
+        if (it != ctx->stencil_choice.end()) {
+            cand = it->second;
+        } else {
+            hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+            QM_HIP(hipStreamIsCapturing(st, &cs));
+            if (cs != hipStreamCaptureStatusNone) {
+                cand = 0;  // no timing inside a capture: the default shape
+            } else {
+                QM_TRY(autotune<T>(ctx, ncomp, kt, out, in, vloc, nb, c, st, &cand));
+                ctx->stencil_choice[key] = cand;
+            }
+        }
+    }
+    return launch_any<T>(ctx, cand, ncomp, kt, out, in, vloc, nb, c, st);
+}
+
 }  // namespace
 
-int launch_fd8_local_f64(const qm_ctx *ctx, double *out, const double *in, int nb, int ncomp, const double k[3],
+int stencil_num_candidates() { return kNumCandidates; }
+
+std::string stencil_candidate_name(int cand) {
+    const Candidate cd = kCandidates[cand];
+    return "fd8_wave_kernel: RY " + std::to_string(cd.ry) + ", " + std::to_string(cd.wy) +
+           " waves stacked in y, z chunk " + std::to_string(cd.zchunk) +
+           " (FP64 / complex: RY " + std::to_string(cd.ry == 2 ? 2 : 4) + ", 4 waves)";
+}
+
+int stencil_choice(const qm_ctx *ctx, int precision, int ncomp, int kt, int nb) {
+    if (ctx->stencil_force >= 0) return ctx->stencil_force;
+    auto it = ctx->stencil_choice.find(tune_key(precision, ncomp, kt != 0, nb));
+    return it == ctx->stencil_choice.end() ? -1 : it->second;
+}
+
+int launch_fd8_local_f64(qm_ctx *ctx, double *out, const double *in, int nb, int ncomp, const double k[3],
                          hipStream_t st) {
     return launch_fd8<double>(ctx, out, in, ctx->d_vloc64, nb, ncomp, k, st);
 }
 
-int launch_fd8_local_f32(const qm_ctx *ctx, float *out, const float *in, int nb, int ncomp, const double k[3],
+int launch_fd8_local_f32(qm_ctx *ctx, float *out, const float *in, int nb, int ncomp, const double k[3],
                          hipStream_t st) {
     return launch_fd8<float>(ctx, out, in, ctx->d_vloc32, nb, ncomp, k, st);
 }

@@ -89,6 +89,27 @@ class Hamiltonian:
         out = torch.empty_like(psi)
         return self.apply(out, psi, k_index=k_index, ncomp=ncomp)
 
+    # -- stencil tile-shape autotuner ------------------------------------------
+    def stencil_candidates(self):
+        """Descriptions of the stencil tile shapes the autotuner chooses from."""
+        buf = ctypes.create_string_buffer(256)
+        names = []
+        for i in range(self.lib.qm_stencil_num_candidates()):
+            _lib.check(self.lib.qm_stencil_candidate_name(i, buf, len(buf)))
+            names.append(buf.value.decode())
+        return names
+
+    def stencil_force(self, candidate):
+        """Use stencil candidate `candidate` for every later apply (-1: autotune)."""
+        _lib.check(self.lib.qm_stencil_force(self.ctx, candidate))
+
+    def stencil_choice(self, precision, nb, ncomp=None, kt=False):
+        """Candidate index in use for this apply shape (-1 if not tuned yet)."""
+        ncomp = self.sys.ncomp if ncomp is None else ncomp
+        out = ctypes.c_int()
+        _lib.check(self.lib.qm_stencil_choice(self.ctx, precision, ncomp, int(kt), nb, ctypes.byref(out)))
+        return out.value
+
     # -- dense products (the blas_DGEMM_t / blas_DGEMM_n sub-boundary) ------
     def gram(self, A, B):
         """C = A^T B (FP64) for column-major A (k x m) and B (k x n) given as
@@ -114,6 +135,26 @@ class Hamiltonian:
         _lib.check(fn(self.ctx, ctypes.c_void_p(Z.data_ptr()), m, ctypes.c_void_p(Cc.data_ptr()), n, k,
                       ctypes.c_void_p(Y.data_ptr()), _stream(torch, Z.device)))
         return Y
+
+    def sygvd(self, A, B, nev=None):
+        """A z = l B z (FP64, symmetric A, SPD B; upper triangles referenced).
+        Returns (all eigenvalues ascending, Z of shape (n, nev) with the lowest
+        nev eigenvectors, U the Cholesky factor of B in its upper triangle), as
+        device tensors."""
+        import torch
+        n = A.shape[0]
+        nev = n if nev is None else nev
+        assert A.dtype == torch.float64 and B.dtype == torch.float64 and A.shape == B.shape == (n, n)
+        # the row-major buffer of A^T is A in column-major order
+        At = A.T.contiguous()
+        Bt = B.T.contiguous()
+        w = torch.empty(n, dtype=torch.float64, device=A.device)
+        Zt = torch.empty((nev, n), dtype=torch.float64, device=A.device)
+        Ut = torch.empty((n, n), dtype=torch.float64, device=A.device)
+        _lib.check(self.lib.qm_sygvd_f64(self.ctx, n, ctypes.c_void_p(At.data_ptr()), ctypes.c_void_p(Bt.data_ptr()),
+                                         ctypes.c_void_p(w.data_ptr()), ctypes.c_void_p(Zt.data_ptr()),
+                                         ctypes.c_void_p(Ut.data_ptr()), nev, _stream(torch, A.device)))
+        return w, Zt.T, Ut.T
 
     # -- adjoint -------------------------------------------------------------
     def band_density(self, psi, occ, ncomp=None, rho=None):

@@ -135,3 +135,58 @@ def test_nonlocal_forces_parity(qm, orc, torch_cuda, prec, cfg, kidx):
     assert abs(e - e0) <= tol * abs(e0)
     assert np.abs(f - f0).max() <= tol * np.abs(f0).max()
     H.close()
+
+
+def _seg4(qm):
+    s = qm.system.System("seg4", 16, 24, 12, (3.0, 4.5, 2.25), np.array([[1.0, 2.0, 1.0]]),
+                         np.array([2], np.int32), np.array([1.2]), 2, 2, np.array([[0.5, 0.0, -0.25]]), 5)
+    return s.build()
+
+
+STENCIL_CASES = [
+    ("C0 32^3 real (2 rows per wave)", lambda qm: qm.system.config(0, nbands=3)),
+    ("64^3 real (1 row per wave)", lambda qm: qm.system.config(1, nbands=2)),
+    ("C2 48^3 complex 4k (x windows)", lambda qm: qm.system.config(2, nbands=2, grid=48)),
+    ("C3 80^3 (2 x windows)", lambda qm: qm.system.config(3, nbands=2, grid=80)),
+    ("small 12^3 k", lambda qm: qm.system.small_system(n=12, L=4.0, natoms=3, rc=2.6, nbands=2, ncomp=2,
+                                                       k_red=(0.5, 0.25, -0.125))),
+    ("ragged 36x20x28 k", lambda qm: _ragged(qm)),
+    ("16x24x12 k (4 rows per wave)", lambda qm: _seg4(qm)),
+]
+
+
+@pytest.mark.parametrize("label,build", STENCIL_CASES, ids=[c[0] for c in STENCIL_CASES])
+def test_stencil_candidates_bit_identical(qm, orc, torch_cuda, label, build):
+    """Every tile shape the autotuner may pick writes every output point and
+    gives bit-identical results (same arithmetic per point), and the shape the
+    autotuner settles on matches the oracle's kinetic + V_loc part."""
+    torch = torch_cuda
+    s = build(qm)
+    H = qm.hamiltonian.Hamiltonian(s)
+    names = H.stencil_candidates()
+    assert len(names) >= 2
+    psi = s.initial_bands()
+    for prec, dtype, tol in (("f64", torch.float64, F64_TOL), ("f32", torch.float32, F32_TOL)):
+        x = torch.from_numpy(psi).to(device="cuda", dtype=dtype)
+        src = psi if prec == "f64" else psi.astype(np.float32).astype(np.float64)
+        for ik in range(s.kpoints.shape[0]):
+            print(f"case {label} prec {prec} k {ik}", flush=True)
+            outs = []
+            for cand in range(len(names)):
+                H.stencil_force(cand)
+                y = torch.full_like(x, float("nan"))
+                H.apply(y, x, k_index=ik, part="local")
+                outs.append(y)
+            torch.cuda.synchronize()
+            for cand in range(1, len(names)):
+                assert torch.equal(outs[cand], outs[0]), (label, prec, ik, names[cand])
+            H.stencil_force(-1)
+            y = torch.full_like(x, float("nan"))
+            H.apply(y, x, k_index=ik, part="local")
+            torch.cuda.synchronize()
+            kt = bool(np.any(np.abs(s.kpoints[ik]) >= 1e-8))
+            assert 0 <= H.stencil_choice(32 if prec == "f32" else 64, s.nbands, kt=kt) < len(names)
+            assert torch.equal(y, outs[0])
+            want = orc.hpsi(s, src, k=s.kpoints[ik], nonlocal_=False)
+            assert _rel(y.double().cpu().numpy(), want) <= tol, (label, prec, ik)
+    H.close()
