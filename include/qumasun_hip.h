@@ -161,7 +161,7 @@ int qm_update_f32(qm_ctx *ctx, const float *d_Z, int m, const float *d_C, int n,
  * d_A, d_B: n x n column-major, upper triangles referenced, not modified.
  * d_w: all n eigenvalues, ascending.  d_Z: n x nev, eigenvectors of the nev
  * lowest, B-orthonormal.  d_U: n x n, upper triangle receives the Cholesky
- * factor of B (strictly lower part unspecified).  n <= 1024.  Synchronises the
+ * factor of B (strictly lower part unspecified).  n <= 4096.  Synchronises the
  * stream.  QM_ERR_NUMERIC when B is not positive definite. */
 int qm_sygvd_f64(qm_ctx *ctx, int n, const double *d_A, const double *d_B, double *d_w, double *d_Z, double *d_U,
                  int nev, void *stream);

@@ -7,17 +7,18 @@
 // The tridiagonal reduction is this file's own: rocSOLVER's sytrd launches
 // about ten small kernels per column, which at the subspace sizes of LOBPCG
 // (n = 2N or 3N, a few hundred) costs ~15 us per column in launch latency
-// alone.  Here one workgroup of 1024 threads reduces a panel of 32 columns in
+// alone.  Here one workgroup of 1024 threads reduces a panel of 16 columns in
 // one launch (the blocked scheme of LAPACK's dlatrd: each column is updated
 // with the panel's earlier reflectors, its Householder vector is formed, and
 // w = tau (A22 v - V W^T v - W V^T v) is accumulated against the trailing
-// matrix as it stood at the panel's start), one rocBLAS GEMM applies the
-// rank-64 update A22 -= V W^T + W V^T, and the last <= 88 columns are reduced
-// unblocked with the trailing block held in LDS.  Output follows LAPACK's
-// lower-storage convention (d, e, tau, reflectors below the subdiagonal), so
-// rocSOLVER's stedc (divide and conquer on T) and ormtr (apply Q) consume it.
-// The matrix is kept symmetric in both triangles, so the matrix-vector
-// product is a plain column sweep.
+// matrix as it stood at the panel's start), with the panel's V and W held in
+// LDS; one rocBLAS GEMM then applies the rank-32 update A22 -= V W^T + W V^T,
+// and the last <= 88 columns are reduced unblocked with the trailing block in
+// LDS.  Output follows LAPACK's lower-storage convention (d, e, tau,
+// reflectors below the subdiagonal), so rocSOLVER's stedc (divide and
+// conquer on T) and ormtr (apply Q) consume it.  The matrix is kept symmetric
+// in both triangles (the upper one is mirrored first), so the matrix-vector
+// product is a plain column sweep.  Matrices above 512 go to rocSOLVER syevd.
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
@@ -28,11 +29,11 @@
 namespace qm {
 namespace {
 
-constexpr int kNB = 32;         // panel width
+constexpr int kNB = 16;         // panel width
 constexpr int kTailMax = 88;    // columns left for the LDS kernel (88^2 doubles = 61952 bytes)
 constexpr int kThreads = 1024;
 constexpr int kWaves = kThreads / 64;
-constexpr int kMaxN = 1024;     // one thread per row of the matrix-vector product
+constexpr int kMaxN = 512;      // the panel's V and W (n x 16 each) live in LDS
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -71,100 +72,127 @@ __device__ void householder(double *xs, int m, double *red, double &tau, double 
     __syncthreads();
 }
 
-// Columns i0 .. i0+nb-1 of the symmetric n x n matrix A (column-major, both
-// triangles valid on entry).  VW holds three n x nb blocks [V | W | V]; rows
-// below each reflector's start are written.
-__global__ __launch_bounds__(kThreads) void trd_panel_kernel(double *__restrict__ A, int lda, int n, int i0, int nb,
+// Columns i0 .. i0+kNB-1 of the symmetric n x n matrix A (column-major, both
+// triangles valid on entry), n - i0 <= kMaxN.  The panel's V and W stay in LDS
+// (column q of V at Vs[q][r - i0]); on exit VW holds [V | W | V] (n x kNB
+// blocks, rows i0 .. n-1) for the trailing update.
+__global__ __launch_bounds__(kThreads) void trd_panel_kernel(double *__restrict__ A, int lda, int n, int i0,
                                                              double *__restrict__ d, double *__restrict__ e,
                                                              double *__restrict__ tau, double *__restrict__ VW,
                                                              int ldp) {
+    // w = A22 v is split over kCG column groups x kRQ row quarters (one wave
+    // each); a lane keeps kRC row accumulators and loads 8 columns per round
+    constexpr int kCG = 4;
+    constexpr int kRQ = kWaves / kCG;
+    constexpr int kRC = kMaxN / (64 * kRQ);
+    __shared__ double Vs[kNB][kMaxN], Ws[kNB][kMaxN];
     __shared__ double vs[kMaxN];
-    __shared__ double part[kThreads];
-    __shared__ double vrow[2 * kNB];
+    __shared__ double part[kCG][kMaxN];
     __shared__ double tt[2 * kNB];
     __shared__ double red[kWaves];
     const int tid = threadIdx.x;
-    double *V = VW, *W = VW + (size_t)nb * ldp, *V2 = VW + (size_t)2 * nb * ldp;
-    for (int j = 0; j < nb; ++j) {
+    const int wv = tid >> 6, ln = tid & 63;
+    const int np = n - i0;
+    for (int idx = tid; idx < kNB * kMaxN; idx += kThreads) {
+        (&Vs[0][0])[idx] = 0.0;
+        (&Ws[0][0])[idx] = 0.0;
+    }
+    __syncthreads();
+    for (int j = 0; j < kNB; ++j) {
         const int c = i0 + j;
         const int m = n - c - 1;
         // A(c:n, c) -= V(c:n, :j) W(c, :j)^T + W(c:n, :j) V(c, :j)^T
-        if (tid < j) {
-            vrow[tid] = V[c + (size_t)tid * ldp];
-            vrow[kNB + tid] = W[c + (size_t)tid * ldp];
-        }
-        __syncthreads();
-        for (int r = c + tid; r < n; r += kThreads) {
-            double a = A[r + (size_t)c * lda];
-            for (int q = 0; q < j; ++q) {
-                a = fma(-V[r + (size_t)q * ldp], vrow[kNB + q], a);
-                a = fma(-W[r + (size_t)q * ldp], vrow[q], a);
-            }
-            if (r == c) d[c] = a;
-            else vs[r - c - 1] = a;
+        for (int rl = j + tid; rl < np; rl += kThreads) {
+            double a = A[(i0 + rl) + (size_t)c * lda];
+            for (int q = 0; q < j; ++q) a = fma(-Vs[q][rl], Ws[q][j], fma(-Ws[q][rl], Vs[q][j], a));
+            if (rl == j) d[c] = a;
+            else vs[rl - j - 1] = a;
         }
         __syncthreads();
         double t, beta;
         householder(vs, m, red, t, beta);
         for (int k = tid; k < m; k += kThreads) {
-            const double v = vs[k];
-            V[c + 1 + k + (size_t)j * ldp] = v;
-            V2[c + 1 + k + (size_t)j * ldp] = v;
-            A[c + 1 + k + (size_t)c * lda] = k == 0 ? beta : v;
+            Vs[j][j + 1 + k] = vs[k];
+            A[c + 1 + k + (size_t)c * lda] = k == 0 ? beta : vs[k];
         }
         if (tid == 0) {
             e[c] = beta;
             tau[c] = t;
         }
-        // w = A22 v over G column groups, A22 = A(c+1:n, c+1:n) as at the panel start
-        const int mp = (m + 63) & ~63;
-        const int G = max(1, min(16, kThreads / mp));
+        // w = A22 v, A22 = A(c+1:n, c+1:n) as at the panel start
         {
-            const int r = tid % mp, g = tid / mp;
-            if (g < G && r < m) {
-                const int k0 = (int)((int64_t)m * g / G), k1 = (int)((int64_t)m * (g + 1) / G);
-                const double *col = A + (c + 1 + r) + (size_t)(c + 1) * lda;
-                double acc = 0.0;
-                int k = k0;
-                for (; k + 3 < k1; k += 4) {
-                    const double a0 = col[(size_t)k * lda], a1 = col[(size_t)(k + 1) * lda];
-                    const double a2 = col[(size_t)(k + 2) * lda], a3 = col[(size_t)(k + 3) * lda];
-                    acc = fma(a0, vs[k], acc);
-                    acc = fma(a1, vs[k + 1], acc);
-                    acc = fma(a2, vs[k + 2], acc);
-                    acc = fma(a3, vs[k + 3], acc);
-                }
-                for (; k < k1; ++k) acc = fma(col[(size_t)k * lda], vs[k], acc);
-                part[g * mp + r] = acc;
+            const int g = wv % kCG, rq = wv / kCG;
+            const int k0 = (int)((int64_t)m * g / kCG), k1 = (int)((int64_t)m * (g + 1) / kCG);
+            const int rbeg = (int)((int64_t)m * rq / kRQ), rend = (int)((int64_t)m * (rq + 1) / kRQ);
+            const double *base = A + (c + 1) + (size_t)(c + 1) * lda;
+            double acc[kRC];
+            bool act[kRC];
+#pragma unroll
+            for (int i = 0; i < kRC; ++i) {
+                acc[i] = 0.0;
+                act[i] = rbeg + ln + 64 * i < rend;
             }
+            int k = k0;
+            for (; k + 7 < k1; k += 8) {
+                double x[8], a[8][kRC];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    x[u] = vs[k + u];
+                    const double *col = base + (size_t)(k + u) * lda + rbeg + ln;
+#pragma unroll
+                    for (int i = 0; i < kRC; ++i) {
+                        a[u][i] = 0.0;
+                        if (act[i]) a[u][i] = col[64 * i];  // rows past the quarter are not read
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+#pragma unroll
+                    for (int i = 0; i < kRC; ++i) acc[i] = fma(a[u][i], x[u], acc[i]);
+            }
+            for (; k < k1; ++k) {
+                const double x0 = vs[k];
+                const double *col = base + (size_t)k * lda + rbeg + ln;
+#pragma unroll
+                for (int i = 0; i < kRC; ++i)
+                    if (act[i]) acc[i] = fma(col[64 * i], x0, acc[i]);
+            }
+#pragma unroll
+            for (int i = 0; i < kRC; ++i)
+                if (act[i]) part[g][rbeg + ln + 64 * i] = acc[i];
         }
         // t1 = W(c+1:n, :j)^T v, t2 = V(c+1:n, :j)^T v: one wave per dot product
-        {
-            const int wv = tid >> 6, ln = tid & 63;
-            for (int q2 = wv; q2 < 2 * j; q2 += kWaves) {
-                const double *col = (q2 < j ? W + (size_t)q2 * ldp : V + (size_t)(q2 - j) * ldp) + c + 1;
-                double acc = 0.0;
-                for (int k = ln; k < m; k += 64) acc = fma(col[k], vs[k], acc);
-                acc = wave_sum(acc);
-                if (ln == 0) tt[q2] = acc;
-            }
+        for (int q2 = wv; q2 < 2 * j; q2 += kWaves) {
+            const double *col = (q2 < j ? &Ws[q2][0] : &Vs[q2 - j][0]) + j + 1;
+            double a0 = 0.0;
+            for (int k = ln; k < m; k += 64) a0 = fma(col[k], vs[k], a0);
+            const double acc = wave_sum(a0);
+            if (ln == 0) tt[q2] = acc;
         }
         __syncthreads();
-        double loc = 0.0;
+        double loc = 0.0, wr[kMaxN / kThreads + 1];
+        int nr = 0;
         for (int r = tid; r < m; r += kThreads) {
             double w = 0.0;
-            for (int g = 0; g < G; ++g) w += part[g * mp + r];
-            for (int q = 0; q < j; ++q) {
-                w = fma(-V[c + 1 + r + (size_t)q * ldp], tt[q], w);
-                w = fma(-W[c + 1 + r + (size_t)q * ldp], tt[j + q], w);
-            }
+#pragma unroll
+            for (int g = 0; g < kCG; ++g) w += part[g][r];
+            for (int q = 0; q < j; ++q) w = fma(-Vs[q][j + 1 + r], tt[q], fma(-Ws[q][j + 1 + r], tt[j + q], w));
             w *= t;
-            part[r] = w;  // only this thread reads part[g * mp + r]
+            wr[nr++] = w;
             loc = fma(w, vs[r], loc);
         }
         const double alpha = -0.5 * t * block_sum(loc, red);
-        for (int r = tid; r < m; r += kThreads) W[c + 1 + r + (size_t)j * ldp] = fma(alpha, vs[r], part[r]);
+        nr = 0;
+        for (int r = tid; r < m; r += kThreads) Ws[j][j + 1 + r] = fma(alpha, vs[r], wr[nr++]);
         __syncthreads();
+    }
+    // [V | W | V] for the trailing rank-2 kNB update
+    for (int idx = tid; idx < kNB * np; idx += kThreads) {
+        const int q = idx / np, rl = idx % np;
+        const size_t row = (size_t)(i0 + rl);
+        VW[row + (size_t)q * ldp] = Vs[q][rl];
+        VW[row + (size_t)(kNB + q) * ldp] = Ws[q][rl];
+        VW[row + (size_t)(2 * kNB + q) * ldp] = Vs[q][rl];
     }
 }
 
@@ -223,6 +251,16 @@ __global__ __launch_bounds__(kThreads) void trd_tail_kernel(double *__restrict__
     }
 }
 
+// C(i, j) = C(j, i) for i > j: the upper triangle defines the matrix, as in
+// LAPACK's uplo = 'U' (a product like U^-T H U^-1 is symmetric only up to
+// rounding, and the reduction below reads both triangles)
+__global__ void mirror_upper_kernel(double *__restrict__ C, int n, int lda) {
+    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= (int64_t)n * n) return;
+    const int i = (int)(idx % n), j = (int)(idx / n);
+    if (i > j) C[i + (int64_t)j * lda] = C[j + (int64_t)i * lda];
+}
+
 #define QM_EIG_BLAS(call)                                                                                   \
     do {                                                                                                    \
         rocblas_status s_ = (call);                                                                         \
@@ -236,16 +274,16 @@ __global__ __launch_bounds__(kThreads) void trd_tail_kernel(double *__restrict__
 
 // C = Q T Q^T for the symmetric n x n matrix C (column-major, lda >= n, both
 // triangles; overwritten with the reflectors).  trd holds d, e, tau (3 n);
-// VW is n x 3*32 workspace.  Runs on the handle's stream.
+// VW is n x 48 workspace.  n <= 512.  Runs on the handle's stream.
 int sytrd_lower(void *blas, int n, double *C, int lda, double *trd, double *VW, hipStream_t st) {
     rocblas_handle h = (rocblas_handle)blas;
-    static_assert(3 * kNB == kTrdPanelCols, "VW workspace width");
-    QM_CHECK(n >= 1 && n <= kMaxN, QM_ERR_ARG, "tridiagonal reduction: n must be in [1, 1024]");
+    static_assert(3 * kNB <= kTrdPanelCols, "VW workspace width");
+    QM_CHECK(n >= 1 && n <= kMaxN, QM_ERR_ARG, "tridiagonal reduction: n must be in [1, 512]");
     double *d = trd, *e = trd + n, *tau = trd + 2 * n;
     int i0 = 0;
     const double minus1 = -1.0, one = 1.0;
     while (n - i0 > kTailMax) {
-        hipLaunchKernelGGL(trd_panel_kernel, dim3(1), dim3(kThreads), 0, st, C, lda, n, i0, kNB, d, e, tau, VW, n);
+        hipLaunchKernelGGL(trd_panel_kernel, dim3(1), dim3(kThreads), 0, st, C, lda, n, i0, d, e, tau, VW, n);
         QM_HIP(hipGetLastError());
         const int m2 = n - i0 - kNB;
         QM_EIG_BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, m2, m2, 2 * kNB, &minus1,
@@ -259,11 +297,22 @@ int sytrd_lower(void *blas, int n, double *C, int lda, double *trd, double *VW, 
 }
 
 // Eigenvalues (all, ascending, into w) and the eigenvectors of the nev lowest
-// into Z[:, :nev] (ldz >= n) of the symmetric C (overwritten).  Z must hold
-// n x n for stedc.  info receives rocSOLVER's stedc status.
+// into Z[:, :nev] (ldz >= n) of the symmetric matrix whose upper triangle C
+// holds (C overwritten).
+// n <= kMaxN: own tridiagonal reduction, rocSOLVER stedc + ormtr (Z must hold
+// n x n); larger n: rocSOLVER syevd.  info receives rocSOLVER's status.
 int syev_lower(void *blas, int n, int nev, double *C, int lda, double *w, double *Z, int ldz, double *trd,
                double *VW, int *info, hipStream_t st) {
     rocblas_handle h = (rocblas_handle)blas;
+    if (n > kMaxN) {
+        QM_EIG_BLAS(rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, n, C, lda, w, trd, info));
+        QM_HIP(hipMemcpy2DAsync(Z, sizeof(double) * ldz, C, sizeof(double) * lda, sizeof(double) * n, nev,
+                                hipMemcpyDeviceToDevice, st));
+        return QM_OK;
+    }
+    hipLaunchKernelGGL(mirror_upper_kernel, dim3((unsigned)(((int64_t)n * n + 255) / 256)), dim3(256), 0, st, C, n,
+                       lda);
+    QM_HIP(hipGetLastError());
     QM_TRY(sytrd_lower(blas, n, C, lda, trd, VW, st));
     QM_HIP(hipMemcpyAsync(w, trd, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
     QM_EIG_BLAS(rocsolver_dstedc(h, rocblas_evect_tridiagonal, n, w, trd + n, Z, ldz, info));
@@ -297,7 +346,7 @@ extern "C" {
 int qm_sygvd_f64(qm_ctx *ctx, int n, const double *d_A, const double *d_B, double *d_w, double *d_Z, double *d_U,
                  int nev, void *stream) {
     QM_CHECK(ctx && d_A && d_B && d_w && d_Z && d_U, QM_ERR_ARG, "qm_sygvd_f64: NULL argument");
-    QM_CHECK(n >= 1 && n <= kMaxN, QM_ERR_ARG, "qm_sygvd_f64: n must be in [1, 1024]");
+    QM_CHECK(n >= 1 && n <= 4096, QM_ERR_ARG, "qm_sygvd_f64: n must be in [1, 4096]");
     QM_CHECK(nev >= 1 && nev <= n, QM_ERR_ARG, "qm_sygvd_f64: nev must be in [1, n]");
     QM_TRY(activate(ctx));
     hipStream_t st = (hipStream_t)stream;

@@ -90,66 +90,54 @@ __device__ __forceinline__ double lane_read(double v, int addr) {
 }
 
 // SEG: nx divides 64 and the wave holds 64/nx whole rows side by side.
-// A block is WY waves stacked in y over one (band, z chunk, x window): wave k
-// owns block rows (k*S + seg)*RY .. +RY (S = 64/nx row segments per wave in
-// SEG mode, else 1).  Each plane, every wave puts its rows' centre values into
-// LDS (double-buffered by plane parity, one barrier per plane); y halo rows
-// inside the block come from there, only rows outside it from global memory.
 // Queue slots: at step phase P (plane w = zz + P), logical slot m of pq holds
 // plane w + m and logical slot m of dz holds D(w - 4 + m); physical slot is
 // (m + P) & 7.
-template <typename T, int NC, bool KT, int RY, bool SEG, int WY>
-__global__ __launch_bounds__(64 * WY) void fd8_wave_kernel(T *__restrict__ out, const T *__restrict__ in,
-                                                           const T *__restrict__ vloc, int nx, int ny, int nz,
-                                                           int zchunk, int nzc, int nyb, int nxt, int nblk,
-                                                           int per_xcd, FD8Coef<T> c) {
-    constexpr int BRMAX = WY * RY * 64;  // block rows x LDS row width <= this
-    __shared__ T sm[2][NC][BRMAX];
+template <typename T, int NC, bool KT, int RY, bool SEG>
+__global__ __launch_bounds__(256) void fd8_wave_kernel(T *__restrict__ out, const T *__restrict__ in,
+                                                       const T *__restrict__ vloc, int nx, int ny, int nz,
+                                                       int zchunk, int nzc, int nyt, int nxt, int nwaves, int nblk,
+                                                       int per_xcd, FD8Coef<T> c) {
     // hardware block b runs on XCD b % 8: give each XCD a contiguous range of tiles
     const int b = blockIdx.x;
     const int lb = (b & 7) * per_xcd + (b >> 3);
-    if (lb >= nblk) return;  // block-uniform
+    if (lb >= nblk) return;
     const int lane = threadIdx.x & (WAVE - 1);
-    const int k = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));  // wave in block
-    int t = lb;
-    const int yb = t % nyb;
-    t /= nyb;
+    // the wave index is uniform: keep everything derived from it (tile, band,
+    // buffer resources, plane offsets) in scalar registers
+    const int gw = lb * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    if (gw >= nwaves) return;
+    int t = gw;
     const int xt = t % nxt;
     t /= nxt;
+    const int yt = t % nyt;
+    t /= nyt;
     const int zc = t % nzc;
     const int band = t / nzc;
 
     // lanes holding x+1, x+2, x+3 (la[0..2]) and x-1 .. x-4 (lb_[0..3])
-    int x, xl, b0, S, LW, la[3], lb_[4];
+    int x, y0, la[3], lb_[4];
     bool xout;
     if constexpr (SEG) {
-        S = WAVE / nx;
+        const int S = WAVE / nx;
         const int seg = lane / nx;
         x = lane - seg * nx;
-        xl = x;
-        LW = nx;
-        b0 = (k * S + seg) * RY;
+        y0 = (yt * S + seg) * RY;
         xout = true;
 #pragma unroll
         for (int i = 0; i < 3; ++i) la[i] = (seg * nx + wrapi(x + i + 1, nx)) * 4;
 #pragma unroll
         for (int i = 0; i < 4; ++i) lb_[i] = (seg * nx + wrapi(x - i - 1, nx)) * 4;
     } else {
-        S = 1;
         const int x0 = xt * XW;
         x = wrapi(x0 - HW + lane, nx);
-        xl = lane;
-        LW = WAVE;
-        b0 = k * RY;
+        y0 = yt * RY;
         xout = lane >= HW && lane < WAVE - HW && x0 + lane - HW < nx;
 #pragma unroll
         for (int i = 0; i < 3; ++i) la[i] = min(lane + i + 1, WAVE - 1) * 4;
 #pragma unroll
         for (int i = 0; i < 4; ++i) lb_[i] = max(lane - i - 1, 0) * 4;
     }
-    const int BR = WY * S * RY;  // rows of the block
-    const int ybase = yb * BR;
-    const int y0 = ybase + b0;
     const int64_t plane = (int64_t)nx * ny;
     const int64_t ng = plane * nz;
     const int pbytes = (int)(plane * sizeof(T));
@@ -173,18 +161,10 @@ __global__ __launch_bounds__(64 * WY) void fd8_wave_kernel(T *__restrict__ out, 
         rowoff[r] = (x + nx * wrapi(y0 + r, ny)) * (int)sizeof(T);
         rowok[r] = xout && (y0 + r < ny);
     }
-    // halo rows y0-4..y0-1 and y0+RY..y0+RY+3: LDS slot inside the block, or a
-    // global byte offset (hsl < 0) outside it
-    int hoff[8], hsl[8];
-    bool hglob = false;
+    int hoff[8];  // rows y0-4..y0-1 and y0+RY..y0+RY+3
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        const int g = wrapi(j < 4 ? y0 - 4 + j : y0 + RY + (j - 4), ny);
-        hoff[j] = (x + nx * g) * (int)sizeof(T);
-        hsl[j] = (g - ybase >= 0 && g - ybase < BR) ? (g - ybase) * LW + xl : -1;
-        hglob = hglob || hsl[j] < 0;
-    }
-    const bool wglob = __any(hglob);  // some lane of the wave needs global halo rows
+    for (int j = 0; j < 8; ++j)
+        hoff[j] = (x + nx * wrapi(j < 4 ? y0 - 4 + j : y0 + RY + (j - 4), ny)) * (int)sizeof(T);
 
     T pq[NC][RY][8], dz[NC][RY][8];
     // planes zb .. zb+7 into pq; D(zb-4) .. D(zb+2) into dz
@@ -218,18 +198,13 @@ __global__ __launch_bounds__(64 * WY) void fd8_wave_kernel(T *__restrict__ out, 
                 dz[cc][r][7] = T(0);
             }
     }
-    T gh[NC][8], vv[RY];  // global halo rows (edge waves only) and V of the current plane
+    T hy[NC][8], vv[RY];
     {
         const int so = zb * pbytes;
-        if (wglob) {
 #pragma unroll
-            for (int cc = 0; cc < NC; ++cc)
+        for (int cc = 0; cc < NC; ++cc)
 #pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    gh[cc][j] = T(0);
-                    if (hsl[j] < 0) bload(gh[cc][j], rin[cc], hoff[j], so);
-                }
-        }
+            for (int j = 0; j < 8; ++j) bload(hy[cc][j], rin[cc], hoff[j], so);
 #pragma unroll
         for (int r = 0; r < RY; ++r) bload(vv[r], rv, rowoff[r], so);
     }
@@ -238,25 +213,15 @@ __global__ __launch_bounds__(64 * WY) void fd8_wave_kernel(T *__restrict__ out, 
 #pragma unroll
         for (int P = 0; P < 8; ++P) {
             const int w = zz + P;
-            if (w < ze) {  // block-uniform
-                // share this plane's centre values with the other waves of the block
-#pragma unroll
-                for (int cc = 0; cc < NC; ++cc)
-#pragma unroll
-                    for (int r = 0; r < RY; ++r) sm[w & 1][cc][(b0 + r) * LW + xl] = pq[cc][r][P & 7];
-                // global halo rows and V of plane w + 1
+            if (w < ze) {  // wave-uniform
+                // halo rows and V of plane w + 1
                 T nh[NC][8], nv[RY];
                 if (w + 1 < ze) {
                     const int so = (w + 1) * pbytes;
-                    if (wglob) {
 #pragma unroll
-                        for (int cc = 0; cc < NC; ++cc)
+                    for (int cc = 0; cc < NC; ++cc)
 #pragma unroll
-                            for (int j = 0; j < 8; ++j) {
-                                nh[cc][j] = T(0);
-                                if (hsl[j] < 0) bload(nh[cc][j], rin[cc], hoff[j], so);
-                            }
-                    }
+                        for (int j = 0; j < 8; ++j) bload(nh[cc][j], rin[cc], hoff[j], so);
 #pragma unroll
                     for (int r = 0; r < RY; ++r) bload(nv[r], rv, rowoff[r], so);
                 }
@@ -266,21 +231,16 @@ __global__ __launch_bounds__(64 * WY) void fd8_wave_kernel(T *__restrict__ out, 
 #pragma unroll
                     for (int r = 0; r < RY; ++r)
                         dz[cc][r][(7 + P) & 7] = pq[cc][r][(4 + P) & 7] - pq[cc][r][(3 + P) & 7];
-                __syncthreads();
                 // y differences of the lane's column: rows y0-4 .. y0+RY+3
                 T dy[NC][RY + 7];
 #pragma unroll
                 for (int cc = 0; cc < NC; ++cc) {
                     T e[RY + 8];
 #pragma unroll
-                    for (int j = 0; j < 8; ++j) {
-                        const T hv = hsl[j] >= 0 ? sm[w & 1][cc][hsl[j] >= 0 ? hsl[j] : 0] : gh[cc][j];
-                        e[j < 4 ? j : RY + j] = hv;
-                    }
+                    for (int k = 0; k < RY + 8; ++k)
+                        e[k] = k < 4 ? hy[cc][k] : (k < RY + 4 ? pq[cc][k - 4][P & 7] : hy[cc][k - RY]);
 #pragma unroll
-                    for (int r = 0; r < RY; ++r) e[4 + r] = pq[cc][r][P & 7];
-#pragma unroll
-                    for (int q = 0; q < RY + 7; ++q) dy[cc][q] = e[q + 1] - e[q];
+                    for (int k = 0; k < RY + 7; ++k) dy[cc][k] = e[k + 1] - e[k];
                 }
                 const int so = w * pbytes;
 #pragma unroll
@@ -336,12 +296,10 @@ __global__ __launch_bounds__(64 * WY) void fd8_wave_kernel(T *__restrict__ out, 
                         for (int r = 0; r < RY; ++r) bload(pq[cc][r][P & 7], rin[cc], rowoff[r], sf);
                 }
                 if (w + 1 < ze) {
-                    if (wglob) {
 #pragma unroll
-                        for (int cc = 0; cc < NC; ++cc)
+                    for (int cc = 0; cc < NC; ++cc)
 #pragma unroll
-                            for (int j = 0; j < 8; ++j) gh[cc][j] = nh[cc][j];
-                    }
+                        for (int j = 0; j < 8; ++j) hy[cc][j] = nh[cc][j];
 #pragma unroll
                     for (int r = 0; r < RY; ++r) vv[r] = nv[r];
                 }
@@ -380,32 +338,32 @@ FD8Coef<T> make_coef(const qm_ctx *ctx, const double k[3]) {
 
 struct Candidate {
     int ry;      // rows per lane
-    int wy;      // waves stacked in y per block
-    int zchunk;  // z planes per block (a multiple of 8)
+    int zchunk;  // z planes per wave (a multiple of 8)
 };
 constexpr Candidate kCandidates[] = {
-    {4, 8, 32}, {4, 8, 16}, {4, 8, 64}, {2, 16, 32}, {2, 16, 64}, {4, 4, 32}, {8, 4, 32}, {8, 8, 64},
+    {4, 32}, {2, 16}, {2, 32}, {4, 16}, {4, 64}, {8, 16}, {8, 32}, {8, 64},
 };
 constexpr int kNumCandidates = sizeof(kCandidates) / sizeof(kCandidates[0]);
 
-template <typename T, int NC, bool KT, int RY, int WY>
+template <typename T, int NC, bool KT, int RY>
 int launch_wave(const qm_ctx *ctx, T *out, const T *in, const T *vloc, int nb, int zchunk, const FD8Coef<T> &c,
                 hipStream_t st) {
     const int nx = ctx->nx, ny = ctx->ny, nz = ctx->nz;
     const bool seg = WAVE % nx == 0;
-    const int rows = (seg ? WAVE / nx : 1) * RY * WY;  // y rows per block
+    const int rows = seg ? (WAVE / nx) * RY : RY;  // y rows per wave
     const int nxt = seg ? 1 : (nx + XW - 1) / XW;
-    const int nyb = (ny + rows - 1) / rows;
+    const int nyt = (ny + rows - 1) / rows;
     const int nzc = (nz + zchunk - 1) / zchunk;
-    const int64_t nblk = (int64_t)nb * nzc * nyb * nxt;
-    QM_CHECK(nblk < (int64_t)INT32_MAX / 2, QM_ERR_ARG, "grid too large for one stencil launch");
-    const int per_xcd = (int)((nblk + 7) / 8);
+    const int64_t nwaves = (int64_t)nb * nzc * nyt * nxt;
+    QM_CHECK(nwaves < (int64_t)INT32_MAX / 2, QM_ERR_ARG, "grid too large for one stencil launch");
+    const int nblk = (int)((nwaves + 3) / 4);
+    const int per_xcd = (nblk + 7) / 8;
     if (seg)
-        hipLaunchKernelGGL((fd8_wave_kernel<T, NC, KT, RY, true, WY>), dim3(8 * per_xcd), dim3(64 * WY), 0, st, out,
-                           in, vloc, nx, ny, nz, zchunk, nzc, nyb, nxt, (int)nblk, per_xcd, c);
+        hipLaunchKernelGGL((fd8_wave_kernel<T, NC, KT, RY, true>), dim3(8 * per_xcd), dim3(256), 0, st, out, in,
+                           vloc, nx, ny, nz, zchunk, nzc, nyt, nxt, (int)nwaves, nblk, per_xcd, c);
     else
-        hipLaunchKernelGGL((fd8_wave_kernel<T, NC, KT, RY, false, WY>), dim3(8 * per_xcd), dim3(64 * WY), 0, st, out,
-                           in, vloc, nx, ny, nz, zchunk, nzc, nyb, nxt, (int)nblk, per_xcd, c);
+        hipLaunchKernelGGL((fd8_wave_kernel<T, NC, KT, RY, false>), dim3(8 * per_xcd), dim3(256), 0, st, out, in,
+                           vloc, nx, ny, nz, zchunk, nzc, nyt, nxt, (int)nwaves, nblk, per_xcd, c);
     QM_HIP(hipGetLastError());
     return QM_OK;
 }
@@ -414,17 +372,12 @@ template <typename T, int NC, bool KT>
 int launch_candidate(const qm_ctx *ctx, int cand, T *out, const T *in, const T *vloc, int nb, const FD8Coef<T> &c,
                      hipStream_t st) {
     const Candidate cd = kCandidates[cand];
-    if constexpr (sizeof(T) == 8 || NC == 2) {
-        // FP64 or complex bands need twice the registers per row: 2 or 4 rows
-        // per lane, 4 waves per block (larger shapes spill)
-        if (cd.ry == 2) return launch_wave<T, NC, KT, 2, 4>(ctx, out, in, vloc, nb, cd.zchunk, c, st);
-        return launch_wave<T, NC, KT, 4, 4>(ctx, out, in, vloc, nb, cd.zchunk, c, st);
-    } else {
-        if (cd.ry == 2) return launch_wave<T, NC, KT, 2, 16>(ctx, out, in, vloc, nb, cd.zchunk, c, st);
-        if (cd.ry == 4 && cd.wy == 8) return launch_wave<T, NC, KT, 4, 8>(ctx, out, in, vloc, nb, cd.zchunk, c, st);
-        if (cd.ry == 4) return launch_wave<T, NC, KT, 4, 4>(ctx, out, in, vloc, nb, cd.zchunk, c, st);
-        if (cd.wy == 8) return launch_wave<T, NC, KT, 8, 8>(ctx, out, in, vloc, nb, cd.zchunk, c, st);
-        return launch_wave<T, NC, KT, 8, 4>(ctx, out, in, vloc, nb, cd.zchunk, c, st);
+    // 8 rows per lane would spill registers for FP64 or complex bands: 4 there
+    constexpr int RY8 = (sizeof(T) == 8 || NC == 2) ? 4 : 8;
+    switch (cd.ry) {
+        case 2: return launch_wave<T, NC, KT, 2>(ctx, out, in, vloc, nb, cd.zchunk, c, st);
+        case 4: return launch_wave<T, NC, KT, 4>(ctx, out, in, vloc, nb, cd.zchunk, c, st);
+        default: return launch_wave<T, NC, KT, RY8>(ctx, out, in, vloc, nb, cd.zchunk, c, st);
     }
 }
 
@@ -511,9 +464,8 @@ int stencil_num_candidates() { return kNumCandidates; }
 
 std::string stencil_candidate_name(int cand) {
     const Candidate cd = kCandidates[cand];
-    return "fd8_wave_kernel: RY " + std::to_string(cd.ry) + ", " + std::to_string(cd.wy) +
-           " waves stacked in y, z chunk " + std::to_string(cd.zchunk) +
-           " (FP64 / complex: RY " + std::to_string(cd.ry == 2 ? 2 : 4) + ", 4 waves)";
+    return "fd8_wave_kernel: RY " + std::to_string(cd.ry) + " (4 for FP64 / complex when 8), z chunk " +
+           std::to_string(cd.zchunk);
 }
 
 int stencil_choice(const qm_ctx *ctx, int precision, int ncomp, int kt, int nb) {
