@@ -89,6 +89,41 @@ def cpu_baseline(s, psi_sample, min_seconds=10.0):
                       f"median of {len(times)} applies ({sum(times):.1f} s)"}
 
 
+def hpsi_config_leg(qm, torch, cfg, steps, warmup):
+    """FP32 H.psi throughput of a larger workload (C2: 96^3, 256 complex bands,
+    4 k-points; C3: 160^3, 512 real bands, uneven spheres), summed over its
+    k-points, with the stencil's HBM fraction.  These grids exceed the 256 MB
+    Infinity Cache, so the fraction is an HBM one.  psi is random device data
+    (the operator's cost does not depend on the values)."""
+    s = qm.system.config(cfg)
+    nb, c = s.nbands, s.ncomp
+    H = qm.hamiltonian.Hamiltonian(s, device=0)
+    g = torch.Generator(device="cuda:0").manual_seed(cfg)
+    x = torch.rand((nb * c, s.ng), device="cuda:0", dtype=torch.float32, generator=g) - 0.5
+    y = torch.empty_like(x)
+    t_full = t_local = 0.0
+    per_k = []
+    for ik in range(s.kpoints.shape[0]):
+        for _ in range(warmup):  # the first apply of a shape tunes the stencil
+            H.apply(y, x, k_index=ik)
+        ms = _time_applies(torch, lambda: H.apply(y, x, k_index=ik), steps)
+        ms_l = _time_applies(torch, lambda: H.apply(y, x, k_index=ik, part="local"), steps)
+        t_full += ms
+        t_local += ms_l
+        per_k.append({"ms_full": ms, "ms_local": ms_l})
+    nk = s.kpoints.shape[0]
+    bytes_local = nb * s.ng * c * 2 * 4 + s.ng * 4
+    ach = bytes_local * nk / (t_local * 1e-3) / 1e9
+    H.close()
+    del x, y
+    torch.cuda.empty_cache()
+    return {"workload": f"{s.name}: {s.natoms} atoms, {s.nx}^3 grid, {nb} bands x {c} components, {nk} k-point(s)",
+            "value": nk * s.ng * nb / (t_full * 1e-3), "unit": "grid-pt·band/s",
+            "ms_per_apply_all_k": t_full, "per_k": per_k,
+            "stencil_roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                 "frac": ach / HBM_PEAK_GBS, "bytes_per_launch": bytes_local}}
+
+
 def lobpcg_leg(qm, torch, H, s, psi64, calls, iters=20):
     """LOBPCG s/iteration of the mixed (FP32 vectors / FP64 subspace) and the
     plain FP64 solver from the same start, and the mixed-vs-FP64 eigenvalue
@@ -135,6 +170,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 512 MiB device copy (the scale for rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--configs", default="2,3",
+                    help="larger workloads whose FP32 H.psi is also timed (comma list of 2, 3; empty = none)")
     ap.add_argument("--lobpcg-calls", type=int, default=10,
                     help="LOBPCG calls of 20 iterations per precision (0 skips the LOBPCG leg)")
     args = ap.parse_args()
@@ -220,6 +257,12 @@ def main():
     del x, y
     if args.lobpcg_calls > 0:
         out["lobpcg"] = lobpcg_leg(qm, torch, H, s, psi64, args.lobpcg_calls)
+    other = {}
+    for cfg in [int(c) for c in args.configs.split(",") if c.strip()]:
+        print(f"[bench] H.psi on config {cfg}", file=sys.stderr, flush=True)
+        other[f"C{cfg}"] = hpsi_config_leg(qm, torch, cfg, max(5, args.steps // 4), max(1, args.warmup))
+    if other:
+        out["other_workloads_hpsi_fp32"] = other
     if not args.no_cpu:
         sample = psi64[:16 * s.ncomp]
         out["cpu_baseline"] = cpu_baseline(s, np.ascontiguousarray(sample))

@@ -41,13 +41,15 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // part[(item * 16 + w) * ncol + col] = <p_w | e^{ik.r} psi>_col summed over the
-// item's voxels (col = real SoA column n * NC + c).  Each wave takes 256 voxels
-// in sub-chunks of 64: the 64 x 16 block of (phase-rotated) psi values is
-// gathered with lanes along voxels (coalesced along x runs) into LDS, then fed to
-// MFMA as the B operand against the voxel-major projector table (A operand).
+// item's voxels (col = real SoA column n * NC + c), for a tile of 16 * CT
+// columns.  Each wave takes kProjChunk / 4 voxels in sub-chunks of 64: the
+// 64 x 16CT block of (phase-rotated) psi values is gathered with lanes along
+// voxels (coalesced along x runs) into LDS, then fed to MFMA as the B operand
+// of CT column tiles that share one A operand, the voxel-major projector table
+// (so the table is read once per 16 CT columns).
 constexpr int PJ_VPW = kProjChunk / 4;  // voxels per wave
 constexpr int PJ_SUB = 64;              // voxels per LDS sub-chunk
-template <typename T, int NC, bool PH>
+template <typename T, int NC, bool PH, int CT>
 __global__ __launch_bounds__(256) void nl_project_kernel(double *__restrict__ part, const T *__restrict__ psi,
                                                          int64_t ng, int ncol, const int *__restrict__ chunk_atom,
                                                          const int *__restrict__ chunk_v0,
@@ -56,19 +58,22 @@ __global__ __launch_bounds__(256) void nl_project_kernel(double *__restrict__ pa
                                                          const T *__restrict__ phase) {
     using M = Mfma<T>;
     using acc_t = typename M::acc_t;
-    __shared__ T stage[4][PJ_SUB][17];
-    __shared__ double red[4][256];
+    constexpr int NCOL = 16 * CT;
+    __shared__ T stage[4][PJ_SUB][NCOL + 1];
+    __shared__ double red[4][16 * NCOL];
     const int item = blockIdx.x;
-    const int tile = blockIdx.y;
+    const int c0 = blockIdx.y * NCOL;
     const int a = chunk_atom[item];
     const int64_t vg0 = vox_off[a];
     const int V = (int)(vox_off[a + 1] - vg0);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int j = lane & 15, g = lane >> 4;
     const int vb = chunk_v0[item] + wave * PJ_VPW;
-    acc_t acc = acc_t{0, 0, 0, 0};
+    acc_t acc[CT];
+#pragma unroll
+    for (int t = 0; t < CT; ++t) acc[t] = acc_t{0, 0, 0, 0};
     for (int sc = 0; sc < PJ_VPW / PJ_SUB; ++sc) {
-        // gather: lane = voxel, 16 columns
+        // gather: lane = voxel, NCOL columns
         {
             const int v = vb + sc * PJ_SUB + lane;
             const bool in = v < V;
@@ -80,8 +85,8 @@ __global__ __launch_bounds__(256) void nl_project_kernel(double *__restrict__ pa
                 sn = phase[2 * vid + 1];
             }
 #pragma unroll
-            for (int jj = 0; jj < 16; jj += NC) {
-                const int col = tile * 16 + jj;
+            for (int jj = 0; jj < NCOL; jj += NC) {
+                const int col = c0 + jj;
                 T re = T(0), im = T(0);
                 if (in && col < ncol) {
                     const T *p = psi + (int64_t)col * ng;  // column col is band col / NC, component 0
@@ -107,18 +112,22 @@ __global__ __launch_bounds__(256) void nl_project_kernel(double *__restrict__ pa
             const int vl = 4 * s + g;
             const int v = vb + sc * PJ_SUB + vl;
             const T av = v < V ? tab16[(vg0 + v) * kMaxProj + j] : T(0);  // A[w = j][k = g]
-            acc = M::mma(av, stage[wave][vl][j], acc);                    // B[k = g][col = j]
+#pragma unroll
+            for (int t = 0; t < CT; ++t) acc[t] = M::mma(av, stage[wave][vl][16 * t + j], acc[t]);  // B[k = g][col = j]
         }
         __syncthreads();
     }
 #pragma unroll
-    for (int r = 0; r < 4; ++r) red[wave][M::row(g, r) * 16 + j] = (double)acc[r];
+    for (int t = 0; t < CT; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) red[wave][(M::row(g, r) * CT + t) * 16 + j] = (double)acc[t][r];
     __syncthreads();
-    const int t = threadIdx.x;
-    const int w = t >> 4, jj = t & 15;
-    if (tile * 16 + jj < ncol) {
-        const double sum = (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
-        part[((int64_t)item * kMaxProj + w) * ncol + tile * 16 + jj] = sum;
+    for (int e = threadIdx.x; e < 16 * NCOL; e += 256) {
+        const int w = e / NCOL, cl = e % NCOL;  // red layout: (w * CT + t) * 16 + j = w * NCOL + cl
+        if (c0 + cl < ncol) {
+            const double sum = (red[0][e] + red[1][e]) + (red[2][e] + red[3][e]);
+            part[((int64_t)item * kMaxProj + w) * ncol + c0 + cl] = sum;
+        }
     }
 }
 
@@ -350,9 +359,13 @@ int nonlocal_apply(qm_ctx *ctx, T *hpsi, const T *psi, int nb, const T *phase, h
     QM_TRY(ctx->coef.reserve(sizeof(T) * (size_t)p.ptot * ncol));
     QM_TRY(ctx->nl_part.reserve(sizeof(double) * (size_t)p.nchunk * kMaxProj * ncol));
     T *coef = ctx->coef.as<T>();
-    const int tiles = (ncol + 15) / 16;
+    // one 16-column tile per projection workgroup: wider tiles (32 / 64 columns
+    // sharing one table read) measured 2-3x slower on 64^3 x 128, from fewer and
+    // heavier workgroups (LDS limits them to one per CU)
+    constexpr int CT = 1;
+    const int tiles = (ncol + 16 * CT - 1) / (16 * CT);
     QM_CHECK(tiles <= 65535, QM_ERR_ARG, "nonlocal apply: too many bands per call");
-    hipLaunchKernelGGL((nl_project_kernel<T, NC, PH>), dim3(p.nchunk, tiles), dim3(256), 0, st,
+    hipLaunchKernelGGL((nl_project_kernel<T, NC, PH, CT>), dim3(p.nchunk, tiles), dim3(256), 0, st,
                        ctx->nl_part.as<double>(), psi, ctx->ng, ncol, p.d_chunk_atom, p.d_chunk_v0, p.d_vox_off,
                        p.d_vox_grid, tab16_of<T>(p), phase);
     QM_HIP(hipGetLastError());
@@ -361,6 +374,8 @@ int nonlocal_apply(qm_ctx *ctx, T *hpsi, const T *psi, int nb, const T *phase, h
                        ctx->nl_part.as<double>(), ncol, p.ptot, p.d_proj_atom, p.d_proj_off, p.d_atom_chunk_off,
                        p.d_energy, ctx->hx * ctx->hy * ctx->hz);
     QM_HIP(hipGetLastError());
+    // columns per scatter thread (the table row and entry list are read once per
+    // NBS columns); 64 measured 1.9x slower than 16 on 64^3 x 128
     constexpr int NBS = 16;
     const int stiles = (ncol + NBS - 1) / NBS;
     hipLaunchKernelGGL((nl_scatter_kernel<T, NC, PH, NBS>), dim3(p.nsblock, stiles), dim3(kScatterBlock), 0, st, hpsi,

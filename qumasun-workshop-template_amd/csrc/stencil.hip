@@ -285,8 +285,9 @@ __global__ __launch_bounds__(256) void fd8_wave_kernel(T *__restrict__ out, cons
                         for (int cc = 0; cc < NC; ++cc) bstore(lap[cc], rout[cc], rowoff[r], so);
                     }
                 }
-                // plane w + 8 into the slot plane w leaves
-                {
+                // plane w + 8 into the slot plane w leaves; the chunk's last step
+                // reads D(ze + 2), so planes past ze + 3 are never needed
+                if (w + 8 <= ze + 3) {
                     int pf = w + 8;
                     while (pf >= nz) pf -= nz;
                     const int sf = pf * pbytes;
