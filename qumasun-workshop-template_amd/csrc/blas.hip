@@ -42,17 +42,29 @@ constexpr int GRAM_FLUSH = 16;  // MFMA K-steps between FP32 -> FP64 flushes
 
 // Wave tile 32 x 32 (2 x 2 MFMA tiles), workgroup tile 64 x 64 (4 waves).
 // partial[chunk][j * m + i] = sum_{k in chunk} A[k][i] B[k][j]
+// Hardware workgroup b runs on XCD b % 8.  Workgroups are renumbered so that
+// consecutive numbers (the tiles of one K chunk) share an XCD and read the
+// chunk's rows of A and B from that XCD's L2 instead of once per XCD.
+__device__ __forceinline__ int xcd_block(int nblk, int per_xcd) {
+    const int b = blockIdx.x;
+    const int lb = (b & 7) * per_xcd + (b >> 3);
+    return lb < nblk ? lb : -1;
+}
+
 template <typename T, bool VECLOAD>
 __global__ __launch_bounds__(256) void gram_partial_kernel(ColBlocks A, ColBlocks B, int m, int n, int64_t ng,
-                                                           int64_t kchunk, int tiles_n, double *__restrict__ partial) {
+                                                           int64_t kchunk, int tiles_n, int tiles, int nblk,
+                                                           int per_xcd, double *__restrict__ partial) {
     using M = Mfma<T>;
     using acc_t = typename M::acc_t;
     using vec_t = typename M::vec_t;
     constexpr int VEC = M::VEC;
     constexpr int KSTEP = 4 * VEC;
-    const int tile = blockIdx.x;
+    const int lb = xcd_block(nblk, per_xcd);
+    if (lb < 0) return;
+    const int tile = lb % tiles;
     const int ti = tile / tiles_n, tj = tile % tiles_n;
-    const int chunk = blockIdx.y;
+    const int chunk = lb / tiles;
     const int64_t k0 = (int64_t)chunk * kchunk;
     const int64_t k1 = min(k0 + kchunk, ng);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -166,12 +178,14 @@ __global__ void gram_reduce_kernel(double *__restrict__ C, const double *__restr
 constexpr int UPD_KC = 32;
 template <typename T>
 __global__ __launch_bounds__(256) void update_kernel(ColBlocks Z, ColBlocks Y, const T *__restrict__ Cm, int m,
-                                                     int n, int64_t ng, int tiles_n) {
+                                                     int n, int64_t ng, int tiles_n, int nblk, int per_xcd) {
     using M = Mfma<T>;
     using acc_t = typename M::acc_t;
     __shared__ T cs[UPD_KC][64];
-    const int tj = blockIdx.x % tiles_n;   // output-column tile (fastest: neighbours share Z rows in L2)
-    const int64_t gt = blockIdx.x / tiles_n;
+    const int lb = xcd_block(nblk, per_xcd);
+    if (lb < 0) return;
+    const int tj = lb % tiles_n;   // output-column tile (fastest: neighbours share Z rows in one XCD's L2)
+    const int64_t gt = lb / tiles_n;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int li = lane & 15, g = lane >> 4;
     const int c0 = tj * 64;
@@ -313,13 +327,16 @@ int gram(DevBuf &partial, double *C, const ColBlocks &A, const ColBlocks &B, int
     nchunk = (ng + kchunk - 1) / kchunk;
     QM_TRY(partial.reserve(sizeof(double) * (size_t)nchunk * m * n));
     const bool vec = (ng % Mfma<T>::VEC) == 0;
-    dim3 grid(tiles, (unsigned)nchunk);
+    const int64_t nblk = (int64_t)tiles * nchunk;
+    QM_CHECK(nblk < ((int64_t)1 << 30), QM_ERR_ARG, "gram: grid too large");
+    const int per_xcd = (int)((nblk + 7) / 8);
+    const dim3 grid((unsigned)(8 * per_xcd));
     if (vec)
         hipLaunchKernelGGL((gram_partial_kernel<T, true>), grid, dim3(256), 0, st, A, B, m, n, ng, kchunk, tiles_n,
-                           partial.as<double>());
+                           tiles, (int)nblk, per_xcd, partial.as<double>());
     else
         hipLaunchKernelGGL((gram_partial_kernel<T, false>), grid, dim3(256), 0, st, A, B, m, n, ng, kchunk, tiles_n,
-                           partial.as<double>());
+                           tiles, (int)nblk, per_xcd, partial.as<double>());
     QM_HIP(hipGetLastError());
     const int64_t mn = (int64_t)m * n;
     hipLaunchKernelGGL(gram_reduce_kernel, dim3((unsigned)((mn + 255) / 256)), dim3(256), 0, st, C,
@@ -332,9 +349,11 @@ template <typename T>
 int update(const ColBlocks &Z, const ColBlocks &Y, const T *Cm, int m, int n, int64_t ng, hipStream_t st) {
     const int tiles_n = (n + 63) / 64;
     const int64_t gtiles = (ng + 127) / 128;
-    QM_CHECK(gtiles * tiles_n < ((int64_t)1 << 31), QM_ERR_ARG, "update: grid too large");
-    hipLaunchKernelGGL((update_kernel<T>), dim3((unsigned)(gtiles * tiles_n)), dim3(256), 0, st, Z, Y, Cm, m, n, ng,
-                       tiles_n);
+    const int64_t nblk = gtiles * tiles_n;
+    QM_CHECK(nblk < ((int64_t)1 << 30), QM_ERR_ARG, "update: grid too large");
+    const int per_xcd = (int)((nblk + 7) / 8);
+    hipLaunchKernelGGL((update_kernel<T>), dim3((unsigned)(8 * per_xcd)), dim3(256), 0, st, Z, Y, Cm, m, n, ng,
+                       tiles_n, (int)nblk, per_xcd);
     QM_HIP(hipGetLastError());
     return QM_OK;
 }
