@@ -157,7 +157,7 @@ int qm_update_f32(qm_ctx *ctx, const float *d_Z, int m, const float *d_C, int n,
 
 /* Generalized symmetric eigenproblem A z = l B z on the GPU in FP64, in place of
  * lapack_DSYGVD(UPLO='U', A, B, eigen_values, eigen_vectors, triangle_factor, N)
- * (w_dsygvd.h:14-72; the Cholesky route of lapack_ZHEGVD_emu, w_zhegvd.h:227-272).
+ * (w_dsygvd.h:14-72), following the Cholesky route of lapack_ZHEGVD_emu (w_zhegvd.h:227-272) in real arithmetic.
  * d_A, d_B: n x n column-major, upper triangles referenced, not modified.
  * d_w: all n eigenvalues, ascending.  d_Z: n x nev, eigenvectors of the nev
  * lowest, B-orthonormal.  d_U: n x n, upper triangle receives the Cholesky

@@ -43,3 +43,29 @@ def test_errors_without_device(qm):
     st = L.qm_create(ctypes.byref(ctx), 2, 8, 8, 0.1, 0.1, 0.1, 0)
     assert st == -1  # QM_ERR_ARG: grid dimensions must be >= 4
     assert b"grid" in L.qm_last_error()
+
+
+def test_stencil_candidate_queries(qm):
+    """The tile-shape autotuner's candidate list is queryable without a GPU;
+    out-of-range queries are argument errors, not crashes."""
+    lib = qm._lib.load()
+    n = lib.qm_stencil_num_candidates()
+    assert n >= 2
+    buf = ctypes.create_string_buffer(128)
+    names = set()
+    for i in range(n):
+        assert lib.qm_stencil_candidate_name(i, buf, len(buf)) == qm._lib.QM_OK
+        assert buf.value.startswith(b"fd8_wave_kernel")
+        names.add(buf.value)
+    assert len(names) == n  # every candidate is a distinct shape
+    assert lib.qm_stencil_candidate_name(n, buf, len(buf)) == -1
+    assert lib.qm_stencil_candidate_name(-1, buf, len(buf)) == -1
+    small = ctypes.create_string_buffer(8)
+    assert lib.qm_stencil_candidate_name(0, small, len(small)) == qm._lib.QM_OK
+    assert len(small.value) == 7  # truncated, NUL-terminated
+    assert lib.qm_stencil_force(None, 0) == -1
+
+
+def test_sygvd_argument_errors(qm):
+    lib = qm._lib.load()
+    assert lib.qm_sygvd_f64(None, 4, None, None, None, None, None, 2, None) == -1
