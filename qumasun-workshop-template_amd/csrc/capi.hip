@@ -130,6 +130,25 @@ static int build_phase(qm_ctx *ctx, KPoint &kp) {
     return QM_OK;
 }
 
+// The context's second stream and its two events, created on first use
+// (default priority, non-blocking).
+int side_stream(qm_ctx *ctx) {
+    if (ctx->side) return QM_OK;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    QM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    if (hipEventCreateWithFlags(&e0, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&e1, hipEventDisableTiming) != hipSuccess) {
+        if (e0) (void)hipEventDestroy(e0);
+        (void)hipStreamDestroy(s);
+        QM_CHECK(false, QM_ERR_HIP, "hipEventCreateWithFlags failed");
+    }
+    ctx->side = s;
+    ctx->ev_fork = e0;
+    ctx->ev_join = e1;
+    return QM_OK;
+}
+
 // part: 3 = local + nonlocal (overwrite), 1 = local only (overwrite), 2 = nonlocal only (accumulate)
 template <typename T>
 int hpsi_apply(qm_ctx *ctx, T *hpsi, const T *psi, int nb, int ncomp, int ik, void *stream, int part = 3) {
@@ -142,6 +161,36 @@ int hpsi_apply(qm_ctx *ctx, T *hpsi, const T *psi, int nb, int ncomp, int ik, vo
     QM_TRY(activate(ctx));
     hipStream_t st = (hipStream_t)stream;
     const KPoint &kp = ctx->kpts[ik];
+    const bool nl = ctx->proj.natoms > 0 && ctx->proj.ptot > 0;
+    if (part == 3 && nl) {
+        // The projection reads psi only and the stencil writes hpsi only: the
+        // projection (and its coefficient sum) runs on the context's second
+        // stream beside the stencil, and the scatter waits for both.  The fork
+        // event is recorded after everything the caller queued before, which
+        // includes the previous apply's scatter, the last reader of the
+        // coefficient buffer.
+        QM_TRY(side_stream(ctx));
+        // the first apply of a shape times the stencil's tile shapes: it runs
+        // alone then, before the projection starts
+        const int kt = (std::fabs(kp.k[0]) >= 1.0e-8 || std::fabs(kp.k[1]) >= 1.0e-8 ||
+                        std::fabs(kp.k[2]) >= 1.0e-8) ? 1 : 0;
+        const bool tuned = stencil_choice(ctx, sizeof(T) == 8 ? 64 : 32, ncomp, kt, nb) >= 0;
+        auto stencil = [&]() -> int {
+            if constexpr (sizeof(T) == 8) return launch_fd8_local_f64(ctx, hpsi, psi, nb, ncomp, kp.k, st);
+            else return launch_fd8_local_f32(ctx, hpsi, psi, nb, ncomp, kp.k, st);
+        };
+        if (!tuned) QM_TRY(stencil());
+        QM_HIP(hipEventRecord(ctx->ev_fork, st));
+        QM_HIP(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+        if constexpr (sizeof(T) == 8) QM_TRY(nonlocal_project_f64(ctx, psi, nb, ncomp, kp, ctx->side));
+        else QM_TRY(nonlocal_project_f32(ctx, psi, nb, ncomp, kp, ctx->side));
+        QM_HIP(hipEventRecord(ctx->ev_join, ctx->side));
+        if (tuned) QM_TRY(stencil());
+        QM_HIP(hipStreamWaitEvent(st, ctx->ev_join, 0));
+        if constexpr (sizeof(T) == 8) QM_TRY(nonlocal_scatter_f64(ctx, hpsi, nb, ncomp, kp, st));
+        else QM_TRY(nonlocal_scatter_f32(ctx, hpsi, nb, ncomp, kp, st));
+        return QM_OK;
+    }
     if constexpr (sizeof(T) == 8) {
         if (part & 1) QM_TRY(launch_fd8_local_f64(ctx, hpsi, psi, nb, ncomp, kp.k, st));
         if (part & 2) QM_TRY(launch_nonlocal_f64(ctx, hpsi, psi, nb, ncomp, kp, st));
@@ -258,6 +307,9 @@ void qm_destroy(qm_ctx *ctx) {
     ctx->nl_part.release();
     ctx->eig_work.release();
     if (ctx->blas) rocblas_destroy_handle((rocblas_handle)ctx->blas);
+    if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+    if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+    if (ctx->side) (void)hipStreamDestroy(ctx->side);
     delete ctx;
 }
 

@@ -41,15 +41,17 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // part[(item * 16 + w) * ncol + col] = <p_w | e^{ik.r} psi>_col summed over the
-// item's voxels (col = real SoA column n * NC + c), for a tile of 16 * CT
-// columns.  Each wave takes kProjChunk / 4 voxels in sub-chunks of 64: the
-// 64 x 16CT block of (phase-rotated) psi values is gathered with lanes along
-// voxels (coalesced along x runs) into LDS, then fed to MFMA as the B operand
-// of CT column tiles that share one A operand, the voxel-major projector table
-// (so the table is read once per 16 CT columns).
+// item's voxels (col = real SoA column n * NC + c), for the GT 16-column tiles
+// of column group blockIdx.y.  Each wave takes kProjChunk / 4 voxels in
+// sub-chunks of 64.  Per sub-chunk it reads its block of the voxel-major
+// projector table once, as the MFMA A operand of all GT tiles; per tile the
+// 64 x 16 block of (phase-rotated) psi values is gathered with lanes along
+// voxels (coalesced along x runs) into the wave's own LDS stage and fed to
+// MFMA as the B operand.  The stage is private to the wave, so no workgroup
+// barrier is needed until the final reduction over the four waves.
 constexpr int PJ_VPW = kProjChunk / 4;  // voxels per wave
 constexpr int PJ_SUB = 64;              // voxels per LDS sub-chunk
-template <typename T, int NC, bool PH, int CT>
+template <typename T, int NC, bool PH, int GT>
 __global__ __launch_bounds__(256) void nl_project_kernel(double *__restrict__ part, const T *__restrict__ psi,
                                                          int64_t ng, int ncol, const int *__restrict__ chunk_atom,
                                                          const int *__restrict__ chunk_v0,
@@ -58,76 +60,97 @@ __global__ __launch_bounds__(256) void nl_project_kernel(double *__restrict__ pa
                                                          const T *__restrict__ phase) {
     using M = Mfma<T>;
     using acc_t = typename M::acc_t;
-    constexpr int NCOL = 16 * CT;
-    __shared__ T stage[4][PJ_SUB][NCOL + 1];
-    __shared__ double red[4][16 * NCOL];
+    __shared__ T stage[4][PJ_SUB][16 + 1];
+    __shared__ double red[4][16 * 16];
     const int item = blockIdx.x;
-    const int c0 = blockIdx.y * NCOL;
+    const int c0 = blockIdx.y * 16 * GT;
     const int a = chunk_atom[item];
     const int64_t vg0 = vox_off[a];
     const int V = (int)(vox_off[a + 1] - vg0);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int j = lane & 15, g = lane >> 4;
     const int vb = chunk_v0[item] + wave * PJ_VPW;
-    acc_t acc[CT];
+    acc_t acc[GT];
 #pragma unroll
-    for (int t = 0; t < CT; ++t) acc[t] = acc_t{0, 0, 0, 0};
+    for (int t = 0; t < GT; ++t) acc[t] = acc_t{0, 0, 0, 0};
     for (int sc = 0; sc < PJ_VPW / PJ_SUB; ++sc) {
-        // gather: lane = voxel, NCOL columns
-        {
-            const int v = vb + sc * PJ_SUB + lane;
-            const bool in = v < V;
-            const int64_t vid = vg0 + (in ? v : 0);
-            const int gi = in ? vox_grid[vid] : 0;
-            T cs = T(1), sn = T(0);
-            if constexpr (PH) {
-                cs = phase[2 * vid];
-                sn = phase[2 * vid + 1];
+        const int vs = vb + sc * PJ_SUB;
+        if (vs >= V) break;  // wave-uniform: the rest of the wave's voxels are past the sphere
+        // A[w = j][k = g] of the 16 k-steps of this sub-chunk
+        T av[PJ_SUB / 4];
+#pragma unroll
+        for (int s = 0; s < PJ_SUB / 4; ++s) {
+            const int v = vs + 4 * s + g;
+            av[s] = v < V ? tab16[(vg0 + v) * kMaxProj + j] : T(0);
+        }
+        // the lane's voxel for the gathers
+        const int v = vs + lane;
+        const bool in = v < V;
+        const int64_t vid = vg0 + (in ? v : 0);
+        const int gi = in ? vox_grid[vid] : 0;
+        T cs = T(1), sn = T(0);
+        if constexpr (PH) {
+            cs = phase[2 * vid];
+            sn = phase[2 * vid + 1];
+        }
+        // all GT x 16 gathers first, so that they are in flight together
+        T gv[GT][16];
+#pragma unroll
+        for (int t = 0; t < GT; ++t)
+#pragma unroll
+            for (int jj = 0; jj < 16; ++jj) {
+                const int col = c0 + 16 * t + jj;
+                gv[t][jj] = (in && col < ncol) ? psi[(int64_t)col * ng + gi] : T(0);
             }
 #pragma unroll
-            for (int jj = 0; jj < NCOL; jj += NC) {
-                const int col = c0 + jj;
-                T re = T(0), im = T(0);
-                if (in && col < ncol) {
-                    const T *p = psi + (int64_t)col * ng;  // column col is band col / NC, component 0
-                    re = p[gi];
-                    if constexpr (NC == 2) im = p[ng + gi];
-                }
-                if constexpr (NC == 2) {
-                    if constexpr (PH) {
-                        stage[wave][lane][jj] = cs * re - sn * im;
-                        stage[wave][lane][jj + 1] = cs * im + sn * re;
+        for (int t = 0; t < GT; ++t) {
+            const int cb = c0 + 16 * t;
+            if (cb < ncol) {  // uniform
+#pragma unroll
+                for (int jj = 0; jj < 16; jj += NC) {
+                    const T re = gv[t][jj];
+                    T im = T(0);
+                    if constexpr (NC == 2) im = gv[t][jj + 1];
+                    if constexpr (NC == 2) {
+                        if constexpr (PH) {
+                            stage[wave][lane][jj] = cs * re - sn * im;
+                            stage[wave][lane][jj + 1] = cs * im + sn * re;
+                        } else {
+                            stage[wave][lane][jj] = re;
+                            stage[wave][lane][jj + 1] = im;
+                        }
                     } else {
                         stage[wave][lane][jj] = re;
-                        stage[wave][lane][jj + 1] = im;
                     }
-                } else {
-                    stage[wave][lane][jj] = re;
                 }
+                // the stage is written and read by this wave only: its LDS
+                // operations complete in order, and the fence keeps the
+                // compiler from moving them across
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+#pragma unroll
+                for (int s = 0; s < PJ_SUB / 4; ++s)
+                    acc[t] = M::mma(av[s], stage[wave][4 * s + g][j], acc[t]);  // B[k = g][col = j]
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            }
+        }
+    }
+    // sum of the four waves, tile by tile, in a fixed order
+#pragma unroll
+    for (int t = 0; t < GT; ++t) {
+        const int cb = c0 + 16 * t;
+        if (cb >= ncol) break;  // uniform over the workgroup
+#pragma unroll
+        for (int r = 0; r < 4; ++r) red[wave][M::row(g, r) * 16 + j] = (double)acc[t][r];
+        __syncthreads();
+        {
+            const int e = threadIdx.x;  // 256 = 16 projectors x 16 columns
+            const int w = e >> 4, cl = e & 15;
+            if (cb + cl < ncol) {
+                const double sum = (red[0][e] + red[1][e]) + (red[2][e] + red[3][e]);
+                part[((int64_t)item * kMaxProj + w) * ncol + cb + cl] = sum;
             }
         }
         __syncthreads();
-#pragma unroll 4
-        for (int s = 0; s < PJ_SUB / 4; ++s) {
-            const int vl = 4 * s + g;
-            const int v = vb + sc * PJ_SUB + vl;
-            const T av = v < V ? tab16[(vg0 + v) * kMaxProj + j] : T(0);  // A[w = j][k = g]
-#pragma unroll
-            for (int t = 0; t < CT; ++t) acc[t] = M::mma(av, stage[wave][vl][16 * t + j], acc[t]);  // B[k = g][col = j]
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int t = 0; t < CT; ++t)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) red[wave][(M::row(g, r) * CT + t) * 16 + j] = (double)acc[t][r];
-    __syncthreads();
-    for (int e = threadIdx.x; e < 16 * NCOL; e += 256) {
-        const int w = e / NCOL, cl = e % NCOL;  // red layout: (w * CT + t) * 16 + j = w * NCOL + cl
-        if (c0 + cl < ncol) {
-            const double sum = (red[0][e] + red[1][e]) + (red[2][e] + red[3][e]);
-            part[((int64_t)item * kMaxProj + w) * ncol + c0 + cl] = sum;
-        }
     }
 }
 
@@ -148,33 +171,39 @@ __global__ void nl_coef_kernel(T *__restrict__ coef, const double *__restrict__ 
 }
 
 // Hpsi[col][g] += sum over (atom, voxel) entries covering g of (e^{-ik.r} sum_w p_w c_w)_col
-// for NBS real columns per thread (NBS a multiple of NC).  The coefficients of
-// the (up to kScatterSlots) atoms a workgroup touches are staged in LDS.
-template <typename T, int NC, bool PH, int NBS>
+// for the NG = NBS x GT real columns (NBS a multiple of NC) of column group
+// blockIdx.y.  The coefficients of the (up to kScatterSlots) atoms a workgroup
+// touches are staged in LDS; each thread reads its point's entry list and
+// table rows once for the whole group.
+template <typename T, int NC, bool PH, int NBS, int GT>
 __global__ __launch_bounds__(kScatterBlock) void nl_scatter_kernel(
     T *__restrict__ hpsi, const T *__restrict__ coef, int64_t ng, int ncol, int ncov,
     const int *__restrict__ cov_grid, const int *__restrict__ cov_off, const int *__restrict__ cov_vox,
     const int *__restrict__ cov_slot, const int *__restrict__ block_atom_off, const int *__restrict__ block_atoms,
     const int *__restrict__ vox_atom, const int *__restrict__ nproj, const int *__restrict__ proj_off,
     const T *__restrict__ tab16, const T *__restrict__ phase) {
-    __shared__ T cf_s[kScatterSlots][kMaxProj][NBS];
-    const int c0 = blockIdx.y * NBS;
+    constexpr int NG = NBS * GT;  // columns of the group
+    __shared__ T cf_s[kScatterSlots][kMaxProj][NG];
+    const int cg = blockIdx.y * NG;
     {
         const int b0 = block_atom_off[blockIdx.x], nat = block_atom_off[blockIdx.x + 1] - b0;
-        for (int e = threadIdx.x; e < nat * kMaxProj * NBS; e += kScatterBlock) {
-            const int sl = e / (kMaxProj * NBS), w = (e / NBS) % kMaxProj, q = e % NBS;
+        for (int e = threadIdx.x; e < nat * kMaxProj * NG; e += kScatterBlock) {
+            const int sl = e / (kMaxProj * NG), w = (e / NG) % kMaxProj, q = e % NG;
             const int a = block_atoms[b0 + sl];
-            cf_s[sl][w][q] = (w < nproj[a] && c0 + q < ncol) ? coef[(int64_t)(proj_off[a] + w) * ncol + c0 + q] : T(0);
+            cf_s[sl][w][q] = (w < nproj[a] && cg + q < ncol) ? coef[(int64_t)(proj_off[a] + w) * ncol + cg + q] : T(0);
         }
     }
     __syncthreads();
     const int i = blockIdx.x * kScatterBlock + threadIdx.x;
     if (i >= ncov) return;
     const int g = cov_grid[i];
-    T acc[NBS];
+    const int e0 = cov_off[i], e1 = cov_off[i + 1];
+    // the point's values of all NG columns, requested together, then updated
+    // in registers and written back once
+    T hv[NG];
 #pragma unroll
-    for (int q = 0; q < NBS; ++q) acc[q] = T(0);
-    for (int e = cov_off[i]; e < cov_off[i + 1]; ++e) {
+    for (int q = 0; q < NG; ++q) hv[q] = cg + q < ncol ? hpsi[(int64_t)(cg + q) * ng + g] : T(0);
+    for (int e = e0; e < e1; ++e) {
         const int vid = cov_vox[e];
         const int sl = cov_slot[e];
         T pw[kMaxProj];
@@ -187,7 +216,7 @@ __global__ __launch_bounds__(kScatterBlock) void nl_scatter_kernel(
         }
         if (sl >= 0) {
 #pragma unroll
-            for (int q = 0; q < NBS; q += NC) {
+            for (int q = 0; q < NG; q += NC) {
                 T tr = T(0), ti = T(0);
 #pragma unroll
                 for (int w = 0; w < kMaxProj; ++w) {
@@ -195,37 +224,39 @@ __global__ __launch_bounds__(kScatterBlock) void nl_scatter_kernel(
                     if constexpr (NC == 2) ti += pw[w] * cf_s[sl][w][q + 1];
                 }
                 if constexpr (PH) {
-                    acc[q] += cs * tr + sn * ti;
-                    acc[q + 1] += cs * ti - sn * tr;
+                    hv[q] += cs * tr + sn * ti;
+                    hv[q + 1] += cs * ti - sn * tr;
                 } else {
-                    acc[q] += tr;
-                    if constexpr (NC == 2) acc[q + 1] += ti;
+                    hv[q] += tr;
+                    if constexpr (NC == 2) hv[q + 1] += ti;
                 }
             }
         } else {  // more atoms in this workgroup than LDS slots: coefficients from global memory
             const int a = vox_atom[vid];
             const int M = nproj[a];
-            const T *cf = coef + (int64_t)proj_off[a] * ncol + c0;
-            for (int q = 0; q < NBS; q += NC) {
-                if (c0 + q >= ncol) break;
+            const T *cf = coef + (int64_t)proj_off[a] * ncol + cg;
+#pragma unroll
+            for (int q = 0; q < NG; q += NC) {
                 T tr = T(0), ti = T(0);
-                for (int w = 0; w < M; ++w) {
-                    tr += pw[w] * cf[(int64_t)w * ncol + q];
-                    if constexpr (NC == 2) ti += pw[w] * cf[(int64_t)w * ncol + q + 1];
+                if (cg + q < ncol) {
+                    for (int w = 0; w < M; ++w) {
+                        tr += pw[w] * cf[(int64_t)w * ncol + q];
+                        if constexpr (NC == 2) ti += pw[w] * cf[(int64_t)w * ncol + q + 1];
+                    }
                 }
                 if constexpr (PH) {
-                    acc[q] += cs * tr + sn * ti;
-                    acc[q + 1] += cs * ti - sn * tr;
+                    hv[q] += cs * tr + sn * ti;
+                    hv[q + 1] += cs * ti - sn * tr;
                 } else {
-                    acc[q] += tr;
-                    if constexpr (NC == 2) acc[q + 1] += ti;
+                    hv[q] += tr;
+                    if constexpr (NC == 2) hv[q + 1] += ti;
                 }
             }
         }
     }
 #pragma unroll
-    for (int q = 0; q < NBS; ++q)
-        if (c0 + q < ncol) hpsi[(int64_t)(c0 + q) * ng + g] += acc[q];
+    for (int q = 0; q < NG; ++q)
+        if (cg + q < ncol) hpsi[(int64_t)(cg + q) * ng + g] = hv[q];
 }
 
 // Per (atom, band group): E_nl and F = 2 sum occ E Re(conj(<p|psi>) <dp/dr|psi>).
@@ -353,19 +384,18 @@ template <>
 const float *tab16_of<float>(const Projectors &p) { return p.d_tab16_32; }
 
 template <typename T, int NC, bool PH>
-int nonlocal_apply(qm_ctx *ctx, T *hpsi, const T *psi, int nb, const T *phase, hipStream_t st) {
+int nonlocal_project(qm_ctx *ctx, const T *psi, int nb, const T *phase, hipStream_t st) {
     const Projectors &p = ctx->proj;
     const int ncol = nb * NC;
     QM_TRY(ctx->coef.reserve(sizeof(T) * (size_t)p.ptot * ncol));
     QM_TRY(ctx->nl_part.reserve(sizeof(double) * (size_t)p.nchunk * kMaxProj * ncol));
     T *coef = ctx->coef.as<T>();
-    // one 16-column tile per projection workgroup: wider tiles (32 / 64 columns
-    // sharing one table read) measured 2-3x slower on 64^3 x 128, from fewer and
-    // heavier workgroups (LDS limits them to one per CU)
-    constexpr int CT = 1;
-    const int tiles = (ncol + 16 * CT - 1) / (16 * CT);
-    QM_CHECK(tiles <= 65535, QM_ERR_ARG, "nonlocal apply: too many bands per call");
-    hipLaunchKernelGGL((nl_project_kernel<T, NC, PH, CT>), dim3(p.nchunk, tiles), dim3(256), 0, st,
+    // 4 tiles of 16 columns per projection workgroup share each table read;
+    // the columns beyond split over blockIdx.y
+    constexpr int GT = 4;
+    const int groups = (ncol + 16 * GT - 1) / (16 * GT);
+    QM_CHECK(groups <= 65535, QM_ERR_ARG, "nonlocal apply: too many bands per call");
+    hipLaunchKernelGGL((nl_project_kernel<T, NC, PH, GT>), dim3(p.nchunk, groups), dim3(256), 0, st,
                        ctx->nl_part.as<double>(), psi, ctx->ng, ncol, p.d_chunk_atom, p.d_chunk_v0, p.d_vox_off,
                        p.d_vox_grid, tab16_of<T>(p), phase);
     QM_HIP(hipGetLastError());
@@ -374,28 +404,50 @@ int nonlocal_apply(qm_ctx *ctx, T *hpsi, const T *psi, int nb, const T *phase, h
                        ctx->nl_part.as<double>(), ncol, p.ptot, p.d_proj_atom, p.d_proj_off, p.d_atom_chunk_off,
                        p.d_energy, ctx->hx * ctx->hy * ctx->hz);
     QM_HIP(hipGetLastError());
-    // columns per scatter thread (the table row and entry list are read once per
-    // NBS columns); 64 measured 1.9x slower than 16 on 64^3 x 128
+    return QM_OK;
+}
+
+template <typename T, int NC, bool PH>
+int nonlocal_scatter(qm_ctx *ctx, T *hpsi, int nb, const T *phase, hipStream_t st) {
+    const Projectors &p = ctx->proj;
+    const int ncol = nb * NC;
+    QM_CHECK(ctx->coef.bytes >= sizeof(T) * (size_t)p.ptot * ncol, QM_ERR_STATE,
+             "nonlocal scatter: no projection coefficients for this band count");
+    const T *coef = ctx->coef.as<T>();
+    // columns per scatter tile (16; 64 per thread measured 1.9x slower on
+    // 64^3 x 128) and tiles per workgroup (the staged coefficients, entry lists
+    // and table rows serve all of them)
     constexpr int NBS = 16;
-    const int stiles = (ncol + NBS - 1) / NBS;
-    hipLaunchKernelGGL((nl_scatter_kernel<T, NC, PH, NBS>), dim3(p.nsblock, stiles), dim3(kScatterBlock), 0, st, hpsi,
-                       coef, ctx->ng, ncol, p.ncov, p.d_cov_grid, p.d_cov_off, p.d_cov_vox, p.d_cov_slot,
+    constexpr int SGT = sizeof(T) == 8 ? 1 : 2;
+    const int sgroups = (ncol + NBS * SGT - 1) / (NBS * SGT);
+    hipLaunchKernelGGL((nl_scatter_kernel<T, NC, PH, NBS, SGT>), dim3(p.nsblock, sgroups), dim3(kScatterBlock), 0, st,
+                       hpsi, coef, ctx->ng, ncol, p.ncov, p.d_cov_grid, p.d_cov_off, p.d_cov_vox, p.d_cov_slot,
                        p.d_block_atom_off, p.d_block_atoms, p.d_vox_atom, p.d_nproj, p.d_proj_off, tab16_of<T>(p),
                        phase);
     QM_HIP(hipGetLastError());
     return QM_OK;
 }
 
+// which: 1 = projection coefficients, 2 = scatter, 3 = both
 template <typename T>
-int launch_nonlocal(qm_ctx *ctx, T *hpsi, const T *psi, int nb, int ncomp, const KPoint &kp, hipStream_t st) {
+int launch_nonlocal(qm_ctx *ctx, T *hpsi, const T *psi, int nb, int ncomp, const KPoint &kp, hipStream_t st,
+                    int which = 3) {
     const Projectors &p = ctx->proj;
     if (p.natoms == 0 || p.ptot == 0) return QM_OK;
     const bool ph = !kp.gamma;
     QM_CHECK(ncomp == 2 || !ph, QM_ERR_ARG, "real (ncomp=1) bands need a Gamma k-point");
     const T *phase = phase_of<T>(kp);
-    if (ncomp == 1) return nonlocal_apply<T, 1, false>(ctx, hpsi, psi, nb, phase, st);
-    if (!ph) return nonlocal_apply<T, 2, false>(ctx, hpsi, psi, nb, phase, st);
-    return nonlocal_apply<T, 2, true>(ctx, hpsi, psi, nb, phase, st);
+    if (which & 1) {
+        if (ncomp == 1) QM_TRY((nonlocal_project<T, 1, false>(ctx, psi, nb, phase, st)));
+        else if (!ph) QM_TRY((nonlocal_project<T, 2, false>(ctx, psi, nb, phase, st)));
+        else QM_TRY((nonlocal_project<T, 2, true>(ctx, psi, nb, phase, st)));
+    }
+    if (which & 2) {
+        if (ncomp == 1) QM_TRY((nonlocal_scatter<T, 1, false>(ctx, hpsi, nb, phase, st)));
+        else if (!ph) QM_TRY((nonlocal_scatter<T, 2, false>(ctx, hpsi, nb, phase, st)));
+        else QM_TRY((nonlocal_scatter<T, 2, true>(ctx, hpsi, nb, phase, st)));
+    }
+    return QM_OK;
 }
 
 template <typename T, int MM, int NC, bool PH>
@@ -466,6 +518,18 @@ int launch_nonlocal_f64(qm_ctx *ctx, double *hpsi, const double *psi, int nb, in
 int launch_nonlocal_f32(qm_ctx *ctx, float *hpsi, const float *psi, int nb, int ncomp, const KPoint &kp,
                         hipStream_t st) {
     return launch_nonlocal<float>(ctx, hpsi, psi, nb, ncomp, kp, st);
+}
+int nonlocal_project_f64(qm_ctx *ctx, const double *psi, int nb, int ncomp, const KPoint &kp, hipStream_t st) {
+    return launch_nonlocal<double>(ctx, nullptr, psi, nb, ncomp, kp, st, 1);
+}
+int nonlocal_project_f32(qm_ctx *ctx, const float *psi, int nb, int ncomp, const KPoint &kp, hipStream_t st) {
+    return launch_nonlocal<float>(ctx, nullptr, psi, nb, ncomp, kp, st, 1);
+}
+int nonlocal_scatter_f64(qm_ctx *ctx, double *hpsi, int nb, int ncomp, const KPoint &kp, hipStream_t st) {
+    return launch_nonlocal<double>(ctx, hpsi, nullptr, nb, ncomp, kp, st, 2);
+}
+int nonlocal_scatter_f32(qm_ctx *ctx, float *hpsi, int nb, int ncomp, const KPoint &kp, hipStream_t st) {
+    return launch_nonlocal<float>(ctx, hpsi, nullptr, nb, ncomp, kp, st, 2);
 }
 int nonlocal_forces_f64(qm_ctx *ctx, const double *psi, const double *occ, int nb, int ncomp, const KPoint &kp,
                         double *forces, double *e_nl, hipStream_t st) {

@@ -96,7 +96,7 @@ struct Projectors {
     int *d_block_atom_off = nullptr, *d_block_atoms = nullptr, *d_cov_slot = nullptr;
 };
 
-constexpr int kProjChunk = 1024;  // voxels per projection work item (4 waves x 256)
+constexpr int kProjChunk = 512;   // voxels per projection work item (4 waves x 128)
 constexpr int kScatterBlock = 256;  // covered points per scatter workgroup
 constexpr int kScatterSlots = 8;    // atoms whose coefficients a scatter workgroup stages in LDS
 
@@ -127,6 +127,10 @@ struct qm_ctx {
     int stencil_force = -1;                 // >= 0: use this candidate, no tuning
     void *blas = nullptr;                   // rocblas_handle of qm_sygvd_f64, created on first use
     qm::DevBuf eig_work;                    // qm_sygvd_f64 workspace
+    // second stream of a full H.psi apply: the nonlocal projection runs on it
+    // beside the stencil, joined by events (created on first use)
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 namespace qm {
@@ -154,6 +158,12 @@ int launch_nonlocal_f64(qm_ctx *ctx, double *hpsi, const double *psi, int nb, in
                         const KPoint &kp, hipStream_t st);
 int launch_nonlocal_f32(qm_ctx *ctx, float *hpsi, const float *psi, int nb, int ncomp,
                         const KPoint &kp, hipStream_t st);
+// the two halves of launch_nonlocal_*: projection coefficients (reads psi,
+// writes the context's coefficient buffer), then the scatter (hpsi +=)
+int nonlocal_project_f64(qm_ctx *ctx, const double *psi, int nb, int ncomp, const KPoint &kp, hipStream_t st);
+int nonlocal_project_f32(qm_ctx *ctx, const float *psi, int nb, int ncomp, const KPoint &kp, hipStream_t st);
+int nonlocal_scatter_f64(qm_ctx *ctx, double *hpsi, int nb, int ncomp, const KPoint &kp, hipStream_t st);
+int nonlocal_scatter_f32(qm_ctx *ctx, float *hpsi, int nb, int ncomp, const KPoint &kp, hipStream_t st);
 int nonlocal_forces_f64(qm_ctx *ctx, const double *psi, const double *occ, int nb, int ncomp,
                         const KPoint &kp, double *forces, double *e_nl, hipStream_t st);
 int nonlocal_forces_f32(qm_ctx *ctx, const float *psi, const double *occ, int nb, int ncomp,

@@ -190,3 +190,40 @@ def test_stencil_candidates_bit_identical(qm, orc, torch_cuda, label, build):
             want = orc.hpsi(s, src, k=s.kpoints[ik], nonlocal_=False)
             assert _rel(y.double().cpu().numpy(), want) <= tol, (label, prec, ik)
     H.close()
+
+
+@pytest.mark.parametrize("label,build,dtype_name", [
+    ("C0 32^3 real f32", lambda qm: qm.system.config(0, nbands=6), "float32"),
+    ("C2 48^3 complex k f64", lambda qm: qm.system.config(2, nbands=3, grid=48), "float64"),
+], ids=["C0-f32", "C2-k-f64"])
+def test_hpsi_graph_capture_replays(qm, torch_cuda, label, build, dtype_name):
+    """A full apply (stencil on the caller's stream, nonlocal projection forked
+    onto the context's second stream and joined back) captured into a graph:
+    every replay, with eager work and synchronisations between replays, equals
+    the eager result bit for bit."""
+    torch = torch_cuda
+    s = build(qm)
+    H = qm.hamiltonian.Hamiltonian(s)
+    dtype = getattr(torch, dtype_name)
+    ik = s.kpoints.shape[0] - 1
+    x = torch.from_numpy(s.initial_bands()).to(device="cuda", dtype=dtype)
+    want = torch.empty_like(x)
+    H.apply(want, x, k_index=ik)  # eager; tunes the stencil shape before capture
+    torch.cuda.synchronize()
+    y = torch.full_like(x, float("nan"))
+    g = torch.cuda.CUDAGraph()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        with torch.cuda.graph(g):
+            H.apply(y, x, k_index=ik)
+    torch.cuda.current_stream().wait_stream(side)
+    for rep in range(4):
+        y.fill_(float("nan"))
+        other = torch.empty_like(x)
+        H.apply(other, x * 0.5, k_index=ik)  # eager work on the same context between replays
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        print(f"case {label} replay {rep}", flush=True)
+        assert torch.equal(y, want), (label, rep)
