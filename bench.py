@@ -174,6 +174,9 @@ def main():
                     help="larger workloads whose FP32 H.psi is also timed (comma list of 2, 3; empty = none)")
     ap.add_argument("--lobpcg-calls", type=int, default=10,
                     help="LOBPCG calls of 20 iterations per precision (0 skips the LOBPCG leg)")
+    ap.add_argument("--stencil-candidates", type=int, default=0, metavar="N",
+                    help="profiling mode: N FP32 local applies with each stencil tile shape in turn, then exit "
+                         "(no JSON line; for rocprofv3 counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -210,6 +213,16 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         del a, b
+    if args.stencil_candidates > 0:
+        names = H.stencil_candidates()
+        for c, name in enumerate(names):
+            H.stencil_force(c)
+            for _ in range(args.stencil_candidates):
+                local()
+            torch.cuda.synchronize()
+            print(f"[bench] stencil candidate {c}: {name}", file=sys.stderr, flush=True)
+        H.stencil_force(-1)
+        return
     for _ in range(args.warmup):
         full()
     torch.cuda.synchronize()

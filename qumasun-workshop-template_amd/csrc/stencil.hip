@@ -63,6 +63,9 @@ constexpr int kSlabLdsMax = 65536;            // bytes of LDS a slab workgroup m
 // longer leads cost more registers than they save in waiting
 constexpr int kHLead = 1;
 constexpr int kVLead = 1;
+// x neighbours through DPP wave shifts where a wave holds one 64-point row or
+// an x window (else ds_bpermute)
+constexpr bool kXDpp = true;
 
 template <typename T>
 struct FD8Coef {
@@ -106,6 +109,40 @@ __device__ __forceinline__ double lane_read(double v, int addr) {
     const unsigned long long b = (unsigned long long)__double_as_longlong(v);
     const unsigned lo = (unsigned)__builtin_amdgcn_ds_bpermute(addr, (int)(unsigned)(b & 0xffffffffull));
     const unsigned hi = (unsigned)__builtin_amdgcn_ds_bpermute(addr, (int)(unsigned)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// x neighbours through DPP wave shifts, which move data between lanes inside
+// the VALU (no LDS traffic).  XM = 1: the wave holds one whole periodic row of
+// 64 points, rotate (wave_rol:1 / wave_ror:1); XM = 2: the wave holds a 64-wide
+// x window, shift (wave_shl:1 / wave_shr:1; an edge lane keeps 0, and edge lanes
+// are halo lanes, which write nothing).  x_next: the value of lane + 1.
+template <int XM>
+__device__ __forceinline__ int dpp_next(int v) {
+    return XM == 1 ? __builtin_amdgcn_update_dpp(0, v, 0x134, 0xf, 0xf, false)
+                   : __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, false);
+}
+template <int XM>
+__device__ __forceinline__ int dpp_prev(int v) {
+    return XM == 1 ? __builtin_amdgcn_update_dpp(0, v, 0x13c, 0xf, 0xf, false)
+                   : __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, false);
+}
+template <int XM>
+__device__ __forceinline__ float x_next(float v) { return __int_as_float(dpp_next<XM>(__float_as_int(v))); }
+template <int XM>
+__device__ __forceinline__ float x_prev(float v) { return __int_as_float(dpp_prev<XM>(__float_as_int(v))); }
+template <int XM>
+__device__ __forceinline__ double x_next(double v) {
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)dpp_next<XM>((int)(unsigned)(b & 0xffffffffll));
+    const unsigned hi = (unsigned)dpp_next<XM>((int)(unsigned)((unsigned long long)b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+template <int XM>
+__device__ __forceinline__ double x_prev(double v) {
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)dpp_prev<XM>((int)(unsigned)(b & 0xffffffffll));
+    const unsigned hi = (unsigned)dpp_prev<XM>((int)(unsigned)((unsigned long long)b >> 32));
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
@@ -179,7 +216,7 @@ __device__ __forceinline__ void fill_queues(T (&pq)[NC][RY][8], T (&dz)[NC][RY][
 // y-halo rows of plane w (rows y0-4 .. y0-1 and y0+RY .. y0+RY+3) and vv the
 // potential of the lane's rows.  stoff[r] is the store offset of row r, out of
 // the buffer's range for points the lane does not write: every lane stores.
-template <typename T, int NC, bool KT, int RY>
+template <typename T, int NC, bool KT, int RY, int XM>
 __device__ __forceinline__ void fd8_plane(T (&pq)[NC][RY][8], T (&dz)[NC][RY][8], const T (&hy)[NC][8],
                                           const T (&vv)[RY], const int P, const int (&la)[3], const int (&lb_)[4],
                                           const int (&stoff)[RY], const rsrc_t (&rout)[NC], const int so,
@@ -207,13 +244,23 @@ __device__ __forceinline__ void fd8_plane(T (&pq)[NC][RY][8], T (&dz)[NC][RY][8]
         for (int cc = 0; cc < NC; ++cc) {
             const T p = pq[cc][r][P & 7];
             // x differences: own D(x), then D(x+1..x+3) and D(x-1..x-4) from lanes
-            const T dx0 = lane_read(p, la[0]) - p;
             T dxp[4], dxm[4];
-            dxp[0] = dx0;
+            if constexpr (XM == 0) {
+                const T dx0 = lane_read(p, la[0]) - p;
+                dxp[0] = dx0;
 #pragma unroll
-            for (int i = 0; i < 3; ++i) dxp[i + 1] = lane_read(dx0, la[i]);
+                for (int i = 0; i < 3; ++i) dxp[i + 1] = lane_read(dx0, la[i]);
 #pragma unroll
-            for (int i = 0; i < 4; ++i) dxm[i] = lane_read(dx0, lb_[i]);
+                for (int i = 0; i < 4; ++i) dxm[i] = lane_read(dx0, lb_[i]);
+            } else {
+                const T dx0 = x_next<XM>(p) - p;
+                dxp[0] = dx0;
+#pragma unroll
+                for (int i = 0; i < 3; ++i) dxp[i + 1] = x_next<XM>(dxp[i]);
+                dxm[0] = x_prev<XM>(dx0);
+#pragma unroll
+                for (int i = 0; i < 3; ++i) dxm[i + 1] = x_prev<XM>(dxm[i]);
+            }
             T l = KT ? c.c0 * p : T(0);
             T g = T(0);
 #pragma unroll
@@ -263,7 +310,7 @@ __device__ __forceinline__ void refill(T (&pq)[NC][RY][8], const rsrc_t (&rin)[N
     }
 }
 
-template <typename T, int NC, bool KT, int RY, bool SEG>
+template <typename T, int NC, bool KT, int RY, bool SEG, int XM>
 __global__ __launch_bounds__(256) void fd8_wave_kernel(T *__restrict__ out, const T *__restrict__ in,
                                                        const T *__restrict__ vloc, int nx, int ny, int nz,
                                                        int zchunk, int nzc, int nyt, int nxt, int nwaves, int nblk,
@@ -375,7 +422,7 @@ __global__ __launch_bounds__(256) void fd8_wave_kernel(T *__restrict__ out, cons
                 for (int j = 0; j < 8; ++j) hy[cc][j] = hq[cc][j][P & 3];
 #pragma unroll
             for (int r = 0; r < RY; ++r) vv[r] = vq[r][P & 3];
-            fd8_plane<T, NC, KT, RY>(pq, dz, hy, vv, P, la, lb_, stoff, rout, w * pbytes, c);
+            fd8_plane<T, NC, KT, RY, XM>(pq, dz, hy, vv, P, la, lb_, stoff, rout, w * pbytes, c);
             refill<T, NC, RY>(pq, rin, rnull, rowoff, P, w, ze, nz, pbytes);
         }
     }
@@ -388,7 +435,7 @@ __global__ __launch_bounds__(256) void fd8_wave_kernel(T *__restrict__ out, cons
 // row pitch makes every tile offset of a lane a constant from one base address.
 // All waves of a workgroup share band, chunk and slab, so they run the same
 // number of steps and meet at every barrier.
-template <typename T, int NC, bool KT, int RY, bool SEG, int MAXT>
+template <typename T, int NC, bool KT, int RY, bool SEG, int XM, int MAXT>
 __global__ __launch_bounds__(MAXT) void fd8_slab_kernel(T *__restrict__ out, const T *__restrict__ in,
                                                         const T *__restrict__ vloc, int nx, int ny, int nz,
                                                         int zchunk, int nzc, int nst, int nxt, int nwg, int per_xcd,
@@ -518,7 +565,7 @@ __global__ __launch_bounds__(MAXT) void fd8_slab_kernel(T *__restrict__ out, con
             T vv[RY];
 #pragma unroll
             for (int r = 0; r < RY; ++r) vv[r] = vq[r][P & 3];
-            fd8_plane<T, NC, KT, RY>(pq, dz, hy, vv, P, la, lb_, stoff, rout, w * pbytes, c);
+            fd8_plane<T, NC, KT, RY, XM>(pq, dz, hy, vv, P, la, lb_, stoff, rout, w * pbytes, c);
             refill<T, NC, RY>(pq, rin, rnull, rowoff, P, w, ze, nz, pbytes);
         }
     }
@@ -579,12 +626,15 @@ int launch_wave(const qm_ctx *ctx, T *out, const T *in, const T *vloc, int nb, i
     QM_CHECK(nwaves < (int64_t)INT32_MAX / 2, QM_ERR_ARG, "grid too large for one stencil launch");
     const int nblk = (int)((nwaves + 3) / 4);
     const int per_xcd = (nblk + 7) / 8;
-    if (seg)
-        hipLaunchKernelGGL((fd8_wave_kernel<T, NC, KT, RY, true>), dim3(8 * per_xcd), dim3(256), 0, st, out, in,
+    if (seg && nx == WAVE && kXDpp)
+        hipLaunchKernelGGL((fd8_wave_kernel<T, NC, KT, RY, true, 1>), dim3(8 * per_xcd), dim3(256), 0, st, out, in,
+                           vloc, nx, ny, nz, zchunk, nzc, nyt, nxt, (int)nwaves, nblk, per_xcd, c);
+    else if (seg)
+        hipLaunchKernelGGL((fd8_wave_kernel<T, NC, KT, RY, true, 0>), dim3(8 * per_xcd), dim3(256), 0, st, out, in,
                            vloc, nx, ny, nz, zchunk, nzc, nyt, nxt, (int)nwaves, nblk, per_xcd, c);
     else
-        hipLaunchKernelGGL((fd8_wave_kernel<T, NC, KT, RY, false>), dim3(8 * per_xcd), dim3(256), 0, st, out, in,
-                           vloc, nx, ny, nz, zchunk, nzc, nyt, nxt, (int)nwaves, nblk, per_xcd, c);
+        hipLaunchKernelGGL((fd8_wave_kernel<T, NC, KT, RY, false, kXDpp ? 2 : 0>), dim3(8 * per_xcd), dim3(256), 0,
+                           st, out, in, vloc, nx, ny, nz, zchunk, nzc, nyt, nxt, (int)nwaves, nblk, per_xcd, c);
     QM_HIP(hipGetLastError());
     return QM_OK;
 }
@@ -614,12 +664,16 @@ int launch_slab(const qm_ctx *ctx, T *out, const T *in, const T *vloc, int nb, i
     const int64_t nwg = (int64_t)nb * nzc * nst * nxt;
     QM_CHECK(nwg < (int64_t)INT32_MAX / 16, QM_ERR_ARG, "grid too large for one stencil launch");
     const int per_xcd = (int)((nwg + 7) / 8);
-    if (seg)
-        hipLaunchKernelGGL((fd8_slab_kernel<T, NC, KT, RY, true, MAXT>), dim3(8 * per_xcd), dim3(nw * WAVE),
+    if (seg && nx == WAVE && kXDpp)
+        hipLaunchKernelGGL((fd8_slab_kernel<T, NC, KT, RY, true, 1, MAXT>), dim3(8 * per_xcd), dim3(nw * WAVE),
+                           (size_t)lds, st, out, in, vloc, nx, ny, nz, zc, nzc, nst, nxt, (int)nwg, per_xcd, ty, c);
+    else if (seg)
+        hipLaunchKernelGGL((fd8_slab_kernel<T, NC, KT, RY, true, 0, MAXT>), dim3(8 * per_xcd), dim3(nw * WAVE),
                            (size_t)lds, st, out, in, vloc, nx, ny, nz, zc, nzc, nst, nxt, (int)nwg, per_xcd, ty, c);
     else
-        hipLaunchKernelGGL((fd8_slab_kernel<T, NC, KT, RY, false, MAXT>), dim3(8 * per_xcd), dim3(nw * WAVE),
-                           (size_t)lds, st, out, in, vloc, nx, ny, nz, zc, nzc, nst, nxt, (int)nwg, per_xcd, ty, c);
+        hipLaunchKernelGGL((fd8_slab_kernel<T, NC, KT, RY, false, kXDpp ? 2 : 0, MAXT>), dim3(8 * per_xcd),
+                           dim3(nw * WAVE), (size_t)lds, st, out, in, vloc, nx, ny, nz, zc, nzc, nst, nxt, (int)nwg,
+                           per_xcd, ty, c);
     QM_HIP(hipGetLastError());
     return QM_OK;
 }
