@@ -60,7 +60,7 @@ def local_bytes(s, nb, itemsize):
 def measured_traffic(candidate, workload):
     """HBM bytes per launch of the tuned stencil shape, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (None if that shape was not profiled)."""
-    path = os.path.join(ROOT, "profiles", "fd8_wave_c1_fp32_pmc.txt")
+    path = os.path.join(ROOT, "profiles", "fd8_stencil_c1_fp32_pmc.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
