@@ -33,6 +33,16 @@ namespace {
 
 constexpr int NL_THREADS = 256;
 
+// Workgroup numbering: hardware block b runs on XCD b % 8, so giving each XCD a
+// contiguous range of logical blocks puts consecutive logical blocks (the
+// column groups of one item or point block) on one XCD, where they share its
+// L2 for the projector table rows and entry lists.  Returns -1 past the end.
+__device__ __forceinline__ int xcd_block(int nblk, int per_xcd) {
+    const int b = blockIdx.x;
+    const int lb = (b & 7) * per_xcd + (b >> 3);
+    return lb < nblk ? lb : -1;
+}
+
 template <typename T>
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -42,7 +52,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 // part[(item * 16 + w) * ncol + col] = <p_w | e^{ik.r} psi>_col summed over the
 // item's voxels (col = real SoA column n * NC + c), for the GT 16-column tiles
-// of column group blockIdx.y.  Each wave takes kProjChunk / 4 voxels in
+// of column group g (logical block = item * groups + g).  Each wave takes kProjChunk / 4 voxels in
 // sub-chunks of 64.  Per sub-chunk it reads its block of the voxel-major
 // projector table once, as the MFMA A operand of all GT tiles; per tile the
 // 64 x 16 block of (phase-rotated) psi values is gathered with lanes along
@@ -57,13 +67,16 @@ __global__ __launch_bounds__(256) void nl_project_kernel(double *__restrict__ pa
                                                          const int *__restrict__ chunk_v0,
                                                          const int64_t *__restrict__ vox_off,
                                                          const int *__restrict__ vox_grid, const T *__restrict__ tab16,
-                                                         const T *__restrict__ phase) {
+                                                         const T *__restrict__ phase, int groups, int nblk,
+                                                         int per_xcd) {
     using M = Mfma<T>;
     using acc_t = typename M::acc_t;
     __shared__ T stage[4][PJ_SUB][16 + 1];
     __shared__ double red[4][16 * 16];
-    const int item = blockIdx.x;
-    const int c0 = blockIdx.y * 16 * GT;
+    const int lb = xcd_block(nblk, per_xcd);
+    if (lb < 0) return;  // the whole workgroup
+    const int item = lb / groups;
+    const int c0 = (lb - item * groups) * 16 * GT;
     const int a = chunk_atom[item];
     const int64_t vg0 = vox_off[a];
     const int V = (int)(vox_off[a + 1] - vg0);
@@ -172,7 +185,7 @@ __global__ void nl_coef_kernel(T *__restrict__ coef, const double *__restrict__ 
 
 // Hpsi[col][g] += sum over (atom, voxel) entries covering g of (e^{-ik.r} sum_w p_w c_w)_col
 // for the NG = NBS x GT real columns (NBS a multiple of NC) of column group
-// blockIdx.y.  The coefficients of the (up to kScatterSlots) atoms a workgroup
+// g (logical block = point block * groups + g).  The coefficients of the (up to kScatterSlots) atoms a workgroup
 // touches are staged in LDS; each thread reads its point's entry list and
 // table rows once for the whole group.
 template <typename T, int NC, bool PH, int NBS, int GT>
@@ -181,12 +194,15 @@ __global__ __launch_bounds__(kScatterBlock) void nl_scatter_kernel(
     const int *__restrict__ cov_grid, const int *__restrict__ cov_off, const int *__restrict__ cov_vox,
     const int *__restrict__ cov_slot, const int *__restrict__ block_atom_off, const int *__restrict__ block_atoms,
     const int *__restrict__ vox_atom, const int *__restrict__ nproj, const int *__restrict__ proj_off,
-    const T *__restrict__ tab16, const T *__restrict__ phase) {
+    const T *__restrict__ tab16, const T *__restrict__ phase, int groups, int nblk, int per_xcd) {
     constexpr int NG = NBS * GT;  // columns of the group
     __shared__ T cf_s[kScatterSlots][kMaxProj][NG];
-    const int cg = blockIdx.y * NG;
+    const int lb = xcd_block(nblk, per_xcd);
+    if (lb < 0) return;  // the whole workgroup
+    const int pb = lb / groups;  // point block
+    const int cg = (lb - pb * groups) * NG;
     {
-        const int b0 = block_atom_off[blockIdx.x], nat = block_atom_off[blockIdx.x + 1] - b0;
+        const int b0 = block_atom_off[pb], nat = block_atom_off[pb + 1] - b0;
         for (int e = threadIdx.x; e < nat * kMaxProj * NG; e += kScatterBlock) {
             const int sl = e / (kMaxProj * NG), w = (e / NG) % kMaxProj, q = e % NG;
             const int a = block_atoms[b0 + sl];
@@ -194,7 +210,7 @@ __global__ __launch_bounds__(kScatterBlock) void nl_scatter_kernel(
         }
     }
     __syncthreads();
-    const int i = blockIdx.x * kScatterBlock + threadIdx.x;
+    const int i = pb * kScatterBlock + threadIdx.x;
     if (i >= ncov) return;
     const int g = cov_grid[i];
     const int e0 = cov_off[i], e1 = cov_off[i + 1];
@@ -391,13 +407,15 @@ int nonlocal_project(qm_ctx *ctx, const T *psi, int nb, const T *phase, hipStrea
     QM_TRY(ctx->nl_part.reserve(sizeof(double) * (size_t)p.nchunk * kMaxProj * ncol));
     T *coef = ctx->coef.as<T>();
     // 4 tiles of 16 columns per projection workgroup share each table read;
-    // the columns beyond split over blockIdx.y
+    // the columns beyond split over more workgroups
     constexpr int GT = 4;
     const int groups = (ncol + 16 * GT - 1) / (16 * GT);
-    QM_CHECK(groups <= 65535, QM_ERR_ARG, "nonlocal apply: too many bands per call");
-    hipLaunchKernelGGL((nl_project_kernel<T, NC, PH, GT>), dim3(p.nchunk, groups), dim3(256), 0, st,
+    const int64_t nblk = (int64_t)p.nchunk * groups;
+    QM_CHECK(nblk < (int64_t)INT32_MAX / 16, QM_ERR_ARG, "nonlocal apply: too many bands per call");
+    const int per_xcd = (int)((nblk + 7) / 8);
+    hipLaunchKernelGGL((nl_project_kernel<T, NC, PH, GT>), dim3(8 * per_xcd), dim3(256), 0, st,
                        ctx->nl_part.as<double>(), psi, ctx->ng, ncol, p.d_chunk_atom, p.d_chunk_v0, p.d_vox_off,
-                       p.d_vox_grid, tab16_of<T>(p), phase);
+                       p.d_vox_grid, tab16_of<T>(p), phase, groups, (int)nblk, per_xcd);
     QM_HIP(hipGetLastError());
     const int64_t ncoef = (int64_t)p.ptot * ncol;
     hipLaunchKernelGGL((nl_coef_kernel<T>), dim3((unsigned)((ncoef + 255) / 256)), dim3(256), 0, st, coef,
@@ -420,10 +438,13 @@ int nonlocal_scatter(qm_ctx *ctx, T *hpsi, int nb, const T *phase, hipStream_t s
     constexpr int NBS = 16;
     constexpr int SGT = sizeof(T) == 8 ? 1 : 2;
     const int sgroups = (ncol + NBS * SGT - 1) / (NBS * SGT);
-    hipLaunchKernelGGL((nl_scatter_kernel<T, NC, PH, NBS, SGT>), dim3(p.nsblock, sgroups), dim3(kScatterBlock), 0, st,
+    const int64_t nblk = (int64_t)p.nsblock * sgroups;
+    QM_CHECK(nblk < (int64_t)INT32_MAX / 16, QM_ERR_ARG, "nonlocal apply: too many bands per call");
+    const int per_xcd = (int)((nblk + 7) / 8);
+    hipLaunchKernelGGL((nl_scatter_kernel<T, NC, PH, NBS, SGT>), dim3(8 * per_xcd), dim3(kScatterBlock), 0, st,
                        hpsi, coef, ctx->ng, ncol, p.ncov, p.d_cov_grid, p.d_cov_off, p.d_cov_vox, p.d_cov_slot,
                        p.d_block_atom_off, p.d_block_atoms, p.d_vox_atom, p.d_nproj, p.d_proj_off, tab16_of<T>(p),
-                       phase);
+                       phase, sgroups, (int)nblk, per_xcd);
     QM_HIP(hipGetLastError());
     return QM_OK;
 }
