@@ -37,6 +37,10 @@ CASES = [
     ("small 12^3 wrapped spheres k", lambda qm: qm.system.small_system(n=12, L=4.0, natoms=3, rc=2.6, nbands=3,
                                                                        ncomp=2, k_red=(0.5, 0.25, -0.125))),
     ("ragged 36x20x28", lambda qm: _ragged(qm)),
+    # more columns than one nonlocal workgroup takes (projection 64, scatter 32
+    # FP32 / 16 FP64), with a partial last group: 70 real, 74 complex columns
+    ("C0 32^3 70 real bands", lambda qm: qm.system.config(0, nbands=70)),
+    ("C2 40^3 37 complex bands 4k", lambda qm: qm.system.config(2, nbands=37, grid=40)),
 ]
 
 
