@@ -104,8 +104,9 @@ def hpsi_config_leg(qm, torch, cfg, steps, warmup):
     t_full = t_local = 0.0
     per_k = []
     for ik in range(s.kpoints.shape[0]):
-        for _ in range(warmup):  # the first apply of a shape tunes the stencil
+        for _ in range(max(1, warmup)):  # the first apply of each kind tunes its stencil shape
             H.apply(y, x, k_index=ik)
+            H.apply(y, x, k_index=ik, part="local")
         ms = _time_applies(torch, lambda: H.apply(y, x, k_index=ik), steps)
         ms_l = _time_applies(torch, lambda: H.apply(y, x, k_index=ik, part="local"), steps)
         t_full += ms
@@ -223,8 +224,9 @@ def main():
             print(f"[bench] stencil candidate {c}: {name}", file=sys.stderr, flush=True)
         H.stencil_force(-1)
         return
-    for _ in range(args.warmup):
-        full()
+    for fn in (full, local, nonlocal_):  # the first apply of each kind tunes its stencil shape
+        for _ in range(max(1, args.warmup)):
+            fn()
     torch.cuda.synchronize()
     ms = _time_applies(torch, full, args.steps)
     ms_local = _time_applies(torch, local, args.steps)
@@ -265,6 +267,8 @@ def main():
                      "traffic": measured_traffic(kernel_name, s.name.split()[0]) if args.config == 1 else None,
                      "bytes_per_launch": bytes_local, "ms_per_launch": ms_local},
         "parts_ms": {"local_fd8_v": ms_local, "nonlocal": ms_nl, "full": ms},
+        "stencil_shape_in_full_apply": (H.stencil_candidates()[H.stencil_choice(32, nb, full=True)]
+                                        if H.stencil_choice(32, nb, full=True) >= 0 else "untuned"),
         "fp64_hpsi": {"ms_per_apply": ms64, "value": s.ng * nb / (ms64 * 1e-3)},
     }
     del x, y

@@ -109,7 +109,11 @@ int qm_nonlocal_add_f32(qm_ctx *ctx, float *d_out, const float *d_psi, int nb, i
  * qm_stencil_candidate_name  its description (NUL-terminated, truncated to buflen);
  * qm_stencil_force           use `candidate` for every later apply, -1 = autotune;
  * qm_stencil_choice          candidate in use for that shape, -1 if not tuned yet
- *                            (kt = 1 for a k-point with |k_d| >= 1e-8 in some d). */
+ *                            (kt bit 0: a k-point with |k_d| >= 1e-8 in some d;
+ *                            kt bit 1: the shape of full qm_hpsi_apply_* calls).
+ * A full qm_hpsi_apply_* runs the stencil beside the nonlocal projection; its
+ * shape is tuned separately, on full applies, the first time that shape of
+ * call is seen (and synchronises the stream then). */
 int qm_stencil_num_candidates(void);
 int qm_stencil_candidate_name(int candidate, char *buf, int buflen);
 int qm_stencil_force(qm_ctx *ctx, int candidate);

@@ -170,26 +170,74 @@ int hpsi_apply(qm_ctx *ctx, T *hpsi, const T *psi, int nb, int ncomp, int ik, vo
         // includes the previous apply's scatter, the last reader of the
         // coefficient buffer.
         QM_TRY(side_stream(ctx));
-        // the first apply of a shape times the stencil's tile shapes: it runs
-        // alone then, before the projection starts
+        const int prec = sizeof(T) == 8 ? 64 : 32;
         const int kt = (std::fabs(kp.k[0]) >= 1.0e-8 || std::fabs(kp.k[1]) >= 1.0e-8 ||
                         std::fabs(kp.k[2]) >= 1.0e-8) ? 1 : 0;
-        const bool tuned = stencil_choice(ctx, sizeof(T) == 8 ? 64 : 32, ncomp, kt, nb) >= 0;
-        auto stencil = [&]() -> int {
-            if constexpr (sizeof(T) == 8) return launch_fd8_local_f64(ctx, hpsi, psi, nb, ncomp, kp.k, st);
-            else return launch_fd8_local_f32(ctx, hpsi, psi, nb, ncomp, kp.k, st);
+        auto full = [&]() -> int {
+            QM_HIP(hipEventRecord(ctx->ev_fork, st));
+            QM_HIP(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+            if constexpr (sizeof(T) == 8) QM_TRY(nonlocal_project_f64(ctx, psi, nb, ncomp, kp, ctx->side));
+            else QM_TRY(nonlocal_project_f32(ctx, psi, nb, ncomp, kp, ctx->side));
+            QM_HIP(hipEventRecord(ctx->ev_join, ctx->side));
+            if constexpr (sizeof(T) == 8) QM_TRY(launch_fd8_local_f64(ctx, hpsi, psi, nb, ncomp, kp.k, st));
+            else QM_TRY(launch_fd8_local_f32(ctx, hpsi, psi, nb, ncomp, kp.k, st));
+            QM_HIP(hipStreamWaitEvent(st, ctx->ev_join, 0));
+            if constexpr (sizeof(T) == 8) QM_TRY(nonlocal_scatter_f64(ctx, hpsi, nb, ncomp, kp, st));
+            else QM_TRY(nonlocal_scatter_f32(ctx, hpsi, nb, ncomp, kp, st));
+            return QM_OK;
         };
-        if (!tuned) QM_TRY(stencil());
-        QM_HIP(hipEventRecord(ctx->ev_fork, st));
-        QM_HIP(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
-        if constexpr (sizeof(T) == 8) QM_TRY(nonlocal_project_f64(ctx, psi, nb, ncomp, kp, ctx->side));
-        else QM_TRY(nonlocal_project_f32(ctx, psi, nb, ncomp, kp, ctx->side));
-        QM_HIP(hipEventRecord(ctx->ev_join, ctx->side));
-        if (tuned) QM_TRY(stencil());
-        QM_HIP(hipStreamWaitEvent(st, ctx->ev_join, 0));
-        if constexpr (sizeof(T) == 8) QM_TRY(nonlocal_scatter_f64(ctx, hpsi, nb, ncomp, kp, st));
-        else QM_TRY(nonlocal_scatter_f32(ctx, hpsi, nb, ncomp, kp, st));
-        return QM_OK;
+        if (ctx->stencil_force >= 0) return full();
+        // The stencil's tile shape for full applies is tuned on full applies:
+        // the shape that runs fastest alone is not always the one that leaves
+        // the projection the most room beside it.  Every shape gives the same
+        // values, so the timed applies also leave the right result.
+        struct Force {  // the shape for this call; restored on every return
+            qm_ctx *c;
+            ~Force() { c->stencil_force = -1; }
+        } force{ctx};
+        const int64_t key = stencil_full_key(prec, ncomp, kt, nb);
+        auto it = ctx->stencil_choice.find(key);
+        int cand = 0;
+        if (it != ctx->stencil_choice.end()) {
+            cand = it->second;
+        } else {
+            hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+            QM_HIP(hipStreamIsCapturing(st, &cs));
+            if (cs == hipStreamCaptureStatusNone) {
+                hipEvent_t ev[2] = {nullptr, nullptr};
+                QM_HIP(hipEventCreate(&ev[0]));
+                if (hipEventCreate(&ev[1]) != hipSuccess) {
+                    (void)hipEventDestroy(ev[0]);
+                    QM_CHECK(false, QM_ERR_HIP, "hipEventCreate failed");
+                }
+                float best_ms = 1e30f;
+                int status = QM_OK;
+                for (int c = 0; c < stencil_num_candidates() && status == QM_OK; ++c) {
+                    if (!stencil_feasible(ctx, c, prec, ncomp)) continue;
+                    ctx->stencil_force = c;
+                    // one warm-up apply, then the mean of 4: overlapped applies vary
+                    // by ~10 % from run to run
+                    if (status == QM_OK) status = full();
+                    float ms = 0.0f;
+                    if (status == QM_OK && hipEventRecord(ev[0], st) != hipSuccess) status = QM_ERR_HIP;
+                    for (int rep = 0; rep < 4 && status == QM_OK; ++rep) status = full();
+                    if (status == QM_OK && hipEventRecord(ev[1], st) != hipSuccess) status = QM_ERR_HIP;
+                    if (status == QM_OK && hipEventSynchronize(ev[1]) != hipSuccess) status = QM_ERR_HIP;
+                    if (status == QM_OK && hipEventElapsedTime(&ms, ev[0], ev[1]) != hipSuccess) status = QM_ERR_HIP;
+                    if (status == QM_OK && ms < best_ms) {
+                        best_ms = ms;
+                        cand = c;
+                    }
+                }
+                (void)hipEventDestroy(ev[0]);
+                (void)hipEventDestroy(ev[1]);
+                if (status == QM_ERR_HIP) set_error("H.psi shape tuning: HIP event failure");
+                if (status != QM_OK) return status;
+                ctx->stencil_choice[key] = cand;
+            }
+        }
+        ctx->stencil_force = cand;
+        return full();
     }
     if constexpr (sizeof(T) == 8) {
         if (part & 1) QM_TRY(launch_fd8_local_f64(ctx, hpsi, psi, nb, ncomp, kp.k, st));

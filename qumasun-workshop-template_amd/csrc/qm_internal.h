@@ -145,6 +145,10 @@ int launch_fd8_local_f32(qm_ctx *ctx, float *out, const float *in, int nb, int n
 int stencil_num_candidates();
 std::string stencil_candidate_name(int cand);
 int stencil_choice(const qm_ctx *ctx, int precision, int ncomp, int kt, int nb);
+// key of the tile shape a full H.psi apply (stencil beside the nonlocal
+// projection) uses, tuned on full applies; and whether a shape can run
+int64_t stencil_full_key(int precision, int ncomp, int kt, int nb);
+bool stencil_feasible(const qm_ctx *ctx, int cand, int precision, int ncomp);
 
 // eigen.hip: FP64 symmetric eigensolver pieces (rocblas_handle passed as void *)
 int sytrd_lower(void *blas, int n, double *C, int lda, double *trd, double *VW, hipStream_t st);

@@ -830,9 +830,19 @@ std::string stencil_candidate_name(int cand) {
            " waves (FP64 or complex: RY 2, 8 waves; FP64 complex: as candidate 0), z chunk " + z;
 }
 
+int64_t stencil_full_key(int precision, int ncomp, int kt, int nb) {
+    return tune_key(precision, ncomp, kt != 0, nb) | ((int64_t)1 << 60);
+}
+
+bool stencil_feasible(const qm_ctx *ctx, int cand, int precision, int ncomp) {
+    return precision == 64 ? feasible_any<double>(ctx, cand, ncomp) : feasible_any<float>(ctx, cand, ncomp);
+}
+
 int stencil_choice(const qm_ctx *ctx, int precision, int ncomp, int kt, int nb) {
     if (ctx->stencil_force >= 0) return ctx->stencil_force;
-    auto it = ctx->stencil_choice.find(tune_key(precision, ncomp, kt != 0, nb));
+    const int64_t key = (kt & 2) ? stencil_full_key(precision, ncomp, kt & 1, nb)
+                                 : tune_key(precision, ncomp, (kt & 1) != 0, nb);
+    auto it = ctx->stencil_choice.find(key);
     return it == ctx->stencil_choice.end() ? -1 : it->second;
 }
 

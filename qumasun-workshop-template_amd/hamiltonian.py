@@ -103,11 +103,14 @@ class Hamiltonian:
         """Use stencil candidate `candidate` for every later apply (-1: autotune)."""
         _lib.check(self.lib.qm_stencil_force(self.ctx, candidate))
 
-    def stencil_choice(self, precision, nb, ncomp=None, kt=False):
-        """Candidate index in use for this apply shape (-1 if not tuned yet)."""
+    def stencil_choice(self, precision, nb, ncomp=None, kt=False, full=False):
+        """Candidate index in use for this apply shape (-1 if not tuned yet);
+        full=True: the shape of full applies (stencil beside the nonlocal
+        projection), tuned separately from local-only applies."""
         ncomp = self.sys.ncomp if ncomp is None else ncomp
         out = ctypes.c_int()
-        _lib.check(self.lib.qm_stencil_choice(self.ctx, precision, ncomp, int(kt), nb, ctypes.byref(out)))
+        _lib.check(self.lib.qm_stencil_choice(self.ctx, precision, ncomp, int(kt) | (2 if full else 0), nb,
+                                              ctypes.byref(out)))
         return out.value
 
     # -- dense products (the blas_DGEMM_t / blas_DGEMM_n sub-boundary) ------
