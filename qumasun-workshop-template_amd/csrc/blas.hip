@@ -39,8 +39,9 @@ __device__ __forceinline__ T *col_ptr_out(const ColBlocks &c, int j, int64_t ng)
 }
 
 constexpr int GRAM_FLUSH = 16;  // MFMA K-steps between FP32 -> FP64 flushes
+constexpr int kGramTN = 4;      // 16-column MFMA tiles per wave along n (workgroup tile 64 x 32 kGramTN)
 
-// Wave tile 32 x 32 (2 x 2 MFMA tiles), workgroup tile 64 x 64 (4 waves).
+// Wave tile 32 x 16 TN (2 x TN MFMA tiles), workgroup tile 64 x 32 TN (4 waves).
 // partial[chunk][j * m + i] = sum_{k in chunk} A[k][i] B[k][j]
 // Hardware workgroup b runs on XCD b % 8.  Workgroups are renumbered so that
 // consecutive numbers (the tiles of one K chunk) share an XCD and read the
@@ -51,7 +52,7 @@ __device__ __forceinline__ int xcd_block(int nblk, int per_xcd) {
     return lb < nblk ? lb : -1;
 }
 
-template <typename T, bool VECLOAD>
+template <typename T, bool VECLOAD, int TN>
 __global__ __launch_bounds__(256) void gram_partial_kernel(ColBlocks A, ColBlocks B, int m, int n, int64_t ng,
                                                            int64_t kchunk, int tiles_n, int tiles, int nblk,
                                                            int per_xcd, double *__restrict__ partial) {
@@ -70,21 +71,25 @@ __global__ __launch_bounds__(256) void gram_partial_kernel(ColBlocks A, ColBlock
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int li = lane & 15, g = lane >> 4;
     const int i0 = ti * 64 + (wave >> 1) * 32;
-    const int j0 = tj * 64 + (wave & 1) * 32;
+    const int j0 = tj * (32 * TN) + (wave & 1) * (16 * TN);
 
-    const T *pa[2], *pb[2];
+    const T *pa[2], *pb[TN];
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
-        const int ia = i0 + 16 * s + li, jb = j0 + 16 * s + li;
+        const int ia = i0 + 16 * s + li;
         pa[s] = ia < m ? col_ptr<T>(A, ia, ng) : nullptr;
-        pb[s] = jb < n ? col_ptr<T>(B, jb, ng) : nullptr;
     }
-    acc_t acc[2][2];
-    double acc64[2][2][4];
+#pragma unroll
+    for (int t = 0; t < TN; ++t) {
+        const int jb = j0 + 16 * t + li;
+        pb[t] = jb < n ? col_ptr<T>(B, jb, ng) : nullptr;
+    }
+    acc_t acc[2][TN];
+    double acc64[2][TN][4];
 #pragma unroll
     for (int s = 0; s < 2; ++s)
 #pragma unroll
-        for (int t = 0; t < 2; ++t) {
+        for (int t = 0; t < TN; ++t) {
             acc[s][t] = acc_t{0, 0, 0, 0};
 #pragma unroll
             for (int r = 0; r < 4; ++r) acc64[s][t][r] = 0.0;
@@ -93,23 +98,27 @@ __global__ __launch_bounds__(256) void gram_partial_kernel(ColBlocks A, ColBlock
     int steps = 0;
     int64_t k = k0;
     for (; k + KSTEP <= k1; k += KSTEP) {
-        T va[2][VEC], vb[2][VEC];
+        T va[2][VEC], vb[TN][VEC];
+        if constexpr (VECLOAD) {
 #pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            if constexpr (VECLOAD) {
+            for (int s = 0; s < 2; ++s) {
                 vec_t a = pa[s] ? *reinterpret_cast<const vec_t *>(pa[s] + k + VEC * g) : vec_t{};
-                vec_t b = pb[s] ? *reinterpret_cast<const vec_t *>(pb[s] + k + VEC * g) : vec_t{};
 #pragma unroll
-                for (int c = 0; c < VEC; ++c) {
-                    va[s][c] = a[c];
-                    vb[s][c] = b[c];
-                }
-            } else {
+                for (int c = 0; c < VEC; ++c) va[s][c] = a[c];
+            }
 #pragma unroll
-                for (int c = 0; c < VEC; ++c) {
-                    va[s][c] = pa[s] ? pa[s][k + VEC * g + c] : T(0);
-                    vb[s][c] = pb[s] ? pb[s][k + VEC * g + c] : T(0);
-                }
+            for (int t = 0; t < TN; ++t) {
+                vec_t bb = pb[t] ? *reinterpret_cast<const vec_t *>(pb[t] + k + VEC * g) : vec_t{};
+#pragma unroll
+                for (int c = 0; c < VEC; ++c) vb[t][c] = bb[c];
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < VEC; ++c) {
+#pragma unroll
+                for (int s = 0; s < 2; ++s) va[s][c] = pa[s] ? pa[s][k + VEC * g + c] : T(0);
+#pragma unroll
+                for (int t = 0; t < TN; ++t) vb[t][c] = pb[t] ? pb[t][k + VEC * g + c] : T(0);
             }
         }
 #pragma unroll
@@ -117,14 +126,14 @@ __global__ __launch_bounds__(256) void gram_partial_kernel(ColBlocks A, ColBlock
 #pragma unroll
             for (int s = 0; s < 2; ++s)
 #pragma unroll
-                for (int t = 0; t < 2; ++t) acc[s][t] = M::mma(va[s][c], vb[t][c], acc[s][t]);
+                for (int t = 0; t < TN; ++t) acc[s][t] = M::mma(va[s][c], vb[t][c], acc[s][t]);
         if constexpr (sizeof(T) == 4) {
             if (++steps == GRAM_FLUSH) {
                 steps = 0;
 #pragma unroll
                 for (int s = 0; s < 2; ++s)
 #pragma unroll
-                    for (int t = 0; t < 2; ++t) {
+                    for (int t = 0; t < TN; ++t) {
 #pragma unroll
                         for (int r = 0; r < 4; ++r) acc64[s][t][r] += (double)acc[s][t][r];
                         acc[s][t] = acc_t{0, 0, 0, 0};
@@ -133,27 +142,27 @@ __global__ __launch_bounds__(256) void gram_partial_kernel(ColBlocks A, ColBlock
         }
     }
     if (k < k1) {  // tail: guarded scalar loads
-        T va[2][VEC], vb[2][VEC];
+        T va[2][VEC], vb[TN][VEC];
 #pragma unroll
-        for (int s = 0; s < 2; ++s)
+        for (int c = 0; c < VEC; ++c) {
+            const int64_t kk = k + VEC * g + c;
 #pragma unroll
-            for (int c = 0; c < VEC; ++c) {
-                const int64_t kk = k + VEC * g + c;
-                va[s][c] = (pa[s] && kk < k1) ? pa[s][kk] : T(0);
-                vb[s][c] = (pb[s] && kk < k1) ? pb[s][kk] : T(0);
-            }
+            for (int s = 0; s < 2; ++s) va[s][c] = (pa[s] && kk < k1) ? pa[s][kk] : T(0);
+#pragma unroll
+            for (int t = 0; t < TN; ++t) vb[t][c] = (pb[t] && kk < k1) ? pb[t][kk] : T(0);
+        }
 #pragma unroll
         for (int c = 0; c < VEC; ++c)
 #pragma unroll
             for (int s = 0; s < 2; ++s)
 #pragma unroll
-                for (int t = 0; t < 2; ++t) acc[s][t] = M::mma(va[s][c], vb[t][c], acc[s][t]);
+                for (int t = 0; t < TN; ++t) acc[s][t] = M::mma(va[s][c], vb[t][c], acc[s][t]);
     }
     double *out = partial + (int64_t)chunk * m * n;
 #pragma unroll
     for (int s = 0; s < 2; ++s)
 #pragma unroll
-        for (int t = 0; t < 2; ++t) {
+        for (int t = 0; t < TN; ++t) {
             const int j = j0 + 16 * t + li;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
@@ -318,7 +327,8 @@ template <typename T>
 int gram(DevBuf &partial, double *C, const ColBlocks &A, const ColBlocks &B, int m, int n, int64_t ng,
          hipStream_t st) {
     constexpr int KSTEP = 4 * Mfma<T>::VEC;
-    const int tiles_m = (m + 63) / 64, tiles_n = (n + 63) / 64;
+    constexpr int TN = kGramTN;  // 16-column MFMA tiles per wave along n
+    const int tiles_m = (m + 63) / 64, tiles_n = (n + 32 * TN - 1) / (32 * TN);
     const int tiles = tiles_m * tiles_n;
     // enough workgroups to fill the 256 CUs a few times; K chunks of >= 1024
     int64_t nchunk = std::max<int64_t>(1, std::min<int64_t>(64, 2048 / tiles));
@@ -332,10 +342,10 @@ int gram(DevBuf &partial, double *C, const ColBlocks &A, const ColBlocks &B, int
     const int per_xcd = (int)((nblk + 7) / 8);
     const dim3 grid((unsigned)(8 * per_xcd));
     if (vec)
-        hipLaunchKernelGGL((gram_partial_kernel<T, true>), grid, dim3(256), 0, st, A, B, m, n, ng, kchunk, tiles_n,
+        hipLaunchKernelGGL((gram_partial_kernel<T, true, TN>), grid, dim3(256), 0, st, A, B, m, n, ng, kchunk, tiles_n,
                            tiles, (int)nblk, per_xcd, partial.as<double>());
     else
-        hipLaunchKernelGGL((gram_partial_kernel<T, false>), grid, dim3(256), 0, st, A, B, m, n, ng, kchunk, tiles_n,
+        hipLaunchKernelGGL((gram_partial_kernel<T, false, TN>), grid, dim3(256), 0, st, A, B, m, n, ng, kchunk, tiles_n,
                            tiles, (int)nblk, per_xcd, partial.as<double>());
     QM_HIP(hipGetLastError());
     const int64_t mn = (int64_t)m * n;
