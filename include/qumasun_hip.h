@@ -197,6 +197,27 @@ void qm_lobpcg_destroy(qm_lobpcg *s);
 int qm_lobpcg_solve(qm_lobpcg *s, void *d_x, double *eigenvalues, double *residual_norms, int iter_max,
                     int ik, int first_time, int *iters_done, void *stream);
 
+/* qm_lobpcg_solve with the reference's locking of converged low bands: the
+ * p_num_fix / residual_threshold arguments of Eigen_z_multi_mpi
+ * (lobpcg_z_multi_mpi.h:1732-1742).
+ * - *num_fix (in/out, 0 <= *num_fix <= nbands/2) is the caller's count of fixed
+ *   lowest bands.  It is kept between calls and reset to 0 by the caller, as
+ *   m_num_fix_state_sk is at each SCF step (qumasun_kpoint.scfloop.h:76-80).
+ * - On entry H.x and the residuals are formed for the bands at or above
+ *   *num_fix.  The new count is the first such band whose ||r||^2 (x scaled to
+ *   sum |x|^2 = 1) exceeds residual_threshold^2, capped at nbands/2 (1962-1986).
+ * - Only the bands above the count iterate.  Their R, and P when the count
+ *   changed, first lose their components along the fixed bands:
+ *   V - X_f (X_f^H V), VectorOrthogonalization (1637-1723).
+ * - The fixed bands of d_x come back as passed, up to the rounding of the
+ *   sqrt(dV) scaling round trip.  Their eigenvalues[] and residual_norms[]
+ *   entries are left as the caller passed them (in/out arrays, 2846-2848).
+ * - iter_max = 0 still updates *num_fix and, like the reference, leaves before
+ *   touching P. */
+int qm_lobpcg_solve_locking(qm_lobpcg *s, void *d_x, double *eigenvalues, double *residual_norms, int iter_max,
+                            int ik, int first_time, int *num_fix, double residual_threshold, int *iters_done,
+                            void *stream);
+
 /* The subspace problem H c = l S c is solved as in lapack_ZHEGVD_emu (w_zhegvd.h:227-272):
  * Cholesky of S, two triangular solves, a standard eigensolve, back substitution.
  * When S is not positive definite, or the squared Cholesky pivots span less than

@@ -31,7 +31,15 @@ as the product).  Follows lobpcg_z_multi_mpi.h:1732-2852:
   P columns are normalised (the reference's EVERY_NORMALIZE_P option,
   2690-2707).  Both are changes of basis of span{X, R, P}, so the iterates are
   unchanged in exact arithmetic, and S stays well conditioned.
-* No locking: residual_threshold = 0 gives num_fix = 0 (1962-1986).
+* Locking (residual_threshold, num_fix; 1962-1986): with a threshold set,
+  num_fix (kept on the instance between calls, like the caller-owned
+  m_num_fix_state_sk, and reset by the caller) becomes the first band at or
+  above its value on entry whose |r|^2 exceeds threshold^2, at most N/2.  Only
+  the bands above it iterate; their R every iteration, and P when num_fix
+  changed, lose their components along the fixed bands first
+  (VectorOrthogonalization, 1637-1723: V - X_f (X_f^H V)).  The fixed bands
+  are returned unscaled and their eigenvalue entries are left as passed
+  (2651-2652, 2796-2797, 2846-2848).  Without a threshold num_fix stays 0.
 
 Vectors are complex internally; the SoA real layout (re, im rows per band, or
 one row per band for real Gamma-point bands) is converted at the boundary.
@@ -137,14 +145,15 @@ def rr_solve(Hm, S, N, have_p, cond):
 class LOBPCGRef:
     """Keeps P between calls like the reference's `keep` block."""
 
-    def __init__(self, apply_soa, ncomp, dV, cond=1.0e-14, rdrop=1.0e-13):
+    def __init__(self, apply_soa, ncomp, dV, cond=1.0e-14, rdrop=1.0e-13, residual_threshold=None):
         self.apply_soa = apply_soa
         self.ncomp = ncomp
         self.dV = dV
         self.cond = cond    # minimum squared Cholesky pivot ratio of S
         self.rdrop = rdrop  # R directions kept above this pivot / eigenvalue ratio
-        self.P = None
-        self.P = None
+        self.residual_threshold = residual_threshold  # None: no locking
+        self.num_fix = 0    # fixed lowest bands, kept between calls
+        self.P = None       # all N bands (rows below num_fix unused)
 
     def _A(self, z):
         return soa_to_cplx(self.apply_soa(cplx_to_soa(z, self.ncomp)), self.ncomp)
@@ -154,20 +163,65 @@ class LOBPCGRef:
         """a^H b with bands as rows: (Na x Ng) . (Nb x Ng)^T."""
         return np.conj(a) @ b.T
 
-    def solve(self, X_soa, iter_max, first_time=True, return_history=False):
+    def _orthogonalise_fixed(self, xf, v):
+        """VectorOrthogonalization (lobpcg_z_multi_mpi.h:1637-1723): V - X_f (X_f^H V),
+        bands as rows."""
+        if xf.shape[0] == 0:
+            return v
+        return v - self._gram(xf, v).T @ xf
+
+    def solve(self, X_soa, iter_max, first_time=True, return_history=False, eigen_values=None):
+        """Returns (X, eigenvalues, |r|^2, iterations[, history]) over all N bands;
+        entries of the bands fixed on return are taken from `eigen_values` (NaN
+        if None) and NaN in |r|^2."""
         nc = self.ncomp
         dt = np.complex128 if nc == 2 else np.float64
-        x = soa_to_cplx(X_soa, nc).astype(dt) * np.sqrt(self.dV)
-        N = x.shape[0]
+        xall = soa_to_cplx(X_soa, nc).astype(dt) * np.sqrt(self.dV)
+        Nall = xall.shape[0]
+        lock = self.residual_threshold is not None
+        f0 = self.num_fix if lock else 0
+        assert 0 <= f0 <= Nall // 2
+        x = xall[f0:]
         Ax = self._A(x)
         eps = np.real(np.sum(np.conj(x) * Ax, axis=1))
         r = Ax - eps[:, None] * x
+        rnorm2 = np.sum(np.abs(r) ** 2, axis=1)
+        f = f0
+        if lock:
+            # first band at or above num_fix whose |r|^2 exceeds threshold^2, at most N/2 (1962-1986)
+            thr2 = self.residual_threshold ** 2
+            pos = next((k for k in range(f0, Nall) if thr2 < rnorm2[k - f0]), Nall)
+            f = min(pos, Nall // 2)
+            self.num_fix = f
+        d = f - f0
+        x, Ax, r, eps, rnorm2 = x[d:], Ax[d:], r[d:], eps[d:], rnorm2[d:]
+        xf = xall[:f]
+        N = Nall - f
+        ev_full = np.full(Nall, np.nan) if eigen_values is None else np.array(eigen_values, dtype=np.float64)
+        rn_full = np.full(Nall, np.nan)
+
+        def finish(xa, ev_a, rn_a, it_done, history):
+            # fixed bands unscaled (MulC_bundle(num_fix, x, 1/sqrt(dV)), 2651-2652 / 2796-2797)
+            out = cplx_to_soa(np.concatenate([xf / np.sqrt(self.dV), xa]), nc)
+            ev_full[f:] = ev_a
+            rn_full[f:] = rn_a
+            res = (out, ev_full, rn_full, it_done)
+            return res + (history,) if return_history else res
+
+        def keep_p(pa):
+            self.P = np.concatenate([np.zeros((f, pa.shape[1]), dtype=dt), pa])
+
+        if iter_max < 1:
+            # no iteration: undo the scaling and leave (1992-2004)
+            return finish(x / np.sqrt(self.dV), eps, rnorm2, 0, [])
         if first_time or self.P is None:
             p = np.zeros_like(x)
             Ap = np.zeros_like(x)
             have_p = False
         else:
-            p = self.P.astype(dt)
+            p = self.P[f:].astype(dt)
+            if f != f0:  # the active set changed: P loses its fixed-band components (2058-2063)
+                p = self._orthogonalise_fixed(xf, p)
             Ap = self._A(p)
             have_p = True
         blocks = {}
@@ -176,7 +230,9 @@ class LOBPCGRef:
         it_done = 0
         for it in range(iter_max):
             it_done = it + 1
-            # R <- orthonormal basis of (1 - X X^H) R (exact-arithmetic invariant), then A R
+            # R loses its fixed-band components (2080), then R <- orthonormal basis of
+            # (1 - X X^H) R (exact-arithmetic invariant), then A R
+            r = self._orthogonalise_fixed(xf, r)
             r, rkeep = orthonormalise_r(x, r, self.rdrop)
             Ar = self._A(r)
             G = self._gram
@@ -224,12 +280,9 @@ class LOBPCGRef:
                 # (Mul_invSqV_bundle with sum_norms, 2639), which leaves x
                 # unnormalised exactly when the update lost normalisation; the old
                 # x is renormalised with its own norms here.
-                p = r.copy()
                 old = np.sum(np.abs(x) ** 2, axis=1)
-                x = x / np.sqrt(old * self.dV)[:, None]
-                self.P = p
-                out = cplx_to_soa(x, nc)
-                return (out, eps, rnorm2, it_done, history) if return_history else (out, eps, rnorm2, it_done)
+                keep_p(r.copy())
+                return finish(x / np.sqrt(old * self.dV)[:, None], eps, rnorm2, it_done, history)
             final = (it + 1 == iter_max)
             # recurrence for the X / P blocks of the next iteration (1104-1436)
             Cp0 = np.concatenate([np.zeros((N, N), dtype=dt), Cr] + ([Cp] if have_p else []), axis=0)
@@ -257,6 +310,5 @@ class LOBPCGRef:
                 history.append(lam.copy())
             if final:
                 x = x / np.sqrt(norms * self.dV)[:, None]
-        self.P = p
-        out = cplx_to_soa(x, nc)
-        return (out, eps, rnorm2, it_done, history) if return_history else (out, eps, rnorm2, it_done)
+        keep_p(p)
+        return finish(x, eps, rnorm2, it_done, history)

@@ -18,7 +18,7 @@ EXPORTS = (
     "qm_nonlocal_add_f64", "qm_nonlocal_add_f32", "qm_band_density_f64", "qm_band_density_f32",
     "qm_nonlocal_forces_f64", "qm_nonlocal_forces_f32", "qm_gram_f64", "qm_gram_f32", "qm_update_f64",
     "qm_update_f32", "qm_lobpcg_create", "qm_lobpcg_destroy",
-    "qm_lobpcg_solve", "qm_lobpcg_set_conditioning", "qm_lobpcg_fallbacks", "qm_lobpcg_timings",
+    "qm_lobpcg_solve", "qm_lobpcg_solve_locking", "qm_lobpcg_set_conditioning", "qm_lobpcg_fallbacks", "qm_lobpcg_timings",
     "qm_stencil_num_candidates", "qm_stencil_candidate_name", "qm_stencil_force", "qm_stencil_choice",
     "qm_sygvd_f64",
 )
@@ -67,6 +67,7 @@ def load():
     L.qm_lobpcg_destroy.argtypes = [P]
     L.qm_lobpcg_destroy.restype = None
     L.qm_lobpcg_solve.argtypes = [P, P, P, P, I, I, I, ctypes.POINTER(I), P]
+    L.qm_lobpcg_solve_locking.argtypes = [P, P, P, P, I, I, I, ctypes.POINTER(I), D, ctypes.POINTER(I), P]
     L.qm_lobpcg_timings.argtypes = [P, P]
     L.qm_lobpcg_set_conditioning.argtypes = [P, D, D]
     L.qm_lobpcg_fallbacks.argtypes = [P]

@@ -18,6 +18,9 @@
 //                     |1 - |x'|^2| > 1e-3 rollback of 2611-2666
 // Gram blocks are recomputed from the vectors at iteration 0 and every 6th
 // iteration (S_MAT_REST_LIMIT, 1808); R blocks every iteration.
+// Locking (qm_lobpcg_solve_locking; 1962-1986, 1637-1723): the lowest num_fix
+// bands are held fixed and everything above runs on the remaining na bands,
+// with R (and P when num_fix changes) orthogonalised against the fixed bands.
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
@@ -291,6 +294,33 @@ __global__ void real_coef_kernel(T *U, const typename Sm<CPLX>::t *W, int m, int
     }
 }
 
+// U = [-C; I] ((c f + c na) x c na, column-major, T) with C the real form of
+// X_f^H V from the real Gram G = X_f^T V of the SoA columns (c f x c na), so
+// that [X_f V] U = V - X_f (X_f^H V) (VectorOrthogonalization,
+// lobpcg_z_multi_mpi.h:1637-1723; the complex unpacking is that of 1653-1670).
+template <bool CPLX, typename T>
+__global__ void fixed_coef_kernel(T *U, const double *G, int f, int na, int c) {
+    const int m = c * (f + na), n = c * na, mf = c * f;
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (int64_t)m * n) return;
+    const int row = (int)(e % m), col = (int)(e / m);
+    double v;
+    if (row >= mf) {
+        v = (row - mf == col) ? 1.0 : 0.0;
+    } else if constexpr (!CPLX) {
+        v = -G[(int64_t)col * mf + row];
+    } else {
+        // (X_f^H V)_{jk} = re + i im; rows 2j / 2j+1 (re / im of x_j), columns 2k / 2k+1
+        const int j = row >> 1, k = col >> 1;
+        auto g = [&](int a, int b) { return G[(int64_t)b * mf + a]; };
+        const double re = g(2 * j, 2 * k) + g(2 * j + 1, 2 * k + 1);
+        const double im = g(2 * j, 2 * k + 1) - g(2 * j + 1, 2 * k);
+        const bool xre = (row & 1) == 0, vre = (col & 1) == 0;
+        v = xre ? (vre ? -re : -im) : (vre ? im : -re);
+    }
+    U[e] = (T)v;
+}
+
 __global__ void cast_kernel(double *__restrict__ y, const float *__restrict__ x, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) y[i] = x[i];
 }
@@ -359,6 +389,7 @@ __global__ void svqb_penalty_kernel(typename Sm<CPLX>::t *Hr, const double *w, i
 struct qm_lobpcg {
     qm_ctx *ctx = nullptr;
     int N = 0, c = 1, prec = 64;
+    int na = 0;  // bands iterated by the current solve (N minus the fixed bands)
     int64_t ng = 0;
     size_t tsz = 8;
     // vector blocks (Ng x c*N each, T)
@@ -435,9 +466,9 @@ int dalloc(void **p, size_t bytes) {
     } while (0)
 
 template <typename T>
-int hpsi(qm_lobpcg *s, T *out, const T *in, int ik, hipStream_t st) {
-    if constexpr (sizeof(T) == 8) return qm_hpsi_apply_f64(s->ctx, out, in, s->N, s->c, ik, st);
-    else return qm_hpsi_apply_f32(s->ctx, out, in, s->N, s->c, ik, st);
+int hpsi(qm_lobpcg *s, T *out, const T *in, int nb, int ik, hipStream_t st) {
+    if constexpr (sizeof(T) == 8) return qm_hpsi_apply_f64(s->ctx, out, in, nb, s->c, ik, st);
+    else return qm_hpsi_apply_f32(s->ctx, out, in, nb, s->c, ik, st);
 }
 
 // H c = l S c as in lapack_ZHEGVD_emu (w_zhegvd.h:227-272): S = U^H U (potrf),
@@ -462,7 +493,7 @@ int chol_factor(qm_lobpcg *s, int n3, hipStream_t st) {
 
 template <bool CPLX>
 int chol_solve(qm_lobpcg *s, int n3, hipStream_t st) {
-    const int N = s->N;
+    const int N = s->na;
     if constexpr (CPLX) {
         const rocblas_double_complex one{1.0, 0.0};
         auto *U = (rocblas_double_complex *)s->Sw, *C = (rocblas_double_complex *)s->Hw;
@@ -558,7 +589,7 @@ int svqb_eig(qm_lobpcg *s, int n3, double tau, hipStream_t st) {
 template <typename T, bool CPLX>
 int orthonormalise_r(qm_lobpcg *s, T *Rout, const T *X, const T *R, hipStream_t st) {
     using SmT = typename Sm<CPLX>::t;
-    const int N = s->N, c = s->c, nc = c * N;
+    const int N = s->na, c = s->c, nc = c * N;
     const int64_t ng = s->ng;
     ColBlocks A0, B0;
     A0.add(X, nc);
@@ -653,10 +684,30 @@ int orthonormalise_r(qm_lobpcg *s, T *Rout, const T *X, const T *R, hipStream_t 
     return QM_OK;
 }
 
+// Y = V - X_f (X_f^H V) for the na active bands V against the f fixed bands X_f
+// (VectorOrthogonalization, lobpcg_z_multi_mpi.h:1637-1723): one Gram product
+// (FP64 result) and one update.  Y must not alias X_f or V.
+template <typename T, bool CPLX>
+int orthogonalise_fixed(qm_lobpcg *s, T *Y, const T *Xf, int f, const T *V, hipStream_t st) {
+    const int c = s->c, mf = c * f, nc = c * s->na;
+    ColBlocks A, B;
+    A.add(Xf, mf);
+    B.add(V, nc);
+    QM_TRY(gram<T>(s->gram_part, s->G0, A, B, mf, nc, s->ng, st));
+    hipLaunchKernelGGL((fixed_coef_kernel<CPLX, T>), dim3((unsigned)(((int64_t)(mf + nc) * nc + 255) / 256)),
+                       dim3(256), 0, st, (T *)s->U, s->G0, f, s->na, c);
+    QM_HIP(hipGetLastError());
+    ColBlocks Z, Yb;
+    Z.add(Xf, mf);
+    Z.add(V, nc);
+    Yb.add(Y, nc);
+    return update<T>(Z, Yb, (const T *)s->U, mf + nc, nc, s->ng, st);
+}
+
 // SS = Cb^H M Cb for M = S, H (n3 x n3), Cb n3 x 2N.
 template <bool CPLX>
 int recurrence(qm_lobpcg *s, int n3) {
-    const int n2 = 2 * s->N;
+    const int n2 = 2 * s->na;
     if constexpr (CPLX) {
         const rocblas_double_complex one{1.0, 0.0}, zero{0.0, 0.0};
         auto *Cb = (rocblas_double_complex *)s->Cb, *T1 = (rocblas_double_complex *)s->T1;
@@ -689,13 +740,18 @@ double now_ms() {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// num_fix == nullptr: no locking (every band iterates).  Otherwise *num_fix is
+// the caller's fixed-band count on entry and receives the new one.
 template <typename T, bool CPLX>
 int solve_impl(qm_lobpcg *s, T *x_user, double *eigenvalues, double *residual_norms, int iter_max, int ik,
-               int first_time, int *iters_done, hipStream_t st) {
+               int first_time, int *num_fix, double res_threshold, int *iters_done, hipStream_t st) {
     using SmT = typename Sm<CPLX>::t;
     qm_ctx *ctx = s->ctx;
-    const int N = s->N, c = s->c, nc = c * N;
+    const int Nall = s->N, c = s->c;
     const int64_t ng = s->ng;
+    const int f0 = num_fix ? *num_fix : 0;
+    // first column of band b of a block (SoA: c columns of ng per band)
+    auto band = [&](T *p, int b) { return p + (int64_t)c * b * ng; };
     const double dV = ctx->hx * ctx->hy * ctx->hz;
     QM_BLAS(rocblas_set_stream(s->blas, st));
     for (double &v : s->t) v = 0.0;
@@ -709,43 +765,92 @@ int solve_impl(qm_lobpcg *s, T *x_user, double *eigenvalues, double *residual_no
     };
     double t0 = now_ms();
 
-    T *X = (T *)s->X, *R = (T *)s->R, *P = (T *)s->P, *AX = (T *)s->AX, *AR = (T *)s->AR, *AP = (T *)s->AP;
-    T *W[4] = {(T *)s->W[0], (T *)s->W[1], (T *)s->W[2], (T *)s->W[3]};
+    // Block bases (all N bands) and, from here on, views of the iterated bands:
+    // every block is viewed from the same first band, so rotating views rotates
+    // the blocks.
+    T *Xb = (T *)s->X, *Rb = (T *)s->R, *Pb = (T *)s->P, *AXb = (T *)s->AX;
 
-    // x := sqrt(dV) x (lobpcg_z_multi_mpi.h:1851); A x; eps = Re x^H A x; r = Ax - eps x
-    QM_TRY(scale_bands<T>(X, x_user, nullptr, std::sqrt(dV), N, c, ng, st));
+    // x := sqrt(dV) x, all bands (lobpcg_z_multi_mpi.h:1851); for the bands not
+    // fixed on entry: A x, eps = Re x^H A x, r = Ax - eps x, |r|^2 (1863-1918)
+    QM_TRY(scale_bands<T>(Xb, x_user, nullptr, std::sqrt(dV), Nall, c, ng, st));
     QM_TRY(sync_ms(s->t[4], t0));
     t0 = now_ms();
-    QM_TRY(hpsi<T>(s, AX, X, ik, st));
-    bool hp = false;
-    if (!first_time && s->have_p) {
-        QM_TRY(hpsi<T>(s, AP, P, ik, st));
-        hp = true;
-    }
+    const int na0 = Nall - f0;
+    QM_TRY(hpsi<T>(s, band(AXb, f0), band(Xb, f0), na0, ik, st));
     QM_TRY(sync_ms(s->t[0], t0));
     t0 = now_ms();
-    QM_TRY(band_reduce<T>(s->red_part, 0, s->eps, nullptr, nullptr, X, AX, nullptr, N, c, ng, st));
-    QM_TRY(band_reduce<T>(s->red_part, 1, s->rnorm, s->xnorm, R, X, AX, s->eps, N, c, ng, st));
-    hipLaunchKernelGGL(inv_sqrt_kernel, dim3(nblk(N, 64)), dim3(64), 0, st, s->dr, s->rnorm, N, 1.0);
-    QM_HIP(hipGetLastError());
-
-    std::vector<double> h_norm(N), h_eps(N);
-    QM_HIP(hipMemcpyAsync(h_eps.data(), s->eps, sizeof(double) * N, hipMemcpyDeviceToHost, st));
+    QM_TRY(band_reduce<T>(s->red_part, 0, s->eps, nullptr, nullptr, band(Xb, f0), band(AXb, f0), nullptr, na0, c, ng,
+                          st));
+    QM_TRY(band_reduce<T>(s->red_part, 1, s->rnorm, s->xnorm, band(Rb, f0), band(Xb, f0), band(AXb, f0), s->eps, na0,
+                          c, ng, st));
+    std::vector<double> h_norm(Nall), h_eps(na0), h_rn0(na0);
+    QM_HIP(hipMemcpyAsync(h_eps.data(), s->eps, sizeof(double) * na0, hipMemcpyDeviceToHost, st));
+    if (num_fix) QM_HIP(hipMemcpyAsync(h_rn0.data(), s->rnorm, sizeof(double) * na0, hipMemcpyDeviceToHost, st));
     QM_TRY(sync_ms(s->t[4], t0));
+
+    // num_fix: the first band at or above the entry count whose |r|^2 exceeds
+    // threshold^2, at most N/2 (1962-1986); it never decreases within a call
+    int f = f0;
+    if (num_fix) {
+        const double thr2 = res_threshold * res_threshold;
+        int pos = Nall;
+        for (int k = f0; k < Nall; ++k)
+            if (thr2 < h_rn0[k - f0]) {
+                pos = k;
+                break;
+            }
+        f = std::min(pos, Nall / 2);
+        *num_fix = f;
+    }
+    const int d = f - f0;  // entry results of the iterated bands start at index d
+    const int N = Nall - f, nc = c * N;
+    s->na = N;
+    h_eps.erase(h_eps.begin(), h_eps.begin() + d);
+    int rn_off = d;  // s->rnorm index of the first iterated band
+    T *X = band(Xb, f), *R = band(Rb, f), *P = band(Pb, f), *AX = band(AXb, f), *AR = band((T *)s->AR, f),
+      *AP = band((T *)s->AP, f);
+    T *W[4] = {band((T *)s->W[0], f), band((T *)s->W[1], f), band((T *)s->W[2], f), band((T *)s->W[3], f)};
+    hipLaunchKernelGGL(inv_sqrt_kernel, dim3(nblk(N, 64)), dim3(64), 0, st, s->dr, s->rnorm + d, N, 1.0);
+    QM_HIP(hipGetLastError());
+    bool hp = false;
+    if (iter_max > 0) {
+        if (f > 0) {
+            // both X buffers of the rotation carry the fixed bands
+            QM_HIP(hipMemcpyAsync(s->W[0], Xb, s->tsz * (size_t)c * f * ng, hipMemcpyDeviceToDevice, st));
+        }
+        if (!first_time && s->have_p) {
+            // P of the iterated bands loses its fixed-band components when the
+            // count changed (2058-2063), then A P
+            t0 = now_ms();
+            if (f != f0) {
+                QM_TRY((orthogonalise_fixed<T, CPLX>(s, W[1], Xb, f, P, st)));
+                std::swap(P, W[1]);
+            }
+            QM_TRY(sync_ms(s->t[1], t0));
+            t0 = now_ms();
+            QM_TRY(hpsi<T>(s, AP, P, N, ik, st));
+            QM_TRY(sync_ms(s->t[0], t0));
+            hp = true;
+        }
+    }
 
     int it_done = 0;
     bool rolled_back = false;
     for (int it = 0; it < iter_max; ++it) {
         it_done = it + 1;
         const int nbk = hp ? 3 : 2;
-        // R <- orthonormal basis of (1 - X X^H) R
+        // R loses its fixed-band components (2080); R <- orthonormal basis of (1 - X X^H) R
         t0 = now_ms();
+        if (f > 0) {
+            QM_TRY((orthogonalise_fixed<T, CPLX>(s, AR, Xb, f, R, st)));
+            std::swap(R, AR);
+        }
         QM_TRY((orthonormalise_r<T, CPLX>(s, AR, X, R, st)));
         std::swap(R, AR);
         QM_TRY(sync_ms(s->t[1], t0));
         // A R
         t0 = now_ms();
-        QM_TRY(hpsi<T>(s, AR, R, ik, st));
+        QM_TRY(hpsi<T>(s, AR, R, N, ik, st));
         QM_TRY(sync_ms(s->t[0], t0));
         // Gram products
         t0 = now_ms();
@@ -848,6 +953,7 @@ int solve_impl(qm_lobpcg *s, T *x_user, double *eigenvalues, double *residual_no
         }
         // R' = AX' - X' L, |R'|^2, |X'|^2
         QM_TRY(band_reduce<T>(s->red_part, 1, s->rnorm, s->xnorm, AR, W[0], W[2], s->evals, N, c, ng, st));
+        rn_off = 0;
         QM_HIP(hipMemcpyAsync(h_norm.data(), s->xnorm, sizeof(double) * N, hipMemcpyDeviceToHost, st));
         QM_TRY(sync_ms(s->t[3], t0));
         double maxdev = 0.0;
@@ -862,7 +968,7 @@ int solve_impl(qm_lobpcg *s, T *x_user, double *eigenvalues, double *residual_no
             QM_TRY(band_reduce<T>(s->red_part, 2, s->xnorm, nullptr, nullptr, X, nullptr, nullptr, N, c, ng, st));
             hipLaunchKernelGGL(inv_sqrt_kernel, dim3(nblk(N, 64)), dim3(64), 0, st, s->dp, s->xnorm, N, dV);
             QM_HIP(hipGetLastError());
-            QM_TRY(scale_bands<T>(x_user, X, s->dp, 0.0, N, c, ng, st));
+            QM_TRY(scale_bands<T>(band(x_user, f), X, s->dp, 0.0, N, c, ng, st));
             QM_TRY(sync_ms(s->t[4], t0));
             rolled_back = true;
             break;
@@ -879,16 +985,18 @@ int solve_impl(qm_lobpcg *s, T *x_user, double *eigenvalues, double *residual_no
         QM_HIP(hipGetLastError());
     }
     t0 = now_ms();
+    // fixed bands: x_out = x / sqrt(dV) (2651-2652, 2796-2797); with no
+    // iteration every band leaves this way (1992-2004)
+    const int nunscale = iter_max > 0 ? f : Nall;
+    if (nunscale > 0) QM_TRY(scale_bands<T>(x_user, Xb, nullptr, 1.0 / std::sqrt(dV), nunscale, c, ng, st));
     if (!rolled_back) {
         // x_out = x / sqrt(|x|^2 dV)  (2796-2800)
         if (iter_max > 0) {
             hipLaunchKernelGGL(inv_sqrt_kernel, dim3(nblk(N, 64)), dim3(64), 0, st, s->dp, s->xnorm, N, dV);
             QM_HIP(hipGetLastError());
-            QM_TRY(scale_bands<T>(x_user, X, s->dp, 0.0, N, c, ng, st));
-        } else {
-            QM_TRY(scale_bands<T>(x_user, X, nullptr, 1.0 / std::sqrt(dV), N, c, ng, st));
+            QM_TRY(scale_bands<T>(band(x_user, f), X, s->dp, 0.0, N, c, ng, st));
         }
-        s->have_p = hp;
+        s->have_p = iter_max > 0 ? hp : (!first_time && s->have_p);
     }
     if constexpr (sizeof(T) == 4) {
         if (iter_max > 0 && !rolled_back) {
@@ -897,9 +1005,9 @@ int solve_impl(qm_lobpcg *s, T *x_user, double *eigenvalues, double *residual_no
             // (second order in the vector error, so the FP32 operator's rounding drops out)
             const int64_t nel = (int64_t)nc * ng;
             hipLaunchKernelGGL(cast_kernel, dim3((unsigned)std::min<int64_t>((nel + 255) / 256, 8192)), dim3(256), 0,
-                               st, s->X64, (const float *)x_user, nel);
+                               st, s->X64, (const float *)band(x_user, f), nel);
             QM_HIP(hipGetLastError());
-            QM_TRY(hpsi<double>(s, s->AX64, s->X64, ik, st));
+            QM_TRY(hpsi<double>(s, s->AX64, s->X64, N, ik, st));
             QM_TRY(band_reduce<double>(s->red_part, 0, s->eps, nullptr, nullptr, s->X64, s->AX64, nullptr, N, c, ng,
                                        st));
             QM_TRY(band_reduce<double>(s->red_part, 2, s->xnorm, nullptr, nullptr, s->X64, nullptr, nullptr, N, c, ng,
@@ -914,20 +1022,22 @@ int solve_impl(qm_lobpcg *s, T *x_user, double *eigenvalues, double *residual_no
             QM_HIP(hipMemcpyAsync(h_eps.data(), s->evals, sizeof(double) * N, hipMemcpyDeviceToHost, st));
         }
     }
-    // keep the rotated buffers
-    s->X = X;
-    s->R = R;
-    s->P = P;
-    s->AX = AX;
-    s->AR = AR;
-    s->AP = AP;
-    for (int i = 0; i < 4; ++i) s->W[i] = W[i];
+    // keep the rotated buffers (block bases)
+    auto base = [&](T *p) { return p - (int64_t)c * f * ng; };
+    s->X = base(X);
+    s->R = base(R);
+    s->P = base(P);
+    s->AX = base(AX);
+    s->AR = base(AR);
+    s->AP = base(AP);
+    for (int i = 0; i < 4; ++i) s->W[i] = base(W[i]);
     std::vector<double> h_rn(N);
-    QM_HIP(hipMemcpyAsync(h_rn.data(), s->rnorm, sizeof(double) * N, hipMemcpyDeviceToHost, st));
+    QM_HIP(hipMemcpyAsync(h_rn.data(), s->rnorm + rn_off, sizeof(double) * N, hipMemcpyDeviceToHost, st));
     QM_TRY(sync_ms(s->t[4], t0));
+    // entries of the fixed bands are left as the caller passed them (2846-2848)
     for (int n = 0; n < N; ++n) {
-        eigenvalues[n] = h_eps[n];
-        if (residual_norms) residual_norms[n] = h_rn[n];
+        eigenvalues[f + n] = h_eps[n];
+        if (residual_norms) residual_norms[f + n] = h_rn[n];
     }
     if (iters_done) *iters_done = it_done;
     s->t[5] = now_ms() - t_start;
@@ -1011,8 +1121,12 @@ void qm_lobpcg_destroy(qm_lobpcg *s) {
     delete s;
 }
 
-int qm_lobpcg_solve(qm_lobpcg *s, void *d_x, double *eigenvalues, double *residual_norms, int iter_max, int ik,
-                    int first_time, int *iters_done, void *stream) {
+}  // extern "C"
+
+namespace qm {
+namespace {
+int solve_dispatch(qm_lobpcg *s, void *d_x, double *eigenvalues, double *residual_norms, int iter_max, int ik,
+                   int first_time, int *num_fix, double res_threshold, int *iters_done, void *stream) {
     QM_CHECK(s && d_x && eigenvalues, QM_ERR_ARG, "qm_lobpcg_solve: NULL argument");
     QM_CHECK(iter_max >= 0, QM_ERR_ARG, "qm_lobpcg_solve: iter_max < 0");
     QM_CHECK(ik >= 0 && ik < (int)s->ctx->kpts.size(), QM_ERR_ARG, "qm_lobpcg_solve: k index out of range");
@@ -1022,15 +1136,36 @@ int qm_lobpcg_solve(qm_lobpcg *s, void *d_x, double *eigenvalues, double *residu
     if (s->prec == 64) {
         if (s->c == 2)
             return solve_impl<double, true>(s, (double *)d_x, eigenvalues, residual_norms, iter_max, ik, first_time,
-                                            iters_done, st);
+                                            num_fix, res_threshold, iters_done, st);
         return solve_impl<double, false>(s, (double *)d_x, eigenvalues, residual_norms, iter_max, ik, first_time,
-                                         iters_done, st);
+                                         num_fix, res_threshold, iters_done, st);
     }
     if (s->c == 2)
         return solve_impl<float, true>(s, (float *)d_x, eigenvalues, residual_norms, iter_max, ik, first_time,
-                                       iters_done, st);
-    return solve_impl<float, false>(s, (float *)d_x, eigenvalues, residual_norms, iter_max, ik, first_time,
-                                    iters_done, st);
+                                       num_fix, res_threshold, iters_done, st);
+    return solve_impl<float, false>(s, (float *)d_x, eigenvalues, residual_norms, iter_max, ik, first_time, num_fix,
+                                    res_threshold, iters_done, st);
+}
+}  // namespace
+}  // namespace qm
+
+extern "C" {
+
+int qm_lobpcg_solve(qm_lobpcg *s, void *d_x, double *eigenvalues, double *residual_norms, int iter_max, int ik,
+                    int first_time, int *iters_done, void *stream) {
+    return solve_dispatch(s, d_x, eigenvalues, residual_norms, iter_max, ik, first_time, nullptr, 0.0, iters_done,
+                          stream);
+}
+
+int qm_lobpcg_solve_locking(qm_lobpcg *s, void *d_x, double *eigenvalues, double *residual_norms, int iter_max,
+                            int ik, int first_time, int *num_fix, double residual_threshold, int *iters_done,
+                            void *stream) {
+    QM_CHECK(s && num_fix, QM_ERR_ARG, "qm_lobpcg_solve_locking: NULL argument");
+    QM_CHECK(*num_fix >= 0 && *num_fix <= s->N / 2, QM_ERR_ARG,
+             "qm_lobpcg_solve_locking: num_fix must be in [0, nbands/2]");
+    QM_CHECK(residual_threshold >= 0.0, QM_ERR_ARG, "qm_lobpcg_solve_locking: residual_threshold < 0");
+    return solve_dispatch(s, d_x, eigenvalues, residual_norms, iter_max, ik, first_time, num_fix, residual_threshold,
+                          iters_done, stream);
 }
 
 int qm_lobpcg_set_conditioning(qm_lobpcg *s, double min_pivot_ratio_fp64, double min_pivot_ratio_fp32) {

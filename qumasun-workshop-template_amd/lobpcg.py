@@ -8,6 +8,12 @@ reference's `keep` argument.  `solve(x, iter_max, k_index, first_time)` runs
 the iterations in place on the torch tensor `x` (SoA band layout, FP64 for
 precision 64, FP32 for precision 32) and returns the eigenvalues (Ha) and the
 squared residual norms of the last residual block.
+
+With `residual_threshold` set, the solver locks converged low bands as the
+reference does (p_num_fix / residual_threshold, 1962-1986): `num_fix` is kept
+on the object between calls (reset it with `reset_locking()` at each SCF step,
+as qumasun_kpoint.scfloop.h:76-80 does), and the eigenvalue / residual entries
+of fixed bands keep their values from the call that last iterated them.
 """
 from __future__ import annotations
 
@@ -38,8 +44,12 @@ def cluster_mean_errors(ev, ref, split=1.0e-5):
 class LOBPCG:
     PHASES = ("hpsi", "gram", "eigensolve", "update", "other", "total")
 
-    def __init__(self, hamiltonian, nbands, precision=64, ncomp=None):
+    def __init__(self, hamiltonian, nbands, precision=64, ncomp=None, residual_threshold=None):
         self.H = hamiltonian
+        self.residual_threshold = residual_threshold  # None: no locking
+        self.num_fix = 0
+        self._ev = np.zeros(nbands)
+        self._rn = np.zeros(nbands)
         self.lib = hamiltonian.lib
         self.nbands = nbands
         self.ncomp = hamiltonian.sys.ncomp if ncomp is None else ncomp
@@ -65,15 +75,26 @@ class LOBPCG:
         want = torch.float64 if self.precision == 64 else torch.float32
         assert x.is_cuda and x.is_contiguous() and x.dtype == want
         assert x.numel() == self.nbands * self.ncomp * self.H.sys.ng
-        ev = np.zeros(self.nbands)
-        rn = np.zeros(self.nbands)
+        ev, rn = self._ev, self._rn  # in/out: entries of fixed bands are kept
         it = ctypes.c_int(0)
         st = ctypes.c_void_p(torch.cuda.current_stream(x.device).cuda_stream)
-        _lib.check(self.lib.qm_lobpcg_solve(self.h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(ev.ctypes.data),
-                                            ctypes.c_void_p(rn.ctypes.data), iter_max, k_index, int(first_time),
-                                            ctypes.byref(it), st))
+        if self.residual_threshold is None:
+            _lib.check(self.lib.qm_lobpcg_solve(self.h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(ev.ctypes.data),
+                                                ctypes.c_void_p(rn.ctypes.data), iter_max, k_index, int(first_time),
+                                                ctypes.byref(it), st))
+        else:
+            nf = ctypes.c_int(self.num_fix)
+            _lib.check(self.lib.qm_lobpcg_solve_locking(
+                self.h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(ev.ctypes.data), ctypes.c_void_p(rn.ctypes.data),
+                iter_max, k_index, int(first_time), ctypes.byref(nf), float(self.residual_threshold), ctypes.byref(it),
+                st))
+            self.num_fix = nf.value
         self.iterations = it.value
-        return ev, rn
+        return ev.copy(), rn.copy()
+
+    def reset_locking(self):
+        """num_fix = 0 (the reference resets it at each SCF step)."""
+        self.num_fix = 0
 
     def set_conditioning(self, min_pivot_ratio_fp64=1e-14, min_pivot_ratio_fp32=1e-4):
         _lib.check(self.lib.qm_lobpcg_set_conditioning(self.h, min_pivot_ratio_fp64, min_pivot_ratio_fp32))

@@ -69,3 +69,11 @@ def test_stencil_candidate_queries(qm):
 def test_sygvd_argument_errors(qm):
     lib = qm._lib.load()
     assert lib.qm_sygvd_f64(None, 4, None, None, None, None, None, 2, None) == -1
+
+
+def test_lobpcg_locking_argument_errors(qm):
+    import ctypes as C
+    lib = qm._lib.load()
+    nf = C.c_int(0)
+    assert lib.qm_lobpcg_solve_locking(None, None, None, None, 1, 0, 1, C.byref(nf), 0.0, None, None) == -1
+    assert b"NULL" in lib.qm_last_error()

@@ -109,3 +109,99 @@ def test_lobpcg_mixed_complex_close_to_fp64(qm, torch_cuda):
     print("mixed - fp64 (Ha):", ev32 - ev64, "converged:", conv)
     assert conv[:4].all()
     assert err[conv].max() <= 1e-6
+
+
+def _locked_runs(qm, orc, torch, s, X0, schedule, prec=64, ik=0):
+    """Runs the oracle and the GPU solver through the same schedule of
+    (iterations, residual_threshold or None) calls; returns per-call
+    (num_fix, eigenvalues) for both."""
+    import lobpcg_ref
+    k = s.kpoints[ik]
+    Lo = lobpcg_ref.LOBPCGRef(lambda a: orc.hpsi(s, a, k=k), s.ncomp, s.dV)
+    H = qm.hamiltonian.Hamiltonian(s)
+    Lg = qm.lobpcg.LOBPCG(H, s.nbands, precision=prec)
+    dtype = torch.float64 if prec == 64 else torch.float32
+    x = torch.from_numpy(X0).to(device="cuda", dtype=dtype)
+    Xo = X0
+    ev_o = np.zeros(s.nbands)
+    out = []
+    for c, (iters, thr) in enumerate(schedule):
+        Lo.residual_threshold = thr
+        Lg.residual_threshold = thr
+        Xo, ev_o, _, _ = Lo.solve(Xo, iters, first_time=(c == 0), eigen_values=ev_o)
+        ev_g, _ = Lg.solve(x, iters, k_index=ik, first_time=(c == 0))
+        out.append((Lo.num_fix if thr is not None else 0, ev_o.copy(), Lg.num_fix, ev_g))
+    Xg = x.double().cpu().numpy()
+    Lg.close()
+    H.close()
+    return out, Xo, Xg
+
+
+@pytest.mark.parametrize("label,build,ik", [
+    ("C0 32^3 24 real bands", lambda qm: qm.system.config(0), 0),
+    ("C2 40^3 8 complex bands k=(1/4,1/4,1/4)", lambda qm: qm.system.config(2, nbands=8, grid=40), 3),
+], ids=["C0-real", "C2-complex"])
+def test_lobpcg_locking_matches_oracle(qm, orc, torch_cuda, label, build, ik):
+    """Locking (qm_lobpcg_solve_locking) against the oracle, FP64.
+    - Calls 1-2: a threshold every residual exceeds fixes N/2 bands on entry.
+    - Call 3: the count stays there.
+    - The last case fixes bands after an unlocked call with P kept, so P is
+      projected too.
+    Eigenvalues agree to <= 1e-12 relative, the fixed entries are left as
+    passed, the fixed bands of x come back unchanged, and the active bands
+    stay orthogonal to them.  Each schedule stops at 9 iterations: beyond
+    about 10, converging bands make S ill-conditioned and rounding
+    differences grow (3e-12 at 12 iterations here), the same limit as the
+    unlocked 8-iteration parity test."""
+    s = build(qm)
+    X0 = s.initial_bands()
+    N = s.nbands
+    print(f"case {label}: locked from the start, 3 x 3 iterations", flush=True)
+    out, Xo, Xg = _locked_runs(qm, orc, torch_cuda, s, X0, [(3, 1.0e6), (3, 1.0e6), (3, 0.0)], ik=ik)
+    for c, (nf_o, ev_o, nf_g, ev_g) in enumerate(out):
+        assert nf_o == nf_g == N // 2, (c, nf_o, nf_g)
+        err = np.abs(ev_g[N // 2:] - ev_o[N // 2:]).max() / np.abs(ev_o[N // 2:]).max()
+        print(f"call {c}: num_fix {nf_g}, max rel eigenvalue diff {err:.2e}")
+        assert err <= 1e-12
+        assert np.all(ev_g[:N // 2] == 0.0)  # never written: fixed before the first iteration
+    nf = (N // 2) * s.ncomp
+    assert np.abs(Xg[:nf] - X0[:nf]).max() <= 1e-14 * np.abs(X0).max()
+    import lobpcg_ref
+    z = lobpcg_ref.soa_to_cplx(Xg, s.ncomp)
+    assert np.abs(np.conj(z[:N // 2]) @ z[N // 2:].T * s.dV).max() < 1e-10  # active stay orthogonal to fixed
+    print(f"case {label}: unlocked 4 iterations, then locked with P kept, 3 + 2 iterations", flush=True)
+    out, Xo, Xg = _locked_runs(qm, orc, torch_cuda, s, X0, [(4, None), (3, 1.0e6), (2, 1.0e6)], ik=ik)
+    ev_first = out[0][3]
+    for c, (nf_o, ev_o, nf_g, ev_g) in enumerate(out):
+        lo = 0 if c == 0 else N // 2
+        err = np.abs(ev_g[lo:] - ev_o[lo:]).max() / np.abs(ev_o[lo:]).max()
+        print(f"call {c}: num_fix {nf_g}, max rel eigenvalue diff {err:.2e}")
+        assert nf_o == nf_g
+        assert err <= 1e-12
+        if c > 0:
+            assert np.array_equal(ev_g[:N // 2], ev_first[:N // 2])
+
+
+@pytest.mark.parametrize("prec,thr,tol", [(64, 1.0e-6, 1e-10), (32, 1.0e-4, 1e-6)])
+def test_lobpcg_locking_converges_c0(qm, torch_cuda, prec, thr, tol):
+    """With locking on from the start (bands fixed once ||r|| < thr), the
+    converged eigenvalues still match ARPACK: the locked ones keep the value
+    from the call that last iterated them.  The bound is |error| <=
+    ||r||^2 / gap, well under tol."""
+    s = qm.system.config(0)
+    X0 = s.initial_bands()
+    print(f"case C0 precision {prec} locking threshold {thr}: 25 calls x 20 iterations", flush=True)
+    H = qm.hamiltonian.Hamiltonian(s)
+    L = qm.lobpcg.LOBPCG(H, s.nbands, precision=prec, residual_threshold=thr)
+    x = torch_cuda.from_numpy(X0).to(device="cuda", dtype=torch_cuda.float64 if prec == 64 else torch_cuda.float32)
+    counts = []
+    for c in range(25):
+        ev, rn = L.solve(x, 20, first_time=(c == 0))
+        counts.append(L.num_fix)
+    L.close()
+    H.close()
+    err = qm.lobpcg.cluster_mean_errors(ev, GOLD)
+    print("num_fix per call:", counts)
+    print("cluster-mean |error| (Ha):", np.array2string(err, precision=2))
+    assert counts == sorted(counts) and counts[-1] > 0
+    assert err[:21].max() <= tol

@@ -52,3 +52,68 @@ def test_oracle_lobpcg_rollback_branch(qm, orc):
     assert L.P is not None
     nrm = (X * X).sum(axis=1) * s.dV
     assert np.abs(nrm - 1).max() < 1e-12  # the kept X is renormalised
+
+
+@pytest.mark.parametrize("ncomp,kred", [(1, (0, 0, 0)), (2, (0.25, -0.1, 0.3))])
+def test_oracle_locking_compressed_spectrum(qm, orc, ncomp, kred):
+    """Locking with a threshold every residual exceeds fixes N/2 bands on the
+    first call (the cap).  The fixed bands come back unchanged, and the active
+    bands stay orthogonal to them.  The active bands converge to the lowest
+    eigenvalues of H compressed onto the orthogonal complement of the fixed
+    bands (dense LAPACK on a null-space basis).  A wrong sign or a missing
+    conjugate in V - X_f (X_f^H V) fails both checks.
+
+    The fixed bands here are random, not eigenvectors.  The reference never
+    re-orthogonalises X itself, so rounding drift along the fixed bands,
+    which lowers the energy, grows over hundreds of iterations.  The checks
+    therefore stop at 80 iterations, where the drift is below 1e-9."""
+    import lobpcg_ref
+    import scipy.linalg
+    s = qm.system.small_system(n=10, L=4.0, natoms=2, rc=1.5, nbands=6, ncomp=ncomp, k_red=kred, seed=3)
+    k = s.kpoints[0]
+    Hd = dense_hamiltonian(s, k)
+    L = lobpcg_ref.LOBPCGRef(lambda a: orc.hpsi(s, a, k=k), ncomp, s.dV, residual_threshold=1.0e6)
+    X0 = s.initial_bands()
+    X = X0
+    ev_in = np.arange(6, dtype=np.float64) + 100.0
+    for c in range(4):
+        X, ev, rn, it = L.solve(X, 20, first_time=(c == 0), eigen_values=ev_in)
+    f = L.num_fix
+    assert f == 3
+    assert np.array_equal(ev[:f], ev_in[:f])  # fixed entries left as passed
+    assert np.all(np.isnan(rn[:f]))
+    assert np.abs(X[:f * ncomp] - X0[:f * ncomp]).max() <= 1e-14 * np.abs(X0).max()
+    z = lobpcg_ref.soa_to_cplx(X, ncomp)
+    zf, za = z[:f], z[f:]
+    assert np.abs(np.conj(zf) @ za.T * s.dV).max() < 1e-9
+    B = scipy.linalg.null_space(np.conj(zf))
+    wc = np.linalg.eigvalsh(np.conj(B).T @ Hd @ B)
+    assert np.abs(ev[f:] - wc[:3]).max() < 1e-7
+    # the unconstrained spectrum is a different answer (the check has teeth)
+    w = np.linalg.eigvalsh(Hd)
+    assert np.abs(w[:2] - wc[:2]).max() > 1e-3
+
+
+def test_oracle_locking_selection_rule(qm, orc):
+    """num_fix selection (lobpcg_z_multi_mpi.h:1962-1986): threshold 0 fixes
+    nothing; after convergence, a threshold above every residual fixes N/2 (the
+    cap); the count never decreases; locked converged bands leave the remaining
+    eigenvalues on the dense spectrum."""
+    import lobpcg_ref
+    s = qm.system.small_system(n=10, L=4.0, natoms=2, rc=1.5, nbands=6, seed=3)
+    w = np.linalg.eigvalsh(dense_hamiltonian(s, s.kpoints[0]))
+    L = lobpcg_ref.LOBPCGRef(lambda a: orc.hpsi(s, a), 1, s.dV, residual_threshold=0.0)
+    X = s.initial_bands()
+    for c in range(10):
+        X, ev, rn, it = L.solve(X, 20, first_time=(c == 0))
+        assert L.num_fix == 0
+    assert np.abs(ev[:5] - w[:5]).max() < 1e-9
+    L.residual_threshold = 1.0e-4  # above every converged residual
+    X, ev2, rn2, it = L.solve(X, 10, first_time=False, eigen_values=ev)
+    assert L.num_fix == 3
+    assert np.array_equal(ev2[:3], ev[:3])
+    assert np.abs(ev2[3:5] - w[3:5]).max() < 1e-9
+    L.residual_threshold = 0.0  # the count does not decrease
+    X, ev3, rn3, it = L.solve(X, 4, first_time=False, eigen_values=ev2)
+    assert L.num_fix == 3
+    assert np.abs(ev3[3:5] - w[3:5]).max() < 1e-9
