@@ -196,6 +196,122 @@ __global__ __launch_bounds__(kThreads) void trd_panel_kernel(double *__restrict_
     }
 }
 
+// Cholesky A = U^T U (upper triangle, column-major, n <= kMaxN) in one
+// 1024-thread workgroup, right-looking over blocks of kCB columns:
+//   - the diagonal block is factored in LDS (two barriers per column);
+//   - the block row right of it is solved, U_kk^T X = A(k, k+1:), one thread
+//     per column, and X is kept in LDS;
+//   - A(k+1:, k+1:) -= X^T X on the upper triangle, 4 x 4 tiles per thread.
+// rocSOLVER's potrf spends ~1 ms on the subspace sizes of LOBPCG (n = N and
+// 3N) in small single-block kernels; this takes one launch.  The strictly
+// lower triangle is never touched.  *info = 0, or the order of the first
+// leading minor whose pivot is not positive (or NaN), where the factorisation
+// stops as LAPACK's dpotrf (upper) does.
+constexpr int kCB = 32;
+__global__ __launch_bounds__(kThreads) void potrf_upper_kernel(double *__restrict__ A, int lda, int n,
+                                                               int *__restrict__ info) {
+    __shared__ double Ds[kCB][kCB + 1];  // diagonal block, Ds[row][col], upper triangle
+    __shared__ double rinv[kCB];         // 1 / U_jj of the block
+    __shared__ double Xs[kCB][kMaxN];    // block row of U right of the diagonal block
+    const int tid = threadIdx.x;
+    if (tid == 0) *info = 0;
+    for (int k0 = 0; k0 < n; k0 += kCB) {
+        const int b = min(kCB, n - k0);
+        const int nr = n - k0 - b;
+        for (int e = tid; e < b * b; e += kThreads) {
+            const int r = e % b, c = e / b;
+            if (r <= c) Ds[r][c] = A[(k0 + r) + (size_t)(k0 + c) * lda];
+        }
+        __syncthreads();
+        for (int j = 0; j < b; ++j) {
+            const double d = Ds[j][j];  // every thread reads the same pivot: one branch for all
+            if (!(d > 0.0)) {
+                if (tid == 0) *info = k0 + j + 1;
+                return;
+            }
+            const double ujj = sqrt(d), inv = 1.0 / ujj;
+            if (j + 1 + tid < b) Ds[j][j + 1 + tid] *= inv;  // row j of U
+            __syncthreads();
+            const int m = b - j - 1;
+            if (tid < m * m) {
+                const int i = j + 1 + tid % m, l = j + 1 + tid / m;
+                if (i <= l) Ds[i][l] = fma(-Ds[j][i], Ds[j][l], Ds[i][l]);
+            }
+            if (tid == 0) {
+                Ds[j][j] = ujj;
+                rinv[j] = inv;
+            }
+            __syncthreads();
+        }
+        for (int e = tid; e < b * b; e += kThreads) {
+            const int r = e % b, c = e / b;
+            if (r <= c) A[(k0 + r) + (size_t)(k0 + c) * lda] = Ds[r][c];
+        }
+        if (nr == 0) break;
+        // U_kk^T X = A(k0:k0+b, k0+b:n): forward substitution per column
+        if (tid < nr) {
+            double *col = A + k0 + (size_t)(k0 + b + tid) * lda;
+            double x[kCB];
+#pragma unroll
+            for (int p = 0; p < kCB; ++p) x[p] = p < b ? col[p] : 0.0;
+#pragma unroll
+            for (int p = 0; p < kCB; ++p) {
+                if (p < b) {
+                    x[p] *= rinv[p];
+#pragma unroll
+                    for (int q = p + 1; q < kCB; ++q)
+                        if (q < b) x[q] = fma(-Ds[p][q], x[p], x[q]);
+                }
+            }
+#pragma unroll
+            for (int p = 0; p < kCB; ++p)
+                if (p < b) {
+                    col[p] = x[p];
+                    Xs[p][tid] = x[p];
+                }
+        }
+        __syncthreads();
+        // A(k0+b:n, k0+b:n) -= X^T X, upper triangle, 4 x 4 tiles (ti <= tl)
+        const int nt = (nr + 3) / 4;
+        const int ntiles = nt * (nt + 1) / 2;
+        double *T = A + (k0 + b) + (size_t)(k0 + b) * lda;
+        for (int q = tid; q < ntiles; q += kThreads) {
+            int tl = (int)((sqrt(8.0 * q + 1.0) - 1.0) * 0.5);
+            while (tl * (tl + 1) / 2 > q) --tl;
+            while ((tl + 1) * (tl + 2) / 2 <= q) ++tl;
+            const int ti = q - tl * (tl + 1) / 2;
+            double acc[4][4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) acc[a][c] = 0.0;
+            for (int p = 0; p < b; ++p) {
+                double xi[4], xl[4];
+#pragma unroll
+                for (int a = 0; a < 4; ++a) {
+                    xi[a] = 4 * ti + a < nr ? Xs[p][4 * ti + a] : 0.0;
+                    xl[a] = 4 * tl + a < nr ? Xs[p][4 * tl + a] : 0.0;
+                }
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) acc[a][c] = fma(xi[a], xl[c], acc[a][c]);
+            }
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const int l = 4 * tl + c;
+                if (l >= nr) continue;
+#pragma unroll
+                for (int a = 0; a < 4; ++a) {
+                    const int i = 4 * ti + a;
+                    if (i <= l) T[i + (size_t)l * lda] -= acc[a][c];
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // Unblocked reduction of the trailing block A(it:n, it:n), nt = n - it <= kTailMax,
 // held in LDS (LAPACK's dsytd2, lower).
 __global__ __launch_bounds__(kThreads) void trd_tail_kernel(double *__restrict__ A, int lda, int n, int it,
@@ -321,6 +437,16 @@ int syev_lower(void *blas, int n, int nev, double *C, int lda, double *w, double
     return QM_OK;
 }
 
+int potrf_upper(void *blas, int n, double *A, int lda, int *info, hipStream_t st) {
+    if (n > kMaxN) {
+        QM_EIG_BLAS(rocsolver_dpotrf((rocblas_handle)blas, rocblas_fill_upper, n, A, lda, info));
+        return QM_OK;
+    }
+    hipLaunchKernelGGL(potrf_upper_kernel, dim3(1), dim3(kThreads), 0, st, A, lda, n, info);
+    QM_HIP(hipGetLastError());
+    return QM_OK;
+}
+
 int own_eig_max_n() { return kMaxN; }
 
 }  // namespace qm
@@ -367,7 +493,7 @@ int qm_sygvd_f64(qm_ctx *ctx, int n, const double *d_A, const double *d_B, doubl
     hipLaunchKernelGGL(symmetrize_upper_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, C, d_A, n);
     QM_HIP(hipGetLastError());
     QM_HIP(hipMemcpyAsync(d_U, d_B, sizeof(double) * nn, hipMemcpyDeviceToDevice, st));
-    QM_EIG_BLAS(rocsolver_dpotrf(h, rocblas_fill_upper, n, d_U, n, info));
+    QM_TRY(potrf_upper(h, n, d_U, n, info, st));
     int h_info = 0;
     QM_HIP(hipMemcpyAsync(&h_info, info, sizeof(int), hipMemcpyDeviceToHost, st));
     QM_HIP(hipStreamSynchronize(st));

@@ -485,7 +485,7 @@ int chol_factor(qm_lobpcg *s, int n3, hipStream_t st) {
     if constexpr (CPLX)
         QM_BLAS(rocsolver_zpotrf(s->blas, rocblas_fill_upper, n3, (rocblas_double_complex *)s->Sw, n3, s->info));
     else
-        QM_BLAS(rocsolver_dpotrf(s->blas, rocblas_fill_upper, n3, (double *)s->Sw, n3, s->info));
+        QM_TRY(potrf_upper(s->blas, n3, (double *)s->Sw, n3, s->info, st));
     hipLaunchKernelGGL((pivot_ratio_kernel<CPLX>), dim3(1), dim3(256), 0, st, s->svals, (const SmT *)s->Sw, n3);
     QM_HIP(hipGetLastError());
     return QM_OK;
@@ -615,7 +615,7 @@ int orthonormalise_r(qm_lobpcg *s, T *Rout, const T *X, const T *R, hipStream_t 
     if constexpr (CPLX)
         QM_BLAS(rocsolver_zpotrf(s->blas, rocblas_fill_upper, N, (rocblas_double_complex *)s->Sw, N, s->info));
     else
-        QM_BLAS(rocsolver_dpotrf(s->blas, rocblas_fill_upper, N, (double *)s->Sw, N, s->info));
+        QM_TRY(potrf_upper(s->blas, N, (double *)s->Sw, N, s->info, st));
     hipLaunchKernelGGL((pivot_ratio_kernel<CPLX>), dim3(1), dim3(256), 0, st, s->svals, (const SmT *)s->Sw, N);
     QM_HIP(hipGetLastError());
     int h_info = 0;

@@ -155,6 +155,9 @@ int sytrd_lower(void *blas, int n, double *C, int lda, double *trd, double *VW, 
 int syev_lower(void *blas, int n, int nev, double *C, int lda, double *w, double *Z, int ldz, double *trd,
                double *VW, int *info, hipStream_t st);
 int own_eig_max_n();
+// A = U^T U, upper triangle of the column-major n x n A (FP64): one workgroup
+// for n <= own_eig_max_n(), rocSOLVER above; *info (device) as LAPACK's dpotrf.
+int potrf_upper(void *blas, int n, double *A, int lda, int *info, hipStream_t st);
 constexpr int kTrdPanelCols = 96;  // VW workspace columns (n x 96) of sytrd_lower
 
 // nonlocal.hip

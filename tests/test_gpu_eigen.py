@@ -68,3 +68,18 @@ def test_sygvd_not_positive_definite(qm, H, torch_cuda):
     B[5, 5] = -1.0
     with pytest.raises(qm._lib.QMError):
         H.sygvd(torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda(), 4)
+
+
+@pytest.mark.parametrize("n,bad", [(40, 5), (100, 33), (384, 200), (384, 383)])
+def test_sygvd_reports_lapack_failing_minor(qm, H, torch_cuda, n, bad):
+    """The Cholesky of B stops at the first leading minor that is not
+    positive definite and reports its order, as LAPACK's dpotrf does (checked
+    against scipy's dpotrf on the same matrix)."""
+    torch = torch_cuda
+    A, B = _problem(n, seed=n + bad)
+    B[bad, bad] = -2.0
+    _, info = scipy.linalg.lapack.dpotrf(B, lower=0)
+    assert info > 0
+    with pytest.raises(qm._lib.QMError) as e:
+        H.sygvd(torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda(), 4)
+    assert f"leading minor {info})" in str(e.value)
