@@ -417,6 +417,13 @@ struct qm_lobpcg {
     qm::DevBuf gram_part, red_part;
     rocblas_handle blas = nullptr;
     double t[6] = {0, 0, 0, 0, 0, 0};
+    // phase timing without stream synchronisation: events recorded at phase
+    // ends, read once when the solve has synchronised (reused between solves)
+    std::vector<hipEvent_t> ev;
+    std::vector<int> ev_phase;
+    size_t ev_used = 0;
+    double *evk = nullptr;   // Ritz values of the last accepted iteration (device, N)
+    int *hinfo = nullptr;    // pinned host word: subspace-solve status, checked at the next synchronisation
 };
 
 namespace qm {
@@ -440,6 +447,13 @@ void lfree(qm_lobpcg *s) {
         }
     if (s->info) (void)hipFree(s->info);
     s->info = nullptr;
+    if (s->evk) (void)hipFree(s->evk);
+    s->evk = nullptr;
+    if (s->hinfo) (void)hipHostFree(s->hinfo);
+    s->hinfo = nullptr;
+    for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
+    s->ev.clear();
+    s->ev_phase.clear();
     s->gram_part.release();
     s->red_part.release();
     if (s->blas) rocblas_destroy_handle(s->blas);
@@ -758,12 +772,20 @@ int solve_impl(qm_lobpcg *s, T *x_user, double *eigenvalues, double *residual_no
     s->fallbacks = 0;
     s->rdrops = 0;
     const double t_start = now_ms();
-    auto sync_ms = [&](double &acc, double t0) -> int {
-        QM_HIP(hipStreamSynchronize(st));
-        acc += now_ms() - t0;
+    // end of a phase: t[ph] receives the stream time since the previous mark
+    s->ev_used = 0;
+    auto mark = [&](int ph) -> int {
+        if (s->ev_used == s->ev.size()) {
+            hipEvent_t e;
+            QM_HIP(hipEventCreate(&e));
+            s->ev.push_back(e);
+            s->ev_phase.push_back(0);
+        }
+        QM_HIP(hipEventRecord(s->ev[s->ev_used], st));
+        s->ev_phase[s->ev_used++] = ph;
         return QM_OK;
     };
-    double t0 = now_ms();
+    QM_TRY(mark(-1));
 
     // Block bases (all N bands) and, from here on, views of the iterated bands:
     // every block is viewed from the same first band, so rotating views rotates
@@ -773,20 +795,20 @@ int solve_impl(qm_lobpcg *s, T *x_user, double *eigenvalues, double *residual_no
     // x := sqrt(dV) x, all bands (lobpcg_z_multi_mpi.h:1851); for the bands not
     // fixed on entry: A x, eps = Re x^H A x, r = Ax - eps x, |r|^2 (1863-1918)
     QM_TRY(scale_bands<T>(Xb, x_user, nullptr, std::sqrt(dV), Nall, c, ng, st));
-    QM_TRY(sync_ms(s->t[4], t0));
-    t0 = now_ms();
+    QM_TRY(mark(4));
     const int na0 = Nall - f0;
     QM_TRY(hpsi<T>(s, band(AXb, f0), band(Xb, f0), na0, ik, st));
-    QM_TRY(sync_ms(s->t[0], t0));
-    t0 = now_ms();
+    QM_TRY(mark(0));
     QM_TRY(band_reduce<T>(s->red_part, 0, s->eps, nullptr, nullptr, band(Xb, f0), band(AXb, f0), nullptr, na0, c, ng,
                           st));
     QM_TRY(band_reduce<T>(s->red_part, 1, s->rnorm, s->xnorm, band(Rb, f0), band(Xb, f0), band(AXb, f0), s->eps, na0,
                           c, ng, st));
-    std::vector<double> h_norm(Nall), h_eps(na0), h_rn0(na0);
-    QM_HIP(hipMemcpyAsync(h_eps.data(), s->eps, sizeof(double) * na0, hipMemcpyDeviceToHost, st));
-    if (num_fix) QM_HIP(hipMemcpyAsync(h_rn0.data(), s->rnorm, sizeof(double) * na0, hipMemcpyDeviceToHost, st));
-    QM_TRY(sync_ms(s->t[4], t0));
+    std::vector<double> h_norm(Nall), h_eps(Nall), h_rn0(na0);
+    QM_TRY(mark(4));
+    if (num_fix) {  // the selection below reads the entry residuals
+        QM_HIP(hipMemcpyAsync(h_rn0.data(), s->rnorm, sizeof(double) * na0, hipMemcpyDeviceToHost, st));
+        QM_HIP(hipStreamSynchronize(st));
+    }
 
     // num_fix: the first band at or above the entry count whose |r|^2 exceeds
     // threshold^2, at most N/2 (1962-1986); it never decreases within a call
@@ -805,8 +827,10 @@ int solve_impl(qm_lobpcg *s, T *x_user, double *eigenvalues, double *residual_no
     const int d = f - f0;  // entry results of the iterated bands start at index d
     const int N = Nall - f, nc = c * N;
     s->na = N;
-    h_eps.erase(h_eps.begin(), h_eps.begin() + d);
     int rn_off = d;  // s->rnorm index of the first iterated band
+    // evk: the Ritz values returned (entry Rayleigh quotients until an iteration is accepted)
+    QM_HIP(hipMemcpyAsync(s->evk, s->eps + d, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
+    *s->hinfo = 0;
     T *X = band(Xb, f), *R = band(Rb, f), *P = band(Pb, f), *AX = band(AXb, f), *AR = band((T *)s->AR, f),
       *AP = band((T *)s->AP, f);
     T *W[4] = {band((T *)s->W[0], f), band((T *)s->W[1], f), band((T *)s->W[2], f), band((T *)s->W[3], f)};
@@ -821,15 +845,13 @@ int solve_impl(qm_lobpcg *s, T *x_user, double *eigenvalues, double *residual_no
         if (!first_time && s->have_p) {
             // P of the iterated bands loses its fixed-band components when the
             // count changed (2058-2063), then A P
-            t0 = now_ms();
             if (f != f0) {
                 QM_TRY((orthogonalise_fixed<T, CPLX>(s, W[1], Xb, f, P, st)));
                 std::swap(P, W[1]);
             }
-            QM_TRY(sync_ms(s->t[1], t0));
-            t0 = now_ms();
+            QM_TRY(mark(1));
             QM_TRY(hpsi<T>(s, AP, P, N, ik, st));
-            QM_TRY(sync_ms(s->t[0], t0));
+            QM_TRY(mark(0));
             hp = true;
         }
     }
@@ -840,20 +862,17 @@ int solve_impl(qm_lobpcg *s, T *x_user, double *eigenvalues, double *residual_no
         it_done = it + 1;
         const int nbk = hp ? 3 : 2;
         // R loses its fixed-band components (2080); R <- orthonormal basis of (1 - X X^H) R
-        t0 = now_ms();
         if (f > 0) {
             QM_TRY((orthogonalise_fixed<T, CPLX>(s, AR, Xb, f, R, st)));
             std::swap(R, AR);
         }
         QM_TRY((orthonormalise_r<T, CPLX>(s, AR, X, R, st)));
         std::swap(R, AR);
-        QM_TRY(sync_ms(s->t[1], t0));
+        QM_TRY(mark(1));
         // A R
-        t0 = now_ms();
         QM_TRY(hpsi<T>(s, AR, R, N, ik, st));
-        QM_TRY(sync_ms(s->t[0], t0));
+        QM_TRY(mark(0));
         // Gram products
-        t0 = now_ms();
         {
             ColBlocks A, B;
             A.add(X, nc);
@@ -878,11 +897,10 @@ int solve_impl(qm_lobpcg *s, T *x_user, double *eigenvalues, double *residual_no
                 QM_HIP(hipGetLastError());
             }
         }
-        QM_TRY(sync_ms(s->t[1], t0));
+        QM_TRY(mark(1));
         // subspace eigenproblem: Cholesky route; if S is not numerically positive
         // definite or its pivots span more than cond_limit, retry without P
         // ([X R], as on the first iteration), then fall back to the S eigenbasis
-        t0 = now_ms();
         int nbk_used = nbk;
         const void *evec = s->Hw;
         const double cond_limit = sizeof(T) == 8 ? s->cond64 : s->cond32;
@@ -900,10 +918,8 @@ int solve_impl(qm_lobpcg *s, T *x_user, double *eigenvalues, double *residual_no
             QM_HIP(hipStreamSynchronize(st));
             if (h_info == 0 && h_ratio >= cond_limit) {
                 QM_TRY(chol_solve<CPLX>(s, n3a, st));
-                QM_HIP(hipMemcpyAsync(&h_info, s->info, sizeof(int), hipMemcpyDeviceToHost, st));
-                QM_HIP(hipStreamSynchronize(st));
-                QM_CHECK(h_info == 0, QM_ERR_NUMERIC,
-                         "qm_lobpcg_solve: subspace eigensolve failed (rocSOLVER info " + std::to_string(h_info) + ")");
+                // status read into pinned memory, checked after the update's synchronisation
+                QM_HIP(hipMemcpyAsync(s->hinfo, s->info, sizeof(int), hipMemcpyDeviceToHost, st));
                 evec = s->evec_t1 ? s->T1 : s->Hw;
                 break;
             }
@@ -923,9 +939,8 @@ int solve_impl(qm_lobpcg *s, T *x_user, double *eigenvalues, double *residual_no
         }
         const int n3 = nbk_used * N;
         const bool hp_used = nbk_used == 3;
-        QM_TRY(sync_ms(s->t[2], t0));
+        QM_TRY(mark(2));
         // recurrence, update matrix, vector updates
-        t0 = now_ms();
         hipLaunchKernelGGL((build_cb_kernel<CPLX>), dim3(nblk((int64_t)n3 * 2 * N)), dim3(256), 0, st, (SmT *)s->Cb,
                            (const SmT *)evec, N, n3);
         QM_HIP(hipGetLastError());
@@ -954,22 +969,24 @@ int solve_impl(qm_lobpcg *s, T *x_user, double *eigenvalues, double *residual_no
         // R' = AX' - X' L, |R'|^2, |X'|^2
         QM_TRY(band_reduce<T>(s->red_part, 1, s->rnorm, s->xnorm, AR, W[0], W[2], s->evals, N, c, ng, st));
         rn_off = 0;
+        QM_TRY(mark(3));
         QM_HIP(hipMemcpyAsync(h_norm.data(), s->xnorm, sizeof(double) * N, hipMemcpyDeviceToHost, st));
-        QM_TRY(sync_ms(s->t[3], t0));
+        QM_HIP(hipStreamSynchronize(st));
+        QM_CHECK(*s->hinfo == 0, QM_ERR_NUMERIC,
+                 "qm_lobpcg_solve: subspace eigensolve failed (rocSOLVER info " + std::to_string(*s->hinfo) + ")");
         double maxdev = 0.0;
         for (int n = 0; n < N; ++n) maxdev = std::max(maxdev, std::fabs(1.0 - h_norm[n]));
         if (maxdev > kLimitNormX) {
             // rollback and escape (lobpcg_z_multi_mpi.h:2611-2666): P = R and leave the
             // loop.  The reference rescales the old x by the norms of the rejected
             // update (2639); the old x is renormalised with its own norms here.
-            t0 = now_ms();
             QM_HIP(hipMemcpyAsync(P, R, s->tsz * (size_t)nc * ng, hipMemcpyDeviceToDevice, st));
             s->have_p = true;
             QM_TRY(band_reduce<T>(s->red_part, 2, s->xnorm, nullptr, nullptr, X, nullptr, nullptr, N, c, ng, st));
             hipLaunchKernelGGL(inv_sqrt_kernel, dim3(nblk(N, 64)), dim3(64), 0, st, s->dp, s->xnorm, N, dV);
             QM_HIP(hipGetLastError());
             QM_TRY(scale_bands<T>(band(x_user, f), X, s->dp, 0.0, N, c, ng, st));
-            QM_TRY(sync_ms(s->t[4], t0));
+            QM_TRY(mark(4));
             rolled_back = true;
             break;
         }
@@ -980,11 +997,10 @@ int solve_impl(qm_lobpcg *s, T *x_user, double *eigenvalues, double *residual_no
         std::swap(AP, W[3]);
         std::swap(R, AR);
         hp = true;
-        QM_HIP(hipMemcpyAsync(h_eps.data(), s->evals, sizeof(double) * N, hipMemcpyDeviceToHost, st));
+        QM_HIP(hipMemcpyAsync(s->evk, s->evals, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
         hipLaunchKernelGGL(inv_sqrt_kernel, dim3(nblk(N, 64)), dim3(64), 0, st, s->dr, s->rnorm, N, 1.0);
         QM_HIP(hipGetLastError());
     }
-    t0 = now_ms();
     // fixed bands: x_out = x / sqrt(dV) (2651-2652, 2796-2797); with no
     // iteration every band leaves this way (1992-2004)
     const int nunscale = iter_max > 0 ? f : Nall;
@@ -1019,7 +1035,7 @@ int solve_impl(qm_lobpcg *s, T *x_user, double *eigenvalues, double *residual_no
                                        st));
             hipLaunchKernelGGL(ratio_kernel, dim3(nblk(N, 64)), dim3(64), 0, st, s->rnorm, s->rnorm, s->xnorm, N);
             QM_HIP(hipGetLastError());
-            QM_HIP(hipMemcpyAsync(h_eps.data(), s->evals, sizeof(double) * N, hipMemcpyDeviceToHost, st));
+            QM_HIP(hipMemcpyAsync(s->evk, s->evals, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
         }
     }
     // keep the rotated buffers (block bases)
@@ -1032,8 +1048,15 @@ int solve_impl(qm_lobpcg *s, T *x_user, double *eigenvalues, double *residual_no
     s->AP = base(AP);
     for (int i = 0; i < 4; ++i) s->W[i] = base(W[i]);
     std::vector<double> h_rn(N);
+    QM_TRY(mark(4));
+    QM_HIP(hipMemcpyAsync(h_eps.data(), s->evk, sizeof(double) * N, hipMemcpyDeviceToHost, st));
     QM_HIP(hipMemcpyAsync(h_rn.data(), s->rnorm + rn_off, sizeof(double) * N, hipMemcpyDeviceToHost, st));
-    QM_TRY(sync_ms(s->t[4], t0));
+    QM_HIP(hipStreamSynchronize(st));
+    for (size_t i = 1; i < s->ev_used; ++i) {
+        float ms = 0.0f;
+        QM_HIP(hipEventElapsedTime(&ms, s->ev[i - 1], s->ev[i]));
+        s->t[s->ev_phase[i]] += ms;
+    }
     // entries of the fixed bands are left as the caller passed them (2846-2848)
     for (int n = 0; n < N; ++n) {
         eigenvalues[f + n] = h_eps[n];
@@ -1098,6 +1121,12 @@ int qm_lobpcg_create(qm_lobpcg **out, qm_ctx *ctx, int nbands, int ncomp, int pr
     for (double **p : dv)
         if (st == QM_OK) st = dalloc((void **)p, sizeof(double) * n3);
     if (st == QM_OK) st = dalloc((void **)&s->info, sizeof(int));
+    if (st == QM_OK) st = dalloc((void **)&s->evk, sizeof(double) * N);
+    if (st == QM_OK && hipHostMalloc((void **)&s->hinfo, 64, hipHostMallocDefault) != hipSuccess) {
+        s->hinfo = nullptr;
+        set_error("qm_lobpcg_create: hipHostMalloc failed");
+        st = QM_ERR_NOMEM;
+    }
     if (st == QM_OK) st = dalloc((void **)&s->trd, sizeof(double) * 3 * (size_t)n3);
     if (st == QM_OK) st = dalloc((void **)&s->VW, sizeof(double) * (size_t)kTrdPanelCols * n3);
     if (st == QM_OK && rocblas_create_handle(&s->blas) != rocblas_status_success) {
