@@ -229,6 +229,21 @@ int qm_lobpcg_set_conditioning(qm_lobpcg *s, double min_pivot_ratio_fp64, double
 /* Number of such fallbacks taken in the last qm_lobpcg_solve (or QM_ERR_ARG). */
 int qm_lobpcg_fallbacks(qm_lobpcg *s);
 
+/* HIP graphs for the iteration segments that hold no host decision (no
+ * reference counterpart: the reference launches every kernel from the host).
+ * Within one qm_lobpcg_solve call, A.R and the Gram products of an iteration
+ * are captured the second time their block buffers recur and replayed after
+ * that.  The first occurrence always runs eagerly, so buffer sizes and the
+ * stencil shape have settled.  Replays launch the same kernels with the same
+ * arguments, so results are bit-identical to eager execution.  The graphs are
+ * destroyed when the call returns.  The Cholesky status, the pivot-ratio tests
+ * and the normalisation rollback stay host decisions between segments, as in
+ * the reference's control flow, and rocSOLVER calls are not captured.
+ * enable = 0 runs everything eagerly.  Default: on. */
+int qm_lobpcg_set_graphs(qm_lobpcg *s, int enable);
+/* Graph replays in the last qm_lobpcg_solve (or QM_ERR_ARG). */
+int qm_lobpcg_graph_replays(qm_lobpcg *s);
+
 /* Milliseconds spent in the last qm_lobpcg_solve per phase:
  * t[0] H.psi, t[1] Gram, t[2] subspace eigensolve, t[3] updates, t[4] other, t[5] total. */
 int qm_lobpcg_timings(qm_lobpcg *s, double *t6);

@@ -19,6 +19,7 @@ EXPORTS = (
     "qm_nonlocal_forces_f64", "qm_nonlocal_forces_f32", "qm_gram_f64", "qm_gram_f32", "qm_update_f64",
     "qm_update_f32", "qm_lobpcg_create", "qm_lobpcg_destroy",
     "qm_lobpcg_solve", "qm_lobpcg_solve_locking", "qm_lobpcg_set_conditioning", "qm_lobpcg_fallbacks", "qm_lobpcg_timings",
+    "qm_lobpcg_set_graphs", "qm_lobpcg_graph_replays",
     "qm_stencil_num_candidates", "qm_stencil_candidate_name", "qm_stencil_force", "qm_stencil_choice",
     "qm_sygvd_f64",
 )
@@ -71,6 +72,8 @@ def load():
     L.qm_lobpcg_timings.argtypes = [P, P]
     L.qm_lobpcg_set_conditioning.argtypes = [P, D, D]
     L.qm_lobpcg_fallbacks.argtypes = [P]
+    L.qm_lobpcg_set_graphs.argtypes = [P, I]
+    L.qm_lobpcg_graph_replays.argtypes = [P]
     L.qm_stencil_num_candidates.argtypes = []
     L.qm_stencil_candidate_name.argtypes = [I, ctypes.c_char_p, I]
     L.qm_stencil_force.argtypes = [P, I]

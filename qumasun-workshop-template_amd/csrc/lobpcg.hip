@@ -27,6 +27,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <map>
+#include <tuple>
 #include <utility>
 
 #include "qm_internal.h"
@@ -424,6 +426,9 @@ struct qm_lobpcg {
     size_t ev_used = 0;
     double *evk = nullptr;   // Ritz values of the last accepted iteration (device, N)
     int *hinfo = nullptr;    // pinned host word: subspace-solve status, checked at the next synchronisation
+    bool graphs = true;      // replay A.R + Gram of an iteration as a HIP graph
+    hipStream_t gstream = nullptr;  // capture stream (graphs launch on the caller's stream)
+    int graph_replays = 0;   // replays in the last solve
 };
 
 namespace qm {
@@ -449,6 +454,8 @@ void lfree(qm_lobpcg *s) {
     s->info = nullptr;
     if (s->evk) (void)hipFree(s->evk);
     s->evk = nullptr;
+    if (s->gstream) (void)hipStreamDestroy(s->gstream);
+    s->gstream = nullptr;
     if (s->hinfo) (void)hipHostFree(s->hinfo);
     s->hinfo = nullptr;
     for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
@@ -754,6 +761,85 @@ double now_ms() {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// A captured segment of one iteration (replayed while its key recurs in a call).
+struct SegKey {
+    int seg;
+    const void *X, *R, *P, *AX, *AR, *AP, *part;
+    bool hp, rec;
+    bool operator<(const SegKey &o) const {
+        return std::tie(seg, X, R, P, AX, AR, AP, part, hp, rec) <
+               std::tie(o.seg, o.X, o.R, o.P, o.AX, o.AR, o.AP, o.part, o.hp, o.rec);
+    }
+};
+struct SegGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+// Captures fn(cs) on the capture stream cs into g (nothing runs).  A capture
+// that fails, or whose graph holds a memset node (such nodes have replayed
+// wrongly on this hardware), leaves g.exec null; fn then runs eagerly.
+template <typename F>
+int capture_segment(SegGraph &g, F &fn, hipStream_t cs) {
+    if (hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed) != hipSuccess) {
+        (void)hipGetLastError();
+        return QM_OK;
+    }
+    const int rc = fn(cs);
+    hipStream_t st = cs;
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(st, &graph);
+    (void)hipGetLastError();
+    if (rc != QM_OK || ec != hipSuccess || !graph) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return QM_OK;
+    }
+    size_t nn = 0;
+    bool ok = hipGraphGetNodes(graph, nullptr, &nn) == hipSuccess && nn > 0;
+    if (ok) {
+        std::vector<hipGraphNode_t> nodes(nn);
+        ok = hipGraphGetNodes(graph, nodes.data(), &nn) == hipSuccess;
+        for (size_t i = 0; ok && i < nn; ++i) {
+            hipGraphNodeType t;
+            ok = hipGraphNodeGetType(nodes[i], &t) == hipSuccess && t != hipGraphNodeTypeMemset;
+        }
+    }
+    if (ok) ok = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0) == hipSuccess;
+    if (!ok) {
+        g.exec = nullptr;
+        (void)hipGraphDestroy(graph);
+        (void)hipGetLastError();
+        return QM_OK;
+    }
+    g.graph = graph;
+    return QM_OK;
+}
+
+// Runs a segment: replays its graph if this key was captured, captures it on
+// the key's second occurrence (while ok), else runs it eagerly.  A segment
+// that cannot be captured turns graphs off for the rest of the call.
+template <typename F>
+int run_segment(std::map<SegKey, SegGraph> &graphs, const SegKey &key, F &fn, bool &ok, hipStream_t st,
+                hipStream_t cs, int &replays) {
+    auto g = graphs.find(key);
+    if (g != graphs.end() && g->second.exec) {
+        QM_HIP(hipGraphLaunch(g->second.exec, st));
+        ++replays;
+        return QM_OK;
+    }
+    if (ok && g != graphs.end()) {
+        QM_TRY(capture_segment(g->second, fn, cs));
+        if (g->second.exec) {
+            QM_HIP(hipGraphLaunch(g->second.exec, st));
+            ++replays;
+            return QM_OK;
+        }
+        ok = false;
+    }
+    graphs[key];
+    return fn(st);
+}
+
 // num_fix == nullptr: no locking (every band iterates).  Otherwise *num_fix is
 // the caller's fixed-band count on entry and receives the new one.
 template <typename T, bool CPLX>
@@ -774,6 +860,21 @@ int solve_impl(qm_lobpcg *s, T *x_user, double *eigenvalues, double *residual_no
     const double t_start = now_ms();
     // end of a phase: t[ph] receives the stream time since the previous mark
     s->ev_used = 0;
+    s->graph_replays = 0;
+    std::map<SegKey, SegGraph> graphs;  // this call's segment graphs
+    struct GraphsGuard {                 // destroyed when the call returns (the stream is idle by then)
+        std::map<SegKey, SegGraph> &m;
+        hipStream_t st;
+        ~GraphsGuard() {
+            if (m.empty()) return;
+            (void)hipStreamSynchronize(st);
+            for (auto &kv : m) {
+                if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+                if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
+            }
+        }
+    } graphs_guard{graphs, st};
+    bool graphs_ok = s->graphs;
     auto mark = [&](int ph) -> int {
         if (s->ev_used == s->ev.size()) {
             hipEvent_t e;
@@ -869,34 +970,45 @@ int solve_impl(qm_lobpcg *s, T *x_user, double *eigenvalues, double *residual_no
         QM_TRY((orthonormalise_r<T, CPLX>(s, AR, X, R, st)));
         std::swap(R, AR);
         QM_TRY(mark(1));
-        // A R
-        QM_TRY(hpsi<T>(s, AR, R, N, ik, st));
-        QM_TRY(mark(0));
-        // Gram products
-        {
-            ColBlocks A, B;
-            A.add(X, nc);
-            A.add(R, nc);
-            if (hp) A.add(P, nc);
-            B.add(R, nc);
-            B.add(AR, nc);
-            QM_TRY(gram<T>(s->gram_part, s->G1, A, B, A.total(), B.total(), ng, st));
-            if (it % kRestLimit == 0) {
-                ColBlocks A2, B2;
-                A2.add(X, nc);
-                B2.add(X, nc);
-                if (hp) {
-                    A2.add(P, nc);
-                    B2.add(P, nc);
+        // A R, then the Gram products: neither holds a host decision or a library
+        // call, so each is a HIP graph from the second time its block pointers
+        // recur in this call (the buffer rotation gives two such sets; the first
+        // run is eager, so buffers and the stencil shape have settled by then)
+        const bool rec = it % kRestLimit == 0;
+        auto seg_hpsi = [&](hipStream_t sq) -> int { return hpsi<T>(s, AR, R, N, ik, sq); };
+        auto seg_gram = [&](hipStream_t sq) -> int {
+            // Gram products
+            {
+                ColBlocks A, B;
+                A.add(X, nc);
+                A.add(R, nc);
+                if (hp) A.add(P, nc);
+                B.add(R, nc);
+                B.add(AR, nc);
+                QM_TRY(gram<T>(s->gram_part, s->G1, A, B, A.total(), B.total(), ng, sq));
+                if (it % kRestLimit == 0) {
+                    ColBlocks A2, B2;
+                    A2.add(X, nc);
+                    B2.add(X, nc);
+                    if (hp) {
+                        A2.add(P, nc);
+                        B2.add(P, nc);
+                    }
+                    B2.add(AX, nc);
+                    if (hp) B2.add(AP, nc);
+                    QM_TRY(gram<T>(s->gram_part, s->G2, A2, B2, A2.total(), B2.total(), ng, sq));
+                    hipLaunchKernelGGL((extract_blocks_kernel<CPLX>), dim3(nblk((int64_t)N * N)), dim3(256), 0, sq,
+                                       (SmT *)s->blk, s->G2, N, c, hp);
+                    QM_HIP(hipGetLastError());
                 }
-                B2.add(AX, nc);
-                if (hp) B2.add(AP, nc);
-                QM_TRY(gram<T>(s->gram_part, s->G2, A2, B2, A2.total(), B2.total(), ng, st));
-                hipLaunchKernelGGL((extract_blocks_kernel<CPLX>), dim3(nblk((int64_t)N * N)), dim3(256), 0, st,
-                                   (SmT *)s->blk, s->G2, N, c, hp);
-                QM_HIP(hipGetLastError());
             }
-        }
+            return QM_OK;
+        };
+        QM_TRY(run_segment(graphs, SegKey{0, X, R, P, AX, AR, AP, nullptr, hp, false}, seg_hpsi, graphs_ok, st,
+                           s->gstream, s->graph_replays));
+        QM_TRY(mark(0));
+        QM_TRY(run_segment(graphs, SegKey{1, X, R, P, AX, AR, AP, s->gram_part.p, hp, rec}, seg_gram, graphs_ok, st,
+                           s->gstream, s->graph_replays));
         QM_TRY(mark(1));
         // subspace eigenproblem: Cholesky route; if S is not numerically positive
         // definite or its pivots span more than cond_limit, retry without P
@@ -1122,6 +1234,11 @@ int qm_lobpcg_create(qm_lobpcg **out, qm_ctx *ctx, int nbands, int ncomp, int pr
         if (st == QM_OK) st = dalloc((void **)p, sizeof(double) * n3);
     if (st == QM_OK) st = dalloc((void **)&s->info, sizeof(int));
     if (st == QM_OK) st = dalloc((void **)&s->evk, sizeof(double) * N);
+    if (st == QM_OK && hipStreamCreateWithFlags(&s->gstream, hipStreamNonBlocking) != hipSuccess) {
+        s->gstream = nullptr;
+        set_error("qm_lobpcg_create: hipStreamCreateWithFlags failed");
+        st = QM_ERR_HIP;
+    }
     if (st == QM_OK && hipHostMalloc((void **)&s->hinfo, 64, hipHostMallocDefault) != hipSuccess) {
         s->hinfo = nullptr;
         set_error("qm_lobpcg_create: hipHostMalloc failed");
@@ -1208,6 +1325,14 @@ int qm_lobpcg_set_conditioning(qm_lobpcg *s, double min_pivot_ratio_fp64, double
 }
 
 int qm_lobpcg_fallbacks(qm_lobpcg *s) { return s ? s->fallbacks : QM_ERR_ARG; }
+
+int qm_lobpcg_set_graphs(qm_lobpcg *s, int enable) {
+    QM_CHECK(s, QM_ERR_ARG, "qm_lobpcg_set_graphs: NULL solver");
+    s->graphs = enable != 0;
+    return QM_OK;
+}
+
+int qm_lobpcg_graph_replays(qm_lobpcg *s) { return s ? s->graph_replays : QM_ERR_ARG; }
 
 int qm_lobpcg_timings(qm_lobpcg *s, double *t6) {
     QM_CHECK(s && t6, QM_ERR_ARG, "qm_lobpcg_timings: NULL argument");

@@ -103,6 +103,14 @@ class LOBPCG:
     def fallbacks(self):
         return self.lib.qm_lobpcg_fallbacks(self.h)
 
+    def set_graphs(self, enable=True):
+        """HIP-graph replay of the A.R and Gram segments of an iteration (default on)."""
+        _lib.check(self.lib.qm_lobpcg_set_graphs(self.h, int(bool(enable))))
+
+    @property
+    def graph_replays(self):
+        return self.lib.qm_lobpcg_graph_replays(self.h)
+
     def timings(self):
         t = np.zeros(6)
         _lib.check(self.lib.qm_lobpcg_timings(self.h, ctypes.c_void_p(t.ctypes.data)))

@@ -205,3 +205,40 @@ def test_lobpcg_locking_converges_c0(qm, torch_cuda, prec, thr, tol):
     print("cluster-mean |error| (Ha):", np.array2string(err, precision=2))
     assert counts == sorted(counts) and counts[-1] > 0
     assert err[:21].max() <= tol
+
+
+@pytest.mark.parametrize("label,build,prec,ik", [
+    ("C0 32^3 24 real bands FP64", lambda qm: qm.system.config(0), 64, 0),
+    ("C0 32^3 24 real bands mixed", lambda qm: qm.system.config(0), 32, 0),
+    ("C2 40^3 8 complex bands k=(1/4,1/4,1/4) FP64", lambda qm: qm.system.config(2, nbands=8, grid=40), 64, 3),
+], ids=["C0-f64", "C0-mixed", "C2-complex"])
+def test_lobpcg_graph_replay_bit_identical(qm, torch_cuda, label, build, prec, ik):
+    """The A.R and Gram segments replayed as HIP graphs give the same bits as
+    eager execution: eigenvalues, residual norms and vectors, over two calls of
+    14 iterations (the first call starts without P; iterations 0, 6 and 12
+    recompute every Gram block)."""
+    torch = torch_cuda
+    s = build(qm)
+    X0 = s.initial_bands()
+    dtype = torch.float64 if prec == 64 else torch.float32
+    out = {}
+    for graphs in (False, True):
+        print(f"case {label}: graphs {graphs}", flush=True)
+        H = qm.hamiltonian.Hamiltonian(s)
+        L = qm.lobpcg.LOBPCG(H, s.nbands, precision=prec)
+        L.set_graphs(graphs)
+        x = torch.from_numpy(X0).to(device="cuda", dtype=dtype)
+        res, replays = [], []
+        for c in range(2):
+            ev, rn = L.solve(x, 14, k_index=ik, first_time=(c == 0))
+            res.append((ev, rn))
+            replays.append(L.graph_replays)
+        out[graphs] = (res, x.cpu().numpy(), replays)
+        L.close()
+        H.close()
+    (r0, x0, rep0), (r1, x1, rep1) = out[False], out[True]
+    print("graph replays per call:", rep1)
+    assert rep0 == [0, 0] and min(rep1) >= 10
+    for (e0, n0), (e1, n1) in zip(r0, r1):
+        assert np.array_equal(e0, e1) and np.array_equal(n0, n1)
+    assert np.array_equal(x0, x1)
