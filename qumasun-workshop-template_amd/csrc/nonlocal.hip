@@ -213,23 +213,25 @@ __global__ __launch_bounds__(kScatterBlock) void nl_scatter_kernel(
     const int i = pb * kScatterBlock + threadIdx.x;
     if (i >= ncov) return;
     const int g = cov_grid[i];
-    const int e0 = cov_off[i], e1 = cov_off[i + 1];
+    const int e0 = cov_off[i], e1 = cov_off[i + 1];  // e1 > e0: a covered point has an entry
+    // An entry's voxel, slot, table row and phase are requested before the
+    // point's NG values: the table row depends on the voxel index, so this
+    // dependent round trip overlaps the NG loads instead of following them.
+    int vid = cov_vox[e0], sl = cov_slot[e0];
+    T pw[kMaxProj];
+#pragma unroll
+    for (int w = 0; w < kMaxProj; ++w) pw[w] = tab16[(int64_t)vid * kMaxProj + w];
+    T cs = T(1), sn = T(0);
+    if constexpr (PH) {
+        cs = phase[2 * (int64_t)vid];
+        sn = phase[2 * (int64_t)vid + 1];
+    }
     // the point's values of all NG columns, requested together, then updated
     // in registers and written back once
     T hv[NG];
 #pragma unroll
     for (int q = 0; q < NG; ++q) hv[q] = cg + q < ncol ? hpsi[(int64_t)(cg + q) * ng + g] : T(0);
-    for (int e = e0; e < e1; ++e) {
-        const int vid = cov_vox[e];
-        const int sl = cov_slot[e];
-        T pw[kMaxProj];
-#pragma unroll
-        for (int w = 0; w < kMaxProj; ++w) pw[w] = tab16[(int64_t)vid * kMaxProj + w];
-        T cs = T(1), sn = T(0);
-        if constexpr (PH) {
-            cs = phase[2 * (int64_t)vid];
-            sn = phase[2 * (int64_t)vid + 1];
-        }
+    for (int e = e0;;) {
         if (sl >= 0) {
 #pragma unroll
             for (int q = 0; q < NG; q += NC) {
@@ -268,6 +270,15 @@ __global__ __launch_bounds__(kScatterBlock) void nl_scatter_kernel(
                     if constexpr (NC == 2) hv[q + 1] += ti;
                 }
             }
+        }
+        if (++e >= e1) break;
+        vid = cov_vox[e];
+        sl = cov_slot[e];
+#pragma unroll
+        for (int w = 0; w < kMaxProj; ++w) pw[w] = tab16[(int64_t)vid * kMaxProj + w];
+        if constexpr (PH) {
+            cs = phase[2 * (int64_t)vid];
+            sn = phase[2 * (int64_t)vid + 1];
         }
     }
 #pragma unroll
