@@ -139,15 +139,21 @@ def lobpcg_leg(qm, torch, H, s, psi64, calls, iters=20):
         ev, rn = L.solve(x, 2, first_time=True)  # warm-up (library and solver set-up)
         tot = {k: 0.0 for k in L.PHASES}
         its = 0
+        per_call, replays = [], []
         torch.cuda.synchronize()
         for c in range(calls):
             ev, rn = L.solve(x, iters, first_time=False)
             its += L.iterations
-            for k, v in L.timings().items():
+            t = L.timings()
+            for k, v in t.items():
                 tot[k] += v
+            per_call.append(t["total"] / max(L.iterations, 1))
+            replays.append(L.graph_replays)
         evs[prec], rns[prec] = ev, rn
         res[f"fp{prec}" if prec == 64 else "mixed"] = {
             "s_per_iter": tot["total"] / its / 1e3, "iterations": its,
+            "s_per_iter_median_call": float(np.median(per_call)) / 1e3,
+            "graph_replays_per_call": int(np.median(replays)) if replays else 0,
             "phase_ms_per_iter": {k: v / its for k, v in tot.items() if k != "total"}}
         print(f"[bench] LOBPCG precision {prec}: {tot['total'] / its:.2f} ms/iter over {its} iterations",
               file=sys.stderr, flush=True)
