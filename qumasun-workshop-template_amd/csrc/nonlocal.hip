@@ -187,8 +187,10 @@ __global__ void nl_coef_kernel(T *__restrict__ coef, const double *__restrict__ 
 // for the NG = NBS x GT real columns (NBS a multiple of NC) of column group
 // g (logical block = point block * groups + g).  The coefficients of the (up to kScatterSlots) atoms a workgroup
 // touches are staged in LDS; each thread reads its point's entry list and
-// table rows once for the whole group.
-template <typename T, int NC, bool PH, int NBS, int GT>
+// table rows once for the whole group.  MP: projector slots per entry (9 when
+// every atom has at most 9 projectors, l <= 2, else kMaxProj); table rows
+// keep their kMaxProj padding.
+template <typename T, int NC, bool PH, int NBS, int GT, int MP>
 __global__ __launch_bounds__(kScatterBlock) void nl_scatter_kernel(
     T *__restrict__ hpsi, const T *__restrict__ coef, int64_t ng, int ncol, int ncov,
     const int *__restrict__ cov_grid, const int *__restrict__ cov_off, const int *__restrict__ cov_vox,
@@ -196,15 +198,15 @@ __global__ __launch_bounds__(kScatterBlock) void nl_scatter_kernel(
     const int *__restrict__ vox_atom, const int *__restrict__ nproj, const int *__restrict__ proj_off,
     const T *__restrict__ tab16, const T *__restrict__ phase, int groups, int nblk, int per_xcd) {
     constexpr int NG = NBS * GT;  // columns of the group
-    __shared__ T cf_s[kScatterSlots][kMaxProj][NG];
+    __shared__ T cf_s[kScatterSlots][MP][NG];
     const int lb = xcd_block(nblk, per_xcd);
     if (lb < 0) return;  // the whole workgroup
     const int pb = lb / groups;  // point block
     const int cg = (lb - pb * groups) * NG;
     {
         const int b0 = block_atom_off[pb], nat = block_atom_off[pb + 1] - b0;
-        for (int e = threadIdx.x; e < nat * kMaxProj * NG; e += kScatterBlock) {
-            const int sl = e / (kMaxProj * NG), w = (e / NG) % kMaxProj, q = e % NG;
+        for (int e = threadIdx.x; e < nat * MP * NG; e += kScatterBlock) {
+            const int sl = e / (MP * NG), w = (e / NG) % MP, q = e % NG;
             const int a = block_atoms[b0 + sl];
             cf_s[sl][w][q] = (w < nproj[a] && cg + q < ncol) ? coef[(int64_t)(proj_off[a] + w) * ncol + cg + q] : T(0);
         }
@@ -218,9 +220,9 @@ __global__ __launch_bounds__(kScatterBlock) void nl_scatter_kernel(
     // point's NG values: the table row depends on the voxel index, so this
     // dependent round trip overlaps the NG loads instead of following them.
     int vid = cov_vox[e0], sl = cov_slot[e0];
-    T pw[kMaxProj];
+    T pw[MP];
 #pragma unroll
-    for (int w = 0; w < kMaxProj; ++w) pw[w] = tab16[(int64_t)vid * kMaxProj + w];
+    for (int w = 0; w < MP; ++w) pw[w] = tab16[(int64_t)vid * kMaxProj + w];
     T cs = T(1), sn = T(0);
     if constexpr (PH) {
         cs = phase[2 * (int64_t)vid];
@@ -237,7 +239,7 @@ __global__ __launch_bounds__(kScatterBlock) void nl_scatter_kernel(
             for (int q = 0; q < NG; q += NC) {
                 T tr = T(0), ti = T(0);
 #pragma unroll
-                for (int w = 0; w < kMaxProj; ++w) {
+                for (int w = 0; w < MP; ++w) {
                     tr += pw[w] * cf_s[sl][w][q];
                     if constexpr (NC == 2) ti += pw[w] * cf_s[sl][w][q + 1];
                 }
@@ -275,7 +277,7 @@ __global__ __launch_bounds__(kScatterBlock) void nl_scatter_kernel(
         vid = cov_vox[e];
         sl = cov_slot[e];
 #pragma unroll
-        for (int w = 0; w < kMaxProj; ++w) pw[w] = tab16[(int64_t)vid * kMaxProj + w];
+        for (int w = 0; w < MP; ++w) pw[w] = tab16[(int64_t)vid * kMaxProj + w];
         if constexpr (PH) {
             cs = phase[2 * (int64_t)vid];
             sn = phase[2 * (int64_t)vid + 1];
@@ -452,10 +454,16 @@ int nonlocal_scatter(qm_ctx *ctx, T *hpsi, int nb, const T *phase, hipStream_t s
     const int64_t nblk = (int64_t)p.nsblock * sgroups;
     QM_CHECK(nblk < (int64_t)INT32_MAX / 16, QM_ERR_ARG, "nonlocal apply: too many bands per call");
     const int per_xcd = (int)((nblk + 7) / 8);
-    hipLaunchKernelGGL((nl_scatter_kernel<T, NC, PH, NBS, SGT>), dim3(8 * per_xcd), dim3(kScatterBlock), 0, st,
-                       hpsi, coef, ctx->ng, ncol, p.ncov, p.d_cov_grid, p.d_cov_off, p.d_cov_vox, p.d_cov_slot,
-                       p.d_block_atom_off, p.d_block_atoms, p.d_vox_atom, p.d_nproj, p.d_proj_off, tab16_of<T>(p),
-                       phase, sgroups, (int)nblk, per_xcd);
+    if (p.max_m <= 9)
+        hipLaunchKernelGGL((nl_scatter_kernel<T, NC, PH, NBS, SGT, 9>), dim3(8 * per_xcd), dim3(kScatterBlock), 0, st,
+                           hpsi, coef, ctx->ng, ncol, p.ncov, p.d_cov_grid, p.d_cov_off, p.d_cov_vox, p.d_cov_slot,
+                           p.d_block_atom_off, p.d_block_atoms, p.d_vox_atom, p.d_nproj, p.d_proj_off,
+                           tab16_of<T>(p), phase, sgroups, (int)nblk, per_xcd);
+    else
+        hipLaunchKernelGGL((nl_scatter_kernel<T, NC, PH, NBS, SGT, kMaxProj>), dim3(8 * per_xcd), dim3(kScatterBlock),
+                           0, st, hpsi, coef, ctx->ng, ncol, p.ncov, p.d_cov_grid, p.d_cov_off, p.d_cov_vox,
+                           p.d_cov_slot, p.d_block_atom_off, p.d_block_atoms, p.d_vox_atom, p.d_nproj, p.d_proj_off,
+                           tab16_of<T>(p), phase, sgroups, (int)nblk, per_xcd);
     QM_HIP(hipGetLastError());
     return QM_OK;
 }

@@ -41,7 +41,39 @@ CASES = [
     # FP32 / 16 FP64), with a partial last group: 70 real, 74 complex columns
     ("C0 32^3 70 real bands", lambda qm: qm.system.config(0, nbands=70)),
     ("C2 40^3 37 complex bands 4k", lambda qm: qm.system.config(2, nbands=37, grid=40)),
+    # 16 projectors per atom (the C ABI's limit; the synthetic generator stops at l = 2, 9)
+    ("C0 32^3 16 projectors per atom", lambda qm: _pad16(qm.system.config(0, nbands=6))),
+    ("small 12^3 k 16 projectors per atom", lambda qm: _pad16(qm.system.small_system(
+        n=12, L=4.0, natoms=3, rc=2.6, nbands=3, ncomp=2, k_red=(0.5, 0.25, -0.125)))),
 ]
+
+
+def _pad16(s):
+    """Every atom's projector set widened to 16 columns: the extra columns are
+    smooth functions of the voxel coordinates times the atom's s-projector,
+    with distinct energies (both signs)."""
+    nat = s.natoms
+    tabs, energies = [], []
+    tab_off = np.zeros(nat + 1, np.int64)
+    proj_off = np.zeros(nat + 1, np.int32)
+    for a in range(nat):
+        M = int(s.nproj[a])
+        V = int(s.vox_off[a + 1] - s.vox_off[a])
+        tab = s.table[s.tab_off[a]:s.tab_off[a + 1]].reshape(M, V)
+        xyz = s.vox_xyz[s.vox_off[a]:s.vox_off[a + 1]].astype(np.float64)
+        extra = [tab[0] * np.cos(0.37 * (w + 1) * xyz[:, 0] + 0.21 * w * xyz[:, 1] - 0.13 * xyz[:, 2])
+                 for w in range(16 - M)]
+        tabs.append(np.concatenate([tab, np.array(extra)]).ravel())
+        e_extra = [0.75 * (-1) ** w * (1 + 0.1 * w) for w in range(16 - M)]
+        energies.append(np.concatenate([s.energy[s.proj_off[a]:s.proj_off[a + 1]], e_extra]))
+        tab_off[a + 1] = tab_off[a] + 16 * V
+        proj_off[a + 1] = proj_off[a] + 16
+    s.nproj = np.full(nat, 16, np.int32)
+    s.tab_off, s.proj_off = tab_off, proj_off
+    s.table = np.ascontiguousarray(np.concatenate(tabs))
+    s.dtable = np.zeros((3, s.table.size))
+    s.energy = np.ascontiguousarray(np.concatenate(energies))
+    return s
 
 
 def _ragged(qm):
