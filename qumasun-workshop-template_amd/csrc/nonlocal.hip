@@ -412,6 +412,19 @@ const double *tab16_of<double>(const Projectors &p) { return p.d_tab16_64; }
 template <>
 const float *tab16_of<float>(const Projectors &p) { return p.d_tab16_32; }
 
+constexpr int64_t kProjMinBlocks = 1024;  // 4 workgroups per CU
+
+template <typename T, int NC, bool PH, int GT>
+void launch_project(qm_ctx *ctx, const T *psi, int ncol, const T *phase, hipStream_t st) {
+    const Projectors &p = ctx->proj;
+    const int groups = (ncol + 16 * GT - 1) / (16 * GT);
+    const int64_t nblk = (int64_t)p.nchunk * groups;
+    const int per_xcd = (int)((nblk + 7) / 8);
+    hipLaunchKernelGGL((nl_project_kernel<T, NC, PH, GT>), dim3(8 * per_xcd), dim3(256), 0, st,
+                       ctx->nl_part.as<double>(), psi, ctx->ng, ncol, p.d_chunk_atom, p.d_chunk_v0, p.d_vox_off,
+                       p.d_vox_grid, tab16_of<T>(p), phase, groups, (int)nblk, per_xcd);
+}
+
 template <typename T, int NC, bool PH>
 int nonlocal_project(qm_ctx *ctx, const T *psi, int nb, const T *phase, hipStream_t st) {
     const Projectors &p = ctx->proj;
@@ -420,15 +433,16 @@ int nonlocal_project(qm_ctx *ctx, const T *psi, int nb, const T *phase, hipStrea
     QM_TRY(ctx->nl_part.reserve(sizeof(double) * (size_t)p.nchunk * kMaxProj * ncol));
     T *coef = ctx->coef.as<T>();
     // 4 tiles of 16 columns per projection workgroup share each table read;
-    // the columns beyond split over more workgroups
-    constexpr int GT = 4;
-    const int groups = (ncol + 16 * GT - 1) / (16 * GT);
-    const int64_t nblk = (int64_t)p.nchunk * groups;
-    QM_CHECK(nblk < (int64_t)INT32_MAX / 16, QM_ERR_ARG, "nonlocal apply: too many bands per call");
-    const int per_xcd = (int)((nblk + 7) / 8);
-    hipLaunchKernelGGL((nl_project_kernel<T, NC, PH, GT>), dim3(8 * per_xcd), dim3(256), 0, st,
-                       ctx->nl_part.as<double>(), psi, ctx->ng, ncol, p.d_chunk_atom, p.d_chunk_v0, p.d_vox_off,
-                       p.d_vox_grid, tab16_of<T>(p), phase, groups, (int)nblk, per_xcd);
+    // the columns beyond split over more workgroups.  A launch that would then
+    // have fewer than kProjMinBlocks workgroups (a small grid or few bands:
+    // 64^3 x 128 has 640) takes 2 tiles per workgroup instead, for twice the
+    // workgroups in flight on this latency-bound gather.  Each column's sum
+    // runs over the same voxels in the same order either way.
+    const int64_t g4 = (ncol + 63) / 64;
+    QM_CHECK((int64_t)p.nchunk * g4 * 2 < (int64_t)INT32_MAX / 16, QM_ERR_ARG,
+             "nonlocal apply: too many bands per call");
+    if ((int64_t)p.nchunk * g4 >= kProjMinBlocks) launch_project<T, NC, PH, 4>(ctx, psi, ncol, phase, st);
+    else launch_project<T, NC, PH, 2>(ctx, psi, ncol, phase, st);
     QM_HIP(hipGetLastError());
     const int64_t ncoef = (int64_t)p.ptot * ncol;
     hipLaunchKernelGGL((nl_coef_kernel<T>), dim3((unsigned)((ncoef + 255) / 256)), dim3(256), 0, st, coef,
