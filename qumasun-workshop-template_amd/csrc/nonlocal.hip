@@ -435,14 +435,15 @@ int nonlocal_project(qm_ctx *ctx, const T *psi, int nb, const T *phase, hipStrea
     // 4 tiles of 16 columns per projection workgroup share each table read;
     // the columns beyond split over more workgroups.  A launch that would then
     // have fewer than kProjMinBlocks workgroups (a small grid or few bands:
-    // 64^3 x 128 has 640) takes 2 tiles per workgroup instead, for twice the
-    // workgroups in flight on this latency-bound gather.  Each column's sum
-    // runs over the same voxels in the same order either way.
+    // 64^3 x 128 has 640) takes 1 tile per workgroup instead, for four times
+    // the workgroups in flight on this latency-bound gather (2 tiles measured
+    // 1-2 % slower there).  Each column's sum runs over the same voxels in the
+    // same order either way.
     const int64_t g4 = (ncol + 63) / 64;
-    QM_CHECK((int64_t)p.nchunk * g4 * 2 < (int64_t)INT32_MAX / 16, QM_ERR_ARG,
+    QM_CHECK((int64_t)p.nchunk * g4 * 4 < (int64_t)INT32_MAX / 16, QM_ERR_ARG,
              "nonlocal apply: too many bands per call");
     if ((int64_t)p.nchunk * g4 >= kProjMinBlocks) launch_project<T, NC, PH, 4>(ctx, psi, ncol, phase, st);
-    else launch_project<T, NC, PH, 2>(ctx, psi, ncol, phase, st);
+    else launch_project<T, NC, PH, 1>(ctx, psi, ncol, phase, st);
     QM_HIP(hipGetLastError());
     const int64_t ncoef = (int64_t)p.ptot * ncol;
     hipLaunchKernelGGL((nl_coef_kernel<T>), dim3((unsigned)((ncoef + 255) / 256)), dim3(256), 0, st, coef,

@@ -42,7 +42,7 @@ CASES = [
     ("C0 32^3 70 real bands", lambda qm: qm.system.config(0, nbands=70)),
     ("C2 40^3 37 complex bands 4k", lambda qm: qm.system.config(2, nbands=37, grid=40)),
     # enough projection workgroups (1578) for the 4-tile projection shape; the cases
-    # above take the 2-tile shape of smaller launches
+    # above take the 1-tile shape of smaller launches
     ("C3 80^3 160 bands", lambda qm: qm.system.config(3, nbands=160, grid=80)),
     # 16 projectors per atom (the C ABI's limit; the synthetic generator stops at l = 2, 9)
     ("C0 32^3 16 projectors per atom", lambda qm: _pad16(qm.system.config(0, nbands=6))),
