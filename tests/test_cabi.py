@@ -77,3 +77,10 @@ def test_lobpcg_locking_argument_errors(qm):
     nf = C.c_int(0)
     assert lib.qm_lobpcg_solve_locking(None, None, None, None, 1, 0, 1, C.byref(nf), 0.0, None, None) == -1
     assert b"NULL" in lib.qm_last_error()
+
+
+def test_lobpcg_graph_switch_argument_errors(qm):
+    lib = qm._lib.load()
+    assert lib.qm_lobpcg_set_graphs(None, 1) == -1
+    assert b"NULL" in lib.qm_last_error()
+    assert lib.qm_lobpcg_graph_replays(None) == -1
