@@ -44,6 +44,15 @@ CASES = [
     # enough projection workgroups (1578) for the 4-tile projection shape; the cases
     # above take the 1-tile shape of smaller launches
     ("C3 80^3 160 bands", lambda qm: qm.system.config(3, nbands=160, grid=80)),
+    # edge cases: one band (real and complex at a k-point), no atoms (no nonlocal term),
+    # s-only projectors (one per atom)
+    ("C0 32^3 one real band", lambda qm: qm.system.config(0, nbands=1)),
+    ("small 16^3 one complex band k", lambda qm: qm.system.small_system(n=16, L=4.0, natoms=2, rc=1.6, nbands=1,
+                                                                         ncomp=2, k_red=(0.25, -0.5, 0.125))),
+    ("small 16^3 no atoms k", lambda qm: qm.system.small_system(n=16, L=4.0, natoms=0, nbands=3, ncomp=2,
+                                                                 k_red=(0.5, 0.0, 0.25))),
+    ("small 20^3 s projectors only", lambda qm: qm.system.small_system(n=20, L=5.0, natoms=4, lmax=0, rc=1.8,
+                                                                        nbands=5)),
     # 16 projectors per atom (the C ABI's limit; the synthetic generator stops at l = 2, 9)
     ("C0 32^3 16 projectors per atom", lambda qm: _pad16(qm.system.config(0, nbands=6))),
     ("small 12^3 k 16 projectors per atom", lambda qm: _pad16(qm.system.small_system(
