@@ -41,14 +41,15 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// Sum over the block in a fixed order; every thread receives the total.
+// Sum over the block in a fixed order; every thread receives the total.  One
+// barrier: `red` may be written again only after another barrier, so the two
+// sums of a column (Householder norm, alpha) use separate buffers.
 __device__ double block_sum(double v, double *red) {
     v = wave_sum(v);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
     __syncthreads();
     double t = 0.0;
     for (int i = 0; i < kWaves; ++i) t += red[i];
-    __syncthreads();
     return t;
 }
 
@@ -89,7 +90,7 @@ __global__ __launch_bounds__(kThreads) void trd_panel_kernel(double *__restrict_
     __shared__ double vs[kMaxN];
     __shared__ double part[kCG][kMaxN];
     __shared__ double tt[2 * kNB];
-    __shared__ double red[kWaves];
+    __shared__ double red[kWaves], red2[kWaves];
     const int tid = threadIdx.x;
     const int wv = tid >> 6, ln = tid & 63;
     const int np = n - i0;
@@ -181,7 +182,7 @@ __global__ __launch_bounds__(kThreads) void trd_panel_kernel(double *__restrict_
             wr[nr++] = w;
             loc = fma(w, vs[r], loc);
         }
-        const double alpha = -0.5 * t * block_sum(loc, red);
+        const double alpha = -0.5 * t * block_sum(loc, red2);
         nr = 0;
         for (int r = tid; r < m; r += kThreads) Ws[j][j + 1 + r] = fma(alpha, vs[r], wr[nr++]);
         __syncthreads();
@@ -319,7 +320,7 @@ __global__ __launch_bounds__(kThreads) void trd_tail_kernel(double *__restrict__
                                                             double *__restrict__ tau) {
     __shared__ double L[kTailMax * kTailMax];
     __shared__ double vs[kTailMax], ws[kTailMax];
-    __shared__ double red[kWaves];
+    __shared__ double red[kWaves], red2[kWaves];
     const int tid = threadIdx.x;
     const int nt = n - it;
     for (int idx = tid; idx < nt * nt; idx += kThreads) {
@@ -349,7 +350,7 @@ __global__ __launch_bounds__(kThreads) void trd_tail_kernel(double *__restrict__
             ws[r] = acc;
             loc = fma(acc, vs[r], loc);
         }
-        const double alpha = -0.5 * t * block_sum(loc, red);
+        const double alpha = -0.5 * t * block_sum(loc, red2);
         for (int r = tid; r < m; r += kThreads) ws[r] = fma(alpha, vs[r], ws[r]);
         __syncthreads();
         for (int idx = tid; idx < m * m; idx += kThreads) {
