@@ -321,6 +321,8 @@ __global__ __launch_bounds__(kThreads) void trd_tail_kernel(double *__restrict__
     __shared__ double L[kTailMax * kTailMax];
     __shared__ double vs[kTailMax], ws[kTailMax];
     __shared__ double red[kWaves], red2[kWaves];
+    constexpr int kKG = kThreads / 128;  // k groups of the matrix-vector product
+    __shared__ double wpart[kKG][128];
     const int tid = threadIdx.x;
     const int nt = n - it;
     for (int idx = tid; idx < nt * nt; idx += kThreads) {
@@ -341,11 +343,24 @@ __global__ __launch_bounds__(kThreads) void trd_tail_kernel(double *__restrict__
             e[c] = beta;
             tau[c] = t;
         }
-        // w = tau A22 v, alpha = -1/2 tau w.v, w += alpha v, A22 -= v w^T + w v^T
+        // w = tau A22 v, alpha = -1/2 tau w.v, w += alpha v, A22 -= v w^T + w v^T.
+        // The product: rows over 128 threads, the k range split over kKG groups
+        // of them (a dependent chain of m / kKG instead of m), partials summed in
+        // a fixed order.
+        {
+            const int r = tid & 127, grp = tid >> 7;
+            const int k0 = (m * grp) / kKG, k1 = (m * (grp + 1)) / kKG;
+            double acc = 0.0;
+            if (r < m)
+                for (int k = k0; k < k1; ++k) acc = fma(L[(j + 1 + r) + (j + 1 + k) * nt], vs[k], acc);
+            wpart[grp][r] = acc;
+        }
+        __syncthreads();
         double loc = 0.0;
         for (int r = tid; r < m; r += kThreads) {
             double acc = 0.0;
-            for (int k = 0; k < m; ++k) acc = fma(L[(j + 1 + r) + (j + 1 + k) * nt], vs[k], acc);
+#pragma unroll
+            for (int g = 0; g < kKG; ++g) acc += wpart[g][r];
             acc *= t;
             ws[r] = acc;
             loc = fma(acc, vs[r], loc);
