@@ -13,7 +13,7 @@
 // w = tau (A22 v - V W^T v - W V^T v) is accumulated against the trailing
 // matrix as it stood at the panel's start), with the panel's V and W held in
 // LDS; one rocBLAS GEMM then applies the rank-32 update A22 -= V W^T + W V^T,
-// and the last <= 88 columns are reduced unblocked with the trailing block in
+// and the last <= 128 columns are reduced unblocked with the trailing block in
 // LDS.  Output follows LAPACK's lower-storage convention (d, e, tau,
 // reflectors below the subdiagonal), so rocSOLVER's stedc (divide and
 // conquer on T) and ormtr (apply Q) consume it.  The matrix is kept symmetric
@@ -30,7 +30,7 @@ namespace qm {
 namespace {
 
 constexpr int kNB = 16;         // panel width
-constexpr int kTailMax = 88;    // columns left for the LDS kernel (88^2 doubles = 61952 bytes)
+constexpr int kTailMax = 128;   // columns left for the LDS kernel (128^2 doubles = 131072 bytes)
 constexpr int kThreads = 1024;
 constexpr int kWaves = kThreads / 64;
 constexpr int kMaxN = 512;      // the panel's V and W (n x 16 each) live in LDS

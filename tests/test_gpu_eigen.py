@@ -38,7 +38,7 @@ def H(qm, torch_cuda):
     h.close()
 
 
-@pytest.mark.parametrize("n", [1, 5, 72, 88, 89, 121, 200, 384, 512, 521])
+@pytest.mark.parametrize("n", [1, 5, 72, 88, 89, 121, 128, 129, 144, 200, 384, 512, 521])
 @pytest.mark.parametrize("clusters", [False, True])
 def test_sygvd_against_lapack(H, torch_cuda, n, clusters):
     torch = torch_cuda
