@@ -529,19 +529,20 @@ int chol_solve(qm_lobpcg *s, int n3, hipStream_t st) {
     } else {
         const double one = 1.0;
         auto *U = (double *)s->Sw, *C = (double *)s->Hw;
+        if (n3 <= own_eig_max_n()) {
+            // C = U^-T H U^-1 through U^-1 and GEMMs, own tridiagonal reduction +
+            // rocSOLVER stedc / ormtr (eigenvectors in T1), then c = U^-1 y into Hw
+            auto *Ui = (double *)s->Ev, *Z = (double *)s->T1;
+            QM_TRY(chol_transform(s->blas, n3, U, C, Ui, Z, st));
+            QM_TRY(syev_lower(s->blas, n3, N, C, n3, s->evals, Z, n3, s->trd, s->VW, s->info, st));
+            QM_TRY(chol_back(s->blas, n3, N, Ui, Z, n3, C, n3));
+            s->evec_t1 = false;
+            return QM_OK;
+        }
         QM_BLAS(rocblas_dtrsm(s->blas, rocblas_side_left, rocblas_fill_upper, rocblas_operation_transpose,
                               rocblas_diagonal_non_unit, n3, n3, &one, U, n3, C, n3));
         QM_BLAS(rocblas_dtrsm(s->blas, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
                               rocblas_diagonal_non_unit, n3, n3, &one, U, n3, C, n3));
-        if (n3 <= own_eig_max_n()) {
-            // own tridiagonal reduction + rocSOLVER stedc / ormtr; eigenvectors into T1
-            auto *Z = (double *)s->T1;
-            QM_TRY(syev_lower(s->blas, n3, N, C, n3, s->evals, Z, n3, s->trd, s->VW, s->info, st));
-            QM_BLAS(rocblas_dtrsm(s->blas, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
-                                  rocblas_diagonal_non_unit, n3, N, &one, U, n3, Z, n3));
-            s->evec_t1 = true;
-            return QM_OK;
-        }
         QM_BLAS(rocsolver_dsyevd(s->blas, rocblas_evect_original, rocblas_fill_upper, n3, C, n3, s->evals, s->ework,
                                  s->info));
         QM_BLAS(rocblas_dtrsm(s->blas, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,

@@ -158,6 +158,10 @@ int own_eig_max_n();
 // A = U^T U, upper triangle of the column-major n x n A (FP64): one workgroup
 // for n <= own_eig_max_n(), rocSOLVER above; *info (device) as LAPACK's dpotrf.
 int potrf_upper(void *blas, int n, double *A, int lda, int *info, hipStream_t st);
+// C := U^-T C U^-1 with Ui = U^-1 kept (n x n each, leading dimension n; T: n x n work),
+// and Zout = Ui Z: the two halves of the Cholesky route around the eigensolve.
+int chol_transform(void *blas, int n, const double *U, double *C, double *Ui, double *T, hipStream_t st);
+int chol_back(void *blas, int n, int nev, const double *Ui, const double *Z, int ldz, double *Zout, int ldo);
 constexpr int kTrdPanelCols = 96;  // VW workspace columns (n x 96) of sytrd_lower
 
 // nonlocal.hip
