@@ -70,7 +70,7 @@ def test_sygvd_not_positive_definite(qm, H, torch_cuda):
         H.sygvd(torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda(), 4)
 
 
-@pytest.mark.parametrize("n,bad", [(40, 5), (100, 33), (384, 200), (384, 383)])
+@pytest.mark.parametrize("n,bad", [(40, 5), (100, 33), (384, 50), (384, 200), (384, 383), (200, 191)])
 def test_sygvd_reports_lapack_failing_minor(qm, H, torch_cuda, n, bad):
     """The Cholesky of B stops at the first leading minor that is not
     positive definite and reports its order, as LAPACK's dpotrf does (checked
