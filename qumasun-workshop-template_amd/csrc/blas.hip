@@ -183,8 +183,9 @@ __global__ void gram_reduce_kernel(double *__restrict__ C, const double *__restr
 
 // Y^T (n x Ng) = C^T (n x m) Z^T (m x Ng).  Workgroup: 64 output columns x 128
 // grid points; each wave 64 columns x 32 points (4 x 2 MFMA tiles).  C tiles of
-// 32 K rows x 64 columns are staged in LDS.
-constexpr int UPD_KC = 32;
+// UPD_KC K rows x 64 columns are staged in LDS (16 measured 4 % faster than 32 or
+// 64 on the FP32 384 -> 256 column update, FP64 equal).
+constexpr int UPD_KC = 16;
 template <typename T>
 __global__ __launch_bounds__(256) void update_kernel(ColBlocks Z, ColBlocks Y, const T *__restrict__ Cm, int m,
                                                      int n, int64_t ng, int tiles_n, int nblk, int per_xcd) {
