@@ -494,6 +494,9 @@ int hpsi(qm_lobpcg *s, T *out, const T *in, int nb, int ik, hipStream_t st) {
 
 // H c = l S c as in lapack_ZHEGVD_emu (w_zhegvd.h:227-272): S = U^H U (potrf),
 // C = U^-H H U^-1 (two trsm), eigh(C) (syevd), c = U^-1 y (trsm, first N columns).
+// Real n <= own_eig_max_n() instead factors S in one launch of the own
+// Cholesky, forms U^-1 once (trtri) and applies it by GEMMs, and diagonalises
+// C by the own tridiagonal reduction + stedc / ormtr.
 // Works on copies Sw, Hw; eigenvalues in s->evals, eigenvectors in Hw.  After
 // potrf the squared pivot ratio is written to s->svals[0] for the caller's
 // conditioning check; on potrf failure s->info > 0 and the rest is skipped by
