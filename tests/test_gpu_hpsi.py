@@ -136,6 +136,66 @@ def test_hpsi_c1_full_size_subset(qm, orc, torch_cuda):
     H.close()
 
 
+@pytest.mark.parametrize("cfg", [2, 3], ids=["C2 96^3 256 complex 4k", "C3 160^3 512 real"])
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_hpsi_full_size_properties(qm, orc, torch_cuda, cfg, prec):
+    """C2 (96^3, 256 complex bands, 4 k-points) and C3 (160^3, 512 real bands,
+    uneven spheres) at full size: the whole random block applied in one call
+    per k-point, 4 bands checked against the oracle, and the block's Gram
+    matrix <x_i | H x_j> Hermitian over all bands.
+
+    Hermiticity is a size-independent property that non-Hermitian mistakes
+    break (a k-gradient term without its factor i, a Bloch phase conjugated on
+    one side of a projector, V_loc read at a shifted point;
+    tests/test_oracle.py plants the first and the last in the oracle).  The defect
+    max |G - G^H| is measured against the largest off-diagonal |G|: for random
+    bands both grow like sqrt(Ng), so the ratio does not shrink with the grid,
+    whereas the diagonal grows like Ng."""
+    torch = torch_cuda
+    s = qm.system.config(cfg)
+    nb, c = s.nbands, s.ncomp
+    print(f"full size C{cfg} prec {prec}: {s.nx}^3 x {nb} bands x {c}", flush=True)
+    dtype = torch.float64 if prec == "f64" else torch.float32
+    # measured defects: ~8e-14 FP64 (the FP64 Gram sums), 1.2-1.6e-7 FP32; planted
+    # non-Hermitian mistakes give 0.15-0.24
+    tol, herm_tol = (F64_TOL, 1e-11) if prec == "f64" else (F32_TOL, 1e-5)
+    off = ~torch.eye(nb, dtype=torch.bool, device="cuda:0")
+    H = qm.hamiltonian.Hamiltonian(s)
+    g = torch.Generator(device="cuda:0").manual_seed(100 + cfg)
+    x = torch.rand((nb * c, s.ng), device="cuda:0", dtype=dtype, generator=g) - 0.5
+    y = torch.full_like(x, float("nan"))
+    bands = [0, 1, nb // 2, nb - 1]
+    rows = [b * c + cc for b in bands for cc in range(c)]
+    src = x[rows].double().cpu().numpy()
+    for ik in range(s.kpoints.shape[0]):
+        y.fill_(float("nan"))
+        H.apply(y, x, k_index=ik)
+        torch.cuda.synchronize()
+        assert bool(torch.isfinite(y).all())
+        want = orc.hpsi(s, src, k=s.kpoints[ik])
+        err = _rel(y[rows].double().cpu().numpy(), want)
+        assert err <= tol, (cfg, prec, ik, err)
+        # Gram of the complex (or real) band block, FP64 sums over grid chunks
+        G = torch.zeros((c, c, nb, nb), device="cuda:0", dtype=torch.float64)
+        for a in range(0, s.ng, 1 << 20):
+            xs = x[:, a:a + (1 << 20)].double().view(nb, c, -1)
+            ys = y[:, a:a + (1 << 20)].double().view(nb, c, -1)
+            for p in range(c):
+                for q in range(c):
+                    G[p, q] += xs[:, p] @ ys[:, q].T.contiguous()
+        if c == 1:
+            gre, gim = G[0, 0], torch.zeros_like(G[0, 0])
+        else:  # x^H y = (xr yr + xi yi) + i (xr yi - xi yr)
+            gre, gim = G[0, 0] + G[1, 1], G[0, 1] - G[1, 0]
+        scale = max(float(gre[off].abs().max()), float(gim[off].abs().max()))
+        defect = max(float((gre - gre.T).abs().max()), float((gim + gim.T).abs().max())) / scale
+        print(f"  k {ik}: oracle rel err {err:.2e}, Hermiticity defect {defect:.2e}", flush=True)
+        assert defect <= herm_tol, (cfg, prec, ik, defect)
+    H.close()
+    del x, y
+    torch.cuda.empty_cache()
+
+
 def test_hpsi_errors(qm, torch_cuda):
     torch = torch_cuda
     s = qm.system.config(0, nbands=2)

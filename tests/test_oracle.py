@@ -151,6 +151,44 @@ def test_hermitian_with_k(qm, orc):
     assert abs(a - b) < 1e-11 * abs(a)
 
 
+def _herm_defect(x, y, c):
+    """max |G - G^H| over the largest off-diagonal |G|, G = <x_i | y_j> over a
+    SoA band block: the Hermiticity check the full-size GPU tests apply to
+    y = H x."""
+    nb = x.shape[0] // c
+    xs, ys = x.reshape(nb, c, -1), y.reshape(nb, c, -1)
+    G = [[xs[:, p] @ ys[:, q].T for q in range(c)] for p in range(c)]
+    gre, gim = (G[0][0], np.zeros_like(G[0][0])) if c == 1 else (G[0][0] + G[1][1], G[0][1] - G[1][0])
+    off = ~np.eye(nb, dtype=bool)
+    scale = max(np.abs(gre[off]).max(), np.abs(gim[off]).max())
+    return max(np.abs(gre - gre.T).max(), np.abs(gim + gim.T).max()) / scale
+
+
+def test_block_hermiticity_detects_planted_mistakes(qm, orc):
+    """The block Hermiticity check holds for the oracle's H at a k-point and
+    fails, far above the GPU tests' FP32 tolerance (1e-5), for two planted
+    non-Hermitian mistakes: the k-gradient term without its factor i, and
+    V_loc multiplying the neighbouring grid point's value."""
+    s = qm.system.small_system(n=14, L=4.5, natoms=3, rc=1.8, nbands=6, ncomp=2, k_red=(0.3, -0.2, 0.1), seed=5)
+    k = s.kpoints[0]
+    x = np.random.default_rng(1).uniform(-0.5, 0.5, size=(12, s.ng))
+    y = orc.hpsi(s, x, k=k)
+    assert _herm_defect(x, y, 2) <= 1e-12
+    # g = -i k.grad x (SoA rows), from the kinetic term at k and at k = 0
+    g = orc.kinetic(s, x, k=k) - orc.kinetic(s, x) - 0.5 * float(np.dot(k, k)) * x
+    ig = np.empty_like(g)  # i g = k.grad x, an anti-Hermitian term
+    ig[0::2], ig[1::2] = -g[1::2], g[0::2]
+    assert _herm_defect(x, y - g + ig, 2) > 1e-3
+    shifted = y + s.vloc * (np.roll(x, 1, axis=1) - x)
+    assert _herm_defect(x, shifted, 2) > 1e-3
+    # real Gamma-point block, same check
+    r = qm.system.small_system(n=14, L=4.5, natoms=3, rc=1.8, nbands=6, seed=5)
+    xr = np.random.default_rng(2).uniform(-0.5, 0.5, size=(6, r.ng))
+    yr = orc.hpsi(r, xr)
+    assert _herm_defect(xr, yr, 1) <= 1e-12
+    assert _herm_defect(xr, yr + r.vloc * (np.roll(xr, 1, axis=1) - xr), 1) > 1e-3
+
+
 def test_nonlocal_bloch_covariance(qm, orc):
     # k = reciprocal lattice vector: e^{ik.r} is periodic on the grid, so
     # Vnl(k) u = e^{-ik.r} Vnl(0) (e^{ik.r} u) exactly.
