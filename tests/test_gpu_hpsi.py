@@ -130,9 +130,13 @@ def test_hpsi_c1_full_size_subset(qm, orc, torch_cuda):
         want = orc.hpsi(s, src)
         got = y[sel].double().cpu().numpy()
         assert _rel(got, want) <= tol, prec
-        # <x_i | H x_j> symmetric over the whole block (size-independent property)
+        # <x_i | H x_j> symmetric over the whole block (size-independent property),
+        # defect measured against the largest off-diagonal entry as below
         G = (x.double() @ y.double().T).cpu().numpy()
-        assert np.abs(G - G.T).max() <= (1e-10 if prec == "f64" else 1e-4) * np.abs(G).max()
+        scale = np.abs(G[~np.eye(G.shape[0], dtype=bool)]).max()
+        defect = np.abs(G - G.T).max() / scale
+        print(f"C1 {prec}: Hermiticity defect {defect:.2e}", flush=True)
+        assert defect <= (1e-11 if prec == "f64" else 1e-5), (prec, defect)
     H.close()
 
 
