@@ -681,15 +681,17 @@ int launch_slab(const qm_ctx *ctx, T *out, const T *in, const T *vloc, int nb, i
 // A candidate's shape for one element type.  The wave kernel's 8 rows per
 // lane would spill registers for FP64 or complex bands, so 4 there.  Slab
 // workgroups over 8 waves leave 128 registers per lane, which only FP32 real
-// bands fit: FP64 real and FP32 complex slabs run 8 waves of 2 rows, and FP64
-// complex bands have no slab shape.
+// bands fit: FP64 real and FP32 complex slabs run 2 rows per lane (128-142
+// registers, 3 waves per SIMD), 8 waves, or 12 for the 12-wave candidate
+// (one 8-wave workgroup per CU leaves a third of the wave slots empty; 12
+// waves fill them), and FP64 complex bands have no slab shape.
 template <typename T, int NC>
 Candidate effective(Candidate cd) {
     constexpr int bytes = (int)sizeof(T) * NC;
     if (!cd.slab) {
         if (cd.ry == 8 && bytes > 4) cd.ry = 4;
     } else if (bytes == 8) {
-        cd.nw = 8;
+        cd.nw = cd.nw == 12 ? 12 : 8;
         cd.ry = 2;
     }
     return cd;
@@ -723,6 +725,7 @@ int launch_candidate(const qm_ctx *ctx, int cand, T *out, const T *in, const T *
             if (cd.ry == 2) return launch_slab<T, NC, KT, 2, 512>(ctx, out, in, vloc, nb, cd.nw, cd.zchunk, c, st);
             return launch_slab<T, NC, KT, 4, 512>(ctx, out, in, vloc, nb, cd.nw, cd.zchunk, c, st);
         } else if constexpr (sizeof(T) * NC == 8) {
+            if (cd.nw > 8) return launch_slab<T, NC, KT, 2, 768>(ctx, out, in, vloc, nb, cd.nw, cd.zchunk, c, st);
             return launch_slab<T, NC, KT, 2, 512>(ctx, out, in, vloc, nb, cd.nw, cd.zchunk, c, st);
         }
     }
@@ -827,7 +830,8 @@ std::string stencil_candidate_name(int cand) {
     if (!cd.slab)
         return "fd8_wave_kernel: RY " + std::to_string(cd.ry) + " (4 for FP64 / complex when 8), z chunk " + z;
     return "fd8_slab_kernel: RY " + std::to_string(cd.ry) + ", " + std::to_string(cd.nw) +
-           " waves (FP64 or complex: RY 2, 8 waves; FP64 complex: as candidate 0), z chunk " + z;
+           " waves (FP64 or complex: RY 2, " + (cd.nw == 12 ? "12" : "8") +
+           " waves; FP64 complex: as candidate 0), z chunk " + z;
 }
 
 int64_t stencil_full_key(int precision, int ncomp, int kt, int nb) {
