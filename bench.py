@@ -57,6 +57,18 @@ def local_bytes(s, nb, itemsize):
     return nb * s.ng * s.ncomp * 2 * itemsize + s.ng * itemsize
 
 
+def nonlocal_bytes(s, nb, itemsize, kpoint=False):
+    """Algorithmic HBM bytes of one nonlocal apply: per projector voxel, the
+    gather of every band column, the read-modify-write of H.psi there, the
+    table row of the atom's M_a projectors and, at k != 0, the phase pair."""
+    c = s.ncomp
+    tot = 0
+    for a in range(s.natoms):
+        V = int(s.vox_off[a + 1] - s.vox_off[a])
+        tot += V * (nb * c * itemsize * 3 + int(s.nproj[a]) * itemsize + (2 * itemsize if kpoint else 0))
+    return tot
+
+
 def measured_traffic(candidate, workload):
     """HBM bytes per launch of the tuned stencil shape, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (None if that shape was not profiled)."""
@@ -69,24 +81,62 @@ def measured_traffic(candidate, workload):
     return None
 
 
-def cpu_baseline(s, psi_sample, min_seconds=10.0):
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle  # the CPU FP64 restatement of the reference path, timed as the baseline
+def cpu_model():
+    """The host CPU's model name (as lscpu prints it), from /proc/cpuinfo."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
-    cores = min(16, os.cpu_count() or 1)
-    oracle.set_threads(cores)
+
+def cpu_threads():
+    """Host threads this process may use: its CPU affinity, capped by
+    OMP_NUM_THREADS when that is set (the GPU host's CPU share)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
+
+
+def _median_rate(oracle, s, psi_sample, min_runs=5, min_seconds=2.0):
     oracle.hpsi(s, psi_sample)  # warm-up
     times = []
     t_start = time.perf_counter()
-    while time.perf_counter() - t_start < min_seconds or len(times) < 3:
+    while len(times) < min_runs or time.perf_counter() - t_start < min_seconds:
         t0 = time.perf_counter()
         oracle.hpsi(s, psi_sample)
         times.append(time.perf_counter() - t0)
     nb = psi_sample.shape[0] // s.ncomp
-    t = statistics.median(times)
-    return {"value": nb * s.ng / t, "unit": "grid-pt·band/s", "cores": cores, "kind": "port",
-            "sample": f"{nb} of {s.nbands} bands of the same workload, FP64 oracle H.psi, "
-                      f"median of {len(times)} applies ({sum(times):.1f} s)"}
+    return nb * s.ng / statistics.median(times), times
+
+
+def cpu_baseline(s, psi_sample):
+    """The FP64 CPU restatement of the reference H.psi (oracle/qm_oracle.c, built
+    -O3 -march=native with OpenMP over bands) on this host: 1 thread, and every
+    thread this process may use with OMP_PROC_BIND=close (set before the OpenMP
+    runtime starts, in main); median of >= 5 applies after a warm-up each."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle  # the CPU FP64 restatement of the reference path, timed as the baseline
+
+    n_all = cpu_threads()
+    oracle.set_threads(1)
+    one, t1 = _median_rate(oracle, s, psi_sample[:2 * s.ncomp])
+    oracle.set_threads(n_all)
+    allv, ta = _median_rate(oracle, s, psi_sample)
+    nb = psi_sample.shape[0] // s.ncomp
+    return {"value": allv, "unit": "grid-pt·band/s", "cores": n_all, "kind": "port",
+            "value_1_thread": one, "cpu_model": cpu_model(),
+            "omp_proc_bind": os.environ.get("OMP_PROC_BIND", ""),
+            "sample": f"FP64 oracle H.psi of the same workload: {nb} of {s.nbands} bands on {n_all} threads "
+                      f"(median of {len(ta)} applies, {sum(ta):.1f} s), 2 bands on 1 thread "
+                      f"(median of {len(t1)}, {sum(t1):.1f} s)"}
 
 
 def hpsi_config_leg(qm, torch, cfg, steps, warmup):
@@ -190,6 +240,10 @@ def main():
               "the multi-GPU sharding is a design, not a measured path", file=sys.stderr)
         sys.exit(2)
 
+    # the CPU baseline's OpenMP threads stay on neighbouring cores; read when the
+    # oracle's OpenMP runtime starts, so set before anything loads it
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     import numpy as np
     import torch
 
@@ -244,13 +298,19 @@ def main():
     y64 = torch.empty_like(x64)
     for _ in range(max(1, args.warmup)):
         H.apply(y64, x64)
+        H.apply(y64, x64, part="local")
     ms64 = _time_applies(torch, lambda: H.apply(y64, x64), max(5, args.steps // 5))
+    ms64_local = _time_applies(torch, lambda: H.apply(y64, x64, part="local"), max(5, args.steps // 5))
+    ms64_nl = _time_applies(torch, lambda: H.apply(y64, x64, part="nonlocal"), max(5, args.steps // 5))
+    cand64 = H.stencil_choice(64, nb)
     del x64, y64
 
     cand = H.stencil_choice(32, nb)
     kernel_name = H.stencil_candidates()[cand] if cand >= 0 else "untuned"
     bytes_local = local_bytes(s, nb, 4)
     achieved = bytes_local / (ms_local * 1e-3) / 1e9
+    bytes64 = local_bytes(s, nb, 8)
+    nl_bytes = nonlocal_bytes(s, nb, 4)
     out = {
         "metric": METRIC,
         "value": value,
@@ -275,7 +335,17 @@ def main():
         "parts_ms": {"local_fd8_v": ms_local, "nonlocal": ms_nl, "full": ms},
         "stencil_shape_in_full_apply": (H.stencil_candidates()[H.stencil_choice(32, nb, full=True)]
                                         if H.stencil_choice(32, nb, full=True) >= 0 else "untuned"),
-        "fp64_hpsi": {"ms_per_apply": ms64, "value": s.ng * nb / (ms64 * 1e-3)},
+        "nonlocal_roofline": {"bound": "hbm", "achieved": nl_bytes / (ms_nl * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                              "unit": "GB/s", "frac": nl_bytes / (ms_nl * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                              "bytes_per_apply": nl_bytes, "ms_per_apply": ms_nl,
+                              "kernels": "nl_project_kernel + nl_coef_kernel + nl_scatter_kernel"},
+        "fp64_hpsi": {"ms_per_apply": ms64, "value": s.ng * nb / (ms64 * 1e-3),
+                      "stencil_roofline": {"bound": "hbm", "kernel": H.stencil_candidates()[cand64] if cand64 >= 0
+                                           else "untuned", "achieved": bytes64 / (ms64_local * 1e-3) / 1e9,
+                                           "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                           "frac": bytes64 / (ms64_local * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                           "bytes_per_launch": bytes64, "ms_per_launch": ms64_local},
+                      "nonlocal_ms": ms64_nl},
     }
     del x, y
     if args.lobpcg_calls > 0:

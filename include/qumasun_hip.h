@@ -52,7 +52,8 @@ int qm_set_vloc(qm_ctx *ctx, const double *vloc);
 
 /* KB projector set (host arrays), replacing the tables that m_pp_SvF holds for
  * ProjectionPP_bundle (PseudoPotNonlocal_SvF2_kpoint.h:4239): per atom a,
- *   nproj[a] projectors (<= 16), voxels vox_off[a]..vox_off[a+1]-1 with UNWRAPPED
+ *   nproj[a] projectors (<= 32: two radial projectors for each l <= 3, as the
+ *   reference's W.upf needs, num_proj_RY of ..._kpoint.h:940-945), voxels vox_off[a]..vox_off[a+1]-1 with UNWRAPPED
  *   integer grid coordinates vox_xyz[3v..3v+2] (the grid index is their periodic
  *   wrap; the Bloch phase uses the unwrapped value, ..._kpoint.h:3578-3596),
  *   table[tab_off[a] + w*V_a + v] (column-major V_a x M_a, ..._kpoint.h:2255-2272),
@@ -60,16 +61,53 @@ int qm_set_vloc(qm_ctx *ctx, const double *vloc);
  * dtable (may be NULL) holds d p/d r_x, d p/d r_y, d p/d r_z in the same layout,
  * one block of tab_off[natoms] values per direction (FORCE_DIFF_PROJ tables,
  * ..._kpoint.h:2255-2267); it is required by qm_nonlocal_forces_*.
- * Overlapping spheres are allowed. */
+ * Overlapping spheres are allowed.  Every array is validated and the new set is
+ * built before the old one is released: on any error the previous set (and its
+ * k-point phases) stays in place. */
 int qm_set_projectors(qm_ctx *ctx, int natoms, const int *nproj, const int64_t *vox_off,
                       const int *vox_xyz, const double *table, const double *dtable,
                       const int64_t *tab_off, const double *energy, const int *proj_off);
 
 /* k-points in 1/bohr (kvec[3*ik..3*ik+2]); precomputes e^{ik.r} on every projector
- * voxel (mCreateBlochExp, ..._kpoint.h:3537-3618).  k-point 0 defaults to Gamma. */
+ * voxel (mCreateBlochExp, ..._kpoint.h:3537-3618).  k-point 0 defaults to Gamma.
+ * On error the previous k-points stay in place. */
 int qm_set_kpoints(qm_ctx *ctx, int nk, const double *kvec);
 
+/* ---------------------------------------------------------------- kinetic term */
+
+#define QM_KINETIC_FD8 0  /* 8th-order finite differences (the default) */
+#define QM_KINETIC_FFT 1  /* spectral, through 3-D FFTs */
+
+/* Kinetic term of every later apply on the context (qm_hpsi_apply_*,
+ * qm_local_apply_*, the LOBPCG solver, the Taylor propagator):
+ * QM_KINETIC_FD8   -1/2 (Delta_FD8 + 2 i k.grad_FD8 - k^2), mKineticMatrixAdd_ddm_kpint
+ *                  (qumasun_base1.matrix.h:298-317, GridDifferenceKpoint8th.h:17-151);
+ * QM_KINETIC_FFT   IFFT(|G + k|^2 / 2 FFT psi) / N, the reference's default
+ *                  (m_algorithm_Laplacian = 1, qumasun_input.h:117) in
+ *                  mHamiltonianMatrix_ddm_bundle (qumasun_base1.matrix.h:171-228)
+ *                  with OperateKineticInKspace_bundle (96-130): G = 2 pi / L_d times the
+ *                  wrapped index i (i > n/2 -> i - n; the Nyquist index n/2 stays +n/2).
+ * Spectral applies hold the band block's spectrum and rocFFT's work buffer on
+ * the context (grown on demand).  The stencil autotuner only concerns
+ * QM_KINETIC_FD8. */
+int qm_set_kinetic(qm_ctx *ctx, int mode);
+
 /* ---------------------------------------------------------------- H.psi */
+
+/* Stream capture.  An apply (qm_hpsi_apply_*, qm_local_apply_*, qm_nonlocal_add_*)
+ * may be recorded into a caller's HIP graph.  The graph then holds the
+ * addresses of this context's projector tables, phases, V_loc and its
+ * nonlocal work buffers.  From the
+ * capture on, every call that would free or move them returns QM_ERR_STATE:
+ * qm_set_projectors, qm_set_kpoints, and an apply that needs larger work
+ * buffers (more bands, or FP64 after FP32).  qm_set_vloc overwrites V_loc in
+ * place, so replays see the new potential.  Capture an apply after an eager
+ * apply of the same shape (the work buffers and the stencil shape are then
+ * settled).  No counterpart in the reference, which launches every kernel from
+ * the host.
+ * qm_release_captures: the caller has destroyed every graph holding an apply of
+ * this context (or will not replay them again); the calls above work again. */
+int qm_release_captures(qm_ctx *ctx);
 
 /* d_hpsi = H(k_ik) d_psi on nb bands, overwriting d_hpsi:
  *   H = -1/2 (Delta_FD8 + 2 i k.grad_FD8 - k^2) + V_loc + sum_a |p> E dV <p|
@@ -102,7 +140,8 @@ int qm_nonlocal_add_f32(qm_ctx *ctx, float *d_out, const float *d_psi, int nb, i
  * whose CPU loops have one shape).  The first qm_hpsi_apply_* / qm_local_apply_*
  * of a (precision, ncomp, k type, nb) shape on a context times every candidate
  * on the caller's buffers and keeps the fastest; it synchronises the stream
- * once.  Every candidate computes each point with the same arithmetic, so the
+ * once.  Band counts in one power-of-two bucket (33..64, 65..128, ...) share
+ * one choice.  Every candidate computes each point with the same arithmetic, so the
  * choice never changes a result.  Under stream capture an untuned shape uses
  * candidate 0.
  * qm_stencil_num_candidates  number of candidates;
@@ -118,6 +157,36 @@ int qm_stencil_num_candidates(void);
 int qm_stencil_candidate_name(int candidate, char *buf, int buflen);
 int qm_stencil_force(qm_ctx *ctx, int candidate);
 int qm_stencil_choice(qm_ctx *ctx, int precision, int ncomp, int kt, int nb, int *candidate);
+
+/* ---------------------------------------------------------------- real-time propagation */
+
+#define QM_TD_OP_FULL 3      /* H(k) = kinetic + V_loc + nonlocal */
+#define QM_TD_OP_LOCAL 1     /* kinetic + V_loc */
+#define QM_TD_OP_NONLOCAL 2  /* the nonlocal term alone */
+
+/* Taylor propagator of nb SoA complex bands (ncomp = 2) in place:
+ *   d_psi <- sum_{n=0..order} (-i dt Op)^n / n! d_psi,  Op as selected by `op`.
+ * order 4 with QM_TD_OP_FULL is QUMASUN_TD::mTimeEvolutionH4 over
+ * QUMASUN::TimeEvoH4_bundle (qumasun_td.time_evo_H4.h:70-204); order 4 with
+ * QM_TD_OP_NONLOCAL is TimeEvoA4_bundle as TestEvolveNL4 calls it
+ * (qumasun_symplectic.time_evo_K4.h:13-96, 305-335).  order <= 32.  Two work
+ * blocks of the band block's size are held by the context (grown on demand).
+ * Asynchronous on `stream`. */
+int qm_td_taylor_f64(qm_ctx *ctx, double *d_psi, int nb, int ik, double dt, int order, int op, void *stream);
+int qm_td_taylor_f32(qm_ctx *ctx, float *d_psi, int nb, int ik, double dt, int order, int op, void *stream);
+
+/* Kinetic step of the split-operator propagator on nb SoA complex bands in place:
+ *   d_psi <- IFFT(e^{-i dt |G + k|^2 / 2} FFT d_psi) / N
+ * (EvolveKineticInKspace / mEvolveK_FFT, qumasun_symplectic.kinetic_fft.h:14-60, 65-140,
+ * which take k = 0; k is the context's k-point ik here).  Asynchronous on `stream`. */
+int qm_td_kinetic_fft_f64(qm_ctx *ctx, double *d_psi, int nb, int ik, double dt, void *stream);
+int qm_td_kinetic_fft_f32(qm_ctx *ctx, float *d_psi, int nb, int ik, double dt, void *stream);
+
+/* d_psi <- e^{-i V_loc dt} d_psi pointwise on nb SoA complex bands, the phase
+ * evaluated in FP64 (mProductExpToV with exp_v = cos(-dt V) + i sin(-dt V),
+ * qumasun_symplectic.evolution.h:80-100, 140-165).  Asynchronous on `stream`. */
+int qm_td_expv_f64(qm_ctx *ctx, double *d_psi, int nb, double dt, void *stream);
+int qm_td_expv_f32(qm_ctx *ctx, float *d_psi, int nb, double dt, void *stream);
 
 /* ---------------------------------------------------------------- adjoint */
 
@@ -226,8 +295,18 @@ int qm_lobpcg_solve_locking(qm_lobpcg *s, void *d_x, double *eigenvalues, double
  * near-null directions dropped.  No counterpart in the reference, whose
  * LAPACK info is only printed (gy_w_zhegvd_hip.h:58-61). */
 int qm_lobpcg_set_conditioning(qm_lobpcg *s, double min_pivot_ratio_fp64, double min_pivot_ratio_fp32);
-/* Number of such fallbacks taken in the last qm_lobpcg_solve (or QM_ERR_ARG). */
+/* Number of such fallbacks taken in the last qm_lobpcg_solve (or QM_ERR_ARG).
+ * A ratio above 1 (at most 2) fails every pivot test, so every step takes the
+ * S-eigenbasis path: a way to exercise the fallback. */
 int qm_lobpcg_fallbacks(qm_lobpcg *s);
+
+/* The |1 - |x'|^2| limit above which an update is rolled back: P = R, the
+ * previous vectors are returned (renormalised) with their Ritz values and
+ * residual norms, and the call leaves its loop (LIMIT_Z_NORM_X_FOR_RESET_MATRIX,
+ * lobpcg_z_multi_mpi.h:101, 2611-2666; default 1e-3).  For precision 32 the
+ * returned values are FP64 Rayleigh quotients of the returned vectors in either
+ * case.  limit = 0 rolls back the first update of a call. */
+int qm_lobpcg_set_rollback_limit(qm_lobpcg *s, double limit);
 
 /* HIP graphs for the iteration segments that hold no host decision (no
  * reference counterpart: the reference launches every kernel from the host).

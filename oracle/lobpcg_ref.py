@@ -152,6 +152,7 @@ class LOBPCGRef:
         self.cond = cond    # minimum squared Cholesky pivot ratio of S
         self.rdrop = rdrop  # R directions kept above this pivot / eigenvalue ratio
         self.residual_threshold = residual_threshold  # None: no locking
+        self.limit_norm_x = LIMIT_NORM_X
         self.num_fix = 0    # fixed lowest bands, kept between calls
         self.P = None       # all N bands (rows below num_fix unused)
 
@@ -274,7 +275,7 @@ class LOBPCGRef:
             Apn = Cr.T @ Ar + (Cp.T @ Ap if have_p else 0)
             Axn = Cx.T @ Ax + Apn
             norms = np.sum(np.abs(xn) ** 2, axis=1)
-            if np.max(np.abs(1.0 - norms)) > LIMIT_NORM_X:
+            if np.max(np.abs(1.0 - norms)) > self.limit_norm_x:
                 # rollback and escape (2611-2666): P = R and leave the loop.  The
                 # reference rescales the old x by the norms of the rejected update
                 # (Mul_invSqV_bundle with sum_norms, 2639), which leaves x

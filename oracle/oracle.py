@@ -39,7 +39,9 @@ def lib():
         L.oracle_ylm.restype = D
         L.oracle_kb_projector.argtypes = [I, I, D, D, D, D, D]
         L.oracle_kb_projector.restype = D
-        L.oracle_projector_table.argtypes = [I, P, P, P, D, D, D, D, P, P, P, P]
+        L.oracle_kb_projector_radial.argtypes = [I, I, I, D, D, D, D, D]
+        L.oracle_kb_projector_radial.restype = D
+        L.oracle_projector_table.argtypes = [I, P, P, P, I, D, D, D, D, P, P, P, P]
         L.oracle_band_density.argtypes = [ctypes.c_int64, I, I, P, P, P]
         L.oracle_nonlocal_forces.argtypes = [I, I, I, D, D, D, I, I, P, P, D, D, D, I, P, P, P, P, P,
                                              ctypes.c_int64, P, P, P, P]
@@ -114,8 +116,8 @@ def ylm(l, m, x, y, z):
     return lib().oracle_ylm(l, m, x, y, z)
 
 
-def kb_projector(l, m, sigma, rc, x, y, z):
-    return lib().oracle_kb_projector(l, m, sigma, rc, x, y, z)
+def kb_projector(l, m, sigma, rc, x, y, z, irad=0):
+    return lib().oracle_kb_projector_radial(l, m, irad, sigma, rc, x, y, z)
 
 
 def projector_table(s, positions=None):
@@ -127,7 +129,8 @@ def projector_table(s, positions=None):
     out = np.zeros(int(s.tab_off[-1]))
     sigma = 0.7  # KB radial width of the synthetic projectors
     hx, hy, hz = s.h
-    lib().oracle_projector_table(s.natoms, _p(pos), _p(lmax), _p(rc), sigma, hx, hy, hz,
+    lib().oracle_projector_table(s.natoms, _p(pos), _p(lmax), _p(rc), int(getattr(s, "nradial", 1)), sigma,
+                                 hx, hy, hz,
                                  _p(vo), _p(vx), _p(to), _p(out))
     return out
 
@@ -158,3 +161,105 @@ def nonlocal_forces(s, psi, occ, k=(0.0, 0.0, 0.0), ncomp=None, dtable=None):
                                      nproj, vox_off, vox_xyz, table, _p(dt), int(s.tab_off[-1]),
                                      tab_off, energy, proj_off, _p(forces))
     return e, forces
+
+
+def td_taylor(s, psi, dt, order=4, op="full", k=(0.0, 0.0, 0.0), kinetic="fd8"):
+    """Taylor propagator psi <- sum_{n=0..order} (-i dt Op)^n / n! psi of SoA
+    complex bands (QUMASUN::TimeEvoH4_bundle, qumasun_td.time_evo_H4.h:70-160, at
+    order 4; TimeEvoA4_bundle, qumasun_symplectic.time_evo_K4.h:13-96, with
+    Op = the nonlocal term alone).  op: "full", "local" (kinetic + V_loc) or
+    "nonlocal".  As in the reference, each term H^n psi is made from H^(n-1) psi
+    and added to psi at once."""
+    assert s.ncomp == 2
+    x = np.array(psi, dtype=np.float64)
+    z = x[0::2] + 1j * x[1::2]
+
+    def apply(v):
+        soa = np.empty((2 * v.shape[0], v.shape[1]))
+        soa[0::2], soa[1::2] = v.real, v.imag
+        if op == "full":
+            out = hpsi(s, soa, k=k) if kinetic == "fd8" else hpsi_fft(s, soa, k=k)
+        elif op == "local":
+            out = hpsi(s, soa, k=k, nonlocal_=False) if kinetic == "fd8" else hpsi_fft(s, soa, k=k, nonlocal_=False)
+        else:
+            out = nonlocal_(s, soa, k=k)
+        return out[0::2] + 1j * out[1::2]
+
+    term = z.copy()
+    coef = 1.0
+    for n in range(1, order + 1):
+        term = apply(term)
+        coef *= dt / n
+        z = z + ((-1j) ** n) * coef * term
+    out = np.empty_like(x)
+    out[0::2], out[1::2] = z.real, z.imag
+    return out
+
+
+def td_expv(s, psi, dt):
+    """psi <- e^{-i V_loc dt} psi pointwise (mProductExpToV with
+    exp_v = cos(-dt V) + i sin(-dt V), qumasun_symplectic.evolution.h:80-100, 140-165)."""
+    x = np.array(psi, dtype=np.float64)
+    c, sn = np.cos(-dt * s.vloc), np.sin(-dt * s.vloc)
+    re, im = x[0::2].copy(), x[1::2].copy()
+    x[0::2] = c * re - sn * im
+    x[1::2] = c * im + sn * re
+    return x
+
+
+def _kspace_e2(s, k=(0.0, 0.0, 0.0)):
+    """|G + k|^2 / 2 on the (nz, ny, nx) FFT grid, G = 2 pi / L_d times the wrapped
+    index (i > n/2 -> i - n; the Nyquist index n/2 stays +n/2), as
+    OperateKineticInKspace_bundle (qumasun_base1.matrix.h:96-130)."""
+    hx, hy, hz = s.h
+
+    def axis(n, h, kk):
+        i = np.arange(n)
+        return 2.0 * np.pi / (h * n) * np.where(i * 2 > n, i - n, i) + kk
+
+    qx, qy, qz = axis(s.nx, hx, k[0]), axis(s.ny, hy, k[1]), axis(s.nz, hz, k[2])
+    return 0.5 * (qz[:, None, None] ** 2 + qy[None, :, None] ** 2 + qx[None, None, :] ** 2)
+
+
+def fft_kinetic(s, psi, k=(0.0, 0.0, 0.0), ncomp=None):
+    """Spectral kinetic term IFFT(|G + k|^2 / 2 FFT psi) of a SoA band block, FP64
+    (numpy FFT; the reference's mHamiltonianMatrix_ddm_bundle FFT branch,
+    qumasun_base1.matrix.h:171-228)."""
+    ncomp = s.ncomp if ncomp is None else ncomp
+    x = np.asarray(psi, dtype=np.float64)
+    z = x if ncomp == 1 else x[0::2] + 1j * x[1::2]
+    shp = (z.shape[0], s.nz, s.ny, s.nx)
+    e2 = _kspace_e2(s, k)
+    t = np.fft.ifftn(e2[None] * np.fft.fftn(z.reshape(shp), axes=(1, 2, 3)), axes=(1, 2, 3)).reshape(z.shape)
+    if ncomp == 1:
+        return np.ascontiguousarray(t.real)
+    out = np.empty_like(x)
+    out[0::2], out[1::2] = t.real, t.imag
+    return out
+
+
+def hpsi_fft(s, psi, k=(0.0, 0.0, 0.0), ncomp=None, nonlocal_=True):
+    """H.psi with the spectral kinetic term: FFT kinetic + V_loc psi + nonlocal."""
+    ncomp = s.ncomp if ncomp is None else ncomp
+    x = np.asarray(psi, dtype=np.float64)
+    out = fft_kinetic(s, x, k, ncomp) + s.vloc[None, :] * x
+    if nonlocal_ and s.natoms > 0:
+        out += nonlocal_add(s, x, k, ncomp)
+    return out
+
+
+def nonlocal_add(s, psi, k=(0.0, 0.0, 0.0), ncomp=None):
+    return nonlocal_(s, psi, k=k, ncomp=ncomp)
+
+
+def td_kinetic_fft(s, psi, dt, k=(0.0, 0.0, 0.0)):
+    """psi <- IFFT(e^{-i dt |G + k|^2 / 2} FFT psi) of SoA complex bands
+    (EvolveKineticInKspace / mEvolveK_FFT, qumasun_symplectic.kinetic_fft.h:14-60, 65-140)."""
+    x = np.asarray(psi, dtype=np.float64)
+    z = x[0::2] + 1j * x[1::2]
+    shp = (z.shape[0], s.nz, s.ny, s.nx)
+    ph = np.exp(-1j * dt * _kspace_e2(s, k))
+    t = np.fft.ifftn(ph[None] * np.fft.fftn(z.reshape(shp), axes=(1, 2, 3)), axes=(1, 2, 3)).reshape(z.shape)
+    out = np.empty_like(x)
+    out[0::2], out[1::2] = t.real, t.imag
+    return out

@@ -16,7 +16,8 @@
  *   band block, ncomp = 1 (real Gamma point): band n at psi + n*Ng.
  *
  * Projector set (per atom a):
- *   nproj[a]            M_a projectors (l<=2 -> up to 9, ordered l, m=-l..l)
+ *   nproj[a]            M_a projectors (up to 64; ordered by radial projector k, then
+ *                       m = -l_k..l_k, as num_proj_RY in ..._kpoint.h:940-945, 960-966)
  *   vox_off[a]..[a+1]   range of this atom's voxels in the global voxel list
  *   vox_xyz[3v..3v+2]   UNWRAPPED integer grid coordinates of voxel v; the
  *                       grid index is the periodic wrap, the Bloch phase uses
@@ -25,7 +26,8 @@
  *                       (column-major V_a x M_a, ..._kpoint.h:2255-2272)
  *   energy[proj_off[a] + w]        E_w (Ha), the KB energy of projector w
  *
- * Parity status of each function is documented in oracle/README.md.
+ * What pins each function (closed forms, dense matrices, scipy, finite
+ * differences) is tested in tests/test_oracle.py; every function has a pin.
  */
 #include <math.h>
 #include <stdint.h>
@@ -184,7 +186,8 @@ void oracle_nonlocal_add(int nx, int ny, int nz, double hx, double hy, double hz
                     cim[v] = i;
                 }
             }
-            double cr[16], ci[16];
+            double cr[64], ci[64];  /* M <= 64 (the library takes at most 32) */
+            if (M > 64) abort();
             for (int w = 0; w < M; ++w) {
                 double sr = 0.0, si = 0.0;
                 for (int64_t v = 0; v < V; ++v) {
@@ -266,29 +269,45 @@ double oracle_ylm(int l, int m, double x, double y, double z) {
     else if (l == 2 && am == 0) P = 0.5 * (3.0 * uz * uz - 1.0);
     else if (l == 2 && am == 1) P = 3.0 * uz;
     else if (l == 2 && am == 2) P = 3.0;
+    else if (l == 3 && am == 0) P = 0.5 * (5.0 * uz * uz * uz - 3.0 * uz);
+    else if (l == 3 && am == 1) P = 1.5 * (5.0 * uz * uz - 1.0);
+    else if (l == 3 && am == 2) P = 15.0 * uz;
+    else if (l == 3 && am == 3) P = 15.0;
+    /* T_m = Re (ux + i uy)^m for m > 0, Im (ux + i uy)^|m| for m < 0 */
     double T = 1.0;
     if (m == 1) T = ux;
     else if (m == -1) T = uy;
     else if (m == 2) T = ux * ux - uy * uy;
     else if (m == -2) T = 2.0 * ux * uy;
+    else if (m == 3) T = ux * ux * ux - 3.0 * ux * uy * uy;
+    else if (m == -3) T = 3.0 * ux * ux * uy - uy * uy * uy;
     return c * P * T;
 }
 
-/* p_lm(r) = r^l exp(-r^2 / (2 sigma^2)) Y_lm(r_hat) for r < rc, else 0. */
-double oracle_kb_projector(int l, int m, double sigma, double rc, double x, double y, double z) {
+/* Radial projector irad of angular momentum l:
+ *   irad 0: p_lm(r) = r^l exp(-r^2 / (2 sigma^2)) Y_lm(r_hat)
+ *   irad 1: p_lm(r) = r^l (r^2 / sigma^2) exp(-r^2 / (2 sigma^2)) Y_lm(r_hat)
+ * for r < rc, else 0. */
+double oracle_kb_projector_radial(int l, int m, int irad, double sigma, double rc, double x, double y, double z) {
     const double rr = x * x + y * y + z * z;
     if (rr >= rc * rc) return 0.0;
     const double r = sqrt(rr);
     double rl = 1.0;
     for (int i = 0; i < l; ++i) rl *= r;
-    return rl * exp(-rr / (2.0 * sigma * sigma)) * oracle_ylm(l, m, x, y, z);
+    const double u = irad == 1 ? rr / (sigma * sigma) : 1.0;
+    return rl * u * exp(-rr / (2.0 * sigma * sigma)) * oracle_ylm(l, m, x, y, z);
+}
+
+double oracle_kb_projector(int l, int m, double sigma, double rc, double x, double y, double z) {
+    return oracle_kb_projector_radial(l, m, 0, sigma, rc, x, y, z);
 }
 
 
 /* Projector table of one atom set: table[tab_off[a] + w*V_a + v] = p_{l(w) m(w)}(r_v - R_a)
- * with w enumerating l = 0..lmax[a], m = -l..l (SetBlockYlm.h:666-740 ordering) and
- * r_v = (ix*hx, iy*hy, iz*hz) on the unwrapped voxel coordinates. */
-void oracle_projector_table(int natoms, const double *pos, const int *lmax, const double *rc,
+ * with w enumerating l = 0..lmax[a], then the nradial radial projectors of that l, then
+ * m = -l..l (the reference's order: radial projector k, then m, ..._kpoint.h:960-966,
+ * SetBlockYlm.h:666-740) and r_v = (ix*hx, iy*hy, iz*hz) on the unwrapped voxel coordinates. */
+void oracle_projector_table(int natoms, const double *pos, const int *lmax, const double *rc, int nradial,
                             double sigma, double hx, double hy, double hz,
                             const int64_t *vox_off, const int *vox_xyz, const int64_t *tab_off,
                             double *table) {
@@ -296,16 +315,16 @@ void oracle_projector_table(int natoms, const double *pos, const int *lmax, cons
     for (int a = 0; a < natoms; ++a) {
         const int64_t v0 = vox_off[a], V = vox_off[a + 1] - vox_off[a];
         int w = 0;
-        for (int l = 0; l <= lmax[a]; ++l) {
-            for (int m = -l; m <= l; ++m, ++w) {
-                for (int64_t v = 0; v < V; ++v) {
-                    const double x = hx * vox_xyz[3 * (v0 + v)] - pos[3 * a];
-                    const double y = hy * vox_xyz[3 * (v0 + v) + 1] - pos[3 * a + 1];
-                    const double z = hz * vox_xyz[3 * (v0 + v) + 2] - pos[3 * a + 2];
-                    table[tab_off[a] + w * V + v] = oracle_kb_projector(l, m, sigma, rc[a], x, y, z);
+        for (int l = 0; l <= lmax[a]; ++l)
+            for (int k = 0; k < nradial; ++k)
+                for (int m = -l; m <= l; ++m, ++w) {
+                    for (int64_t v = 0; v < V; ++v) {
+                        const double x = hx * vox_xyz[3 * (v0 + v)] - pos[3 * a];
+                        const double y = hy * vox_xyz[3 * (v0 + v) + 1] - pos[3 * a + 1];
+                        const double z = hz * vox_xyz[3 * (v0 + v) + 2] - pos[3 * a + 2];
+                        table[tab_off[a] + w * V + v] = oracle_kb_projector_radial(l, m, k, sigma, rc[a], x, y, z);
+                    }
                 }
-            }
-        }
     }
 }
 

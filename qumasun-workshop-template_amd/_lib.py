@@ -21,7 +21,9 @@ EXPORTS = (
     "qm_lobpcg_solve", "qm_lobpcg_solve_locking", "qm_lobpcg_set_conditioning", "qm_lobpcg_fallbacks", "qm_lobpcg_timings",
     "qm_lobpcg_set_graphs", "qm_lobpcg_graph_replays",
     "qm_stencil_num_candidates", "qm_stencil_candidate_name", "qm_stencil_force", "qm_stencil_choice",
-    "qm_sygvd_f64",
+    "qm_sygvd_f64", "qm_release_captures", "qm_lobpcg_set_rollback_limit",
+    "qm_td_taylor_f64", "qm_td_taylor_f32", "qm_td_expv_f64", "qm_td_expv_f32",
+    "qm_set_kinetic", "qm_td_kinetic_fft_f64", "qm_td_kinetic_fft_f32",
 )
 
 QM_OK = 0
@@ -55,8 +57,8 @@ def load():
     L.qm_set_vloc.argtypes = [P, P]
     L.qm_set_projectors.argtypes = [P, I, P, P, P, P, P, P, P, P]
     L.qm_set_kpoints.argtypes = [P, I, P]
-    for n in ("qm_hpsi_apply_f64", "qm_hpsi_apply_f32", "qm_local_apply_f64", "qm_local_apply_f32",
-              "qm_nonlocal_add_f64", "qm_nonlocal_add_f32"):
+    for n in ("qm_hpsi_apply_f64", "qm_hpsi_apply_f32", "qm_local_apply_f64",
+              "qm_local_apply_f32", "qm_nonlocal_add_f64", "qm_nonlocal_add_f32"):
         getattr(L, n).argtypes = [P, P, P, I, I, I, P]
     for n in ("qm_band_density_f64", "qm_band_density_f32"):
         getattr(L, n).argtypes = [P, P, P, P, I, I, P]
@@ -72,6 +74,14 @@ def load():
     L.qm_lobpcg_timings.argtypes = [P, P]
     L.qm_lobpcg_set_conditioning.argtypes = [P, D, D]
     L.qm_lobpcg_fallbacks.argtypes = [P]
+    L.qm_lobpcg_set_rollback_limit.argtypes = [P, D]
+    for n in ("qm_td_taylor_f64", "qm_td_taylor_f32"):
+        getattr(L, n).argtypes = [P, P, I, I, D, I, I, P]
+    for n in ("qm_td_expv_f64", "qm_td_expv_f32"):
+        getattr(L, n).argtypes = [P, P, I, D, P]
+    L.qm_set_kinetic.argtypes = [P, I]
+    for n in ("qm_td_kinetic_fft_f64", "qm_td_kinetic_fft_f32"):
+        getattr(L, n).argtypes = [P, P, I, I, D, P]
     L.qm_lobpcg_set_graphs.argtypes = [P, I]
     L.qm_lobpcg_graph_replays.argtypes = [P]
     L.qm_stencil_num_candidates.argtypes = []
@@ -79,6 +89,7 @@ def load():
     L.qm_stencil_force.argtypes = [P, I]
     L.qm_stencil_choice.argtypes = [P, I, I, I, I, ctypes.POINTER(I)]
     L.qm_sygvd_f64.argtypes = [P, I, P, P, P, P, P, I, P]
+    L.qm_release_captures.argtypes = [P]
     for n in EXPORTS:
         if n not in ("qm_last_error", "qm_destroy", "qm_lobpcg_destroy"):
             getattr(L, n).restype = I

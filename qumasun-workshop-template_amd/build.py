@@ -3,8 +3,10 @@
     python qumasun-workshop-template_amd/build.py [--jobs N] [--force]
 
 Each csrc/*.hip file is compiled to an object under _build/ (rebuilt when it or
-a header is newer), then linked with rocBLAS / rocSOLVER into
-qumasun-workshop-template_amd/libqumasun_hip.so.
+a header is newer), then linked with rocBLAS / rocSOLVER / rocFFT into
+qumasun-workshop-template_amd/libqumasun_hip.so.  The C++ consumer test
+(tests/consumer/opea_consumer.cpp, a QUMASUN-style caller of the C ABI) is
+built and linked against it afterwards.
 """
 from __future__ import annotations
 
@@ -30,8 +32,10 @@ CXXFLAGS = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-but-set-variable",
     f"-I{ROOT}/include",
 ]
-LDFLAGS = ["--offload-arch=gfx950", "-shared", f"-L{ROCM}/lib", "-lrocsolver", "-lrocblas",
+LDFLAGS = ["--offload-arch=gfx950", "-shared", f"-L{ROCM}/lib", "-lrocsolver", "-lrocblas", "-lrocfft",
            f"-Wl,-rpath,{ROCM}/lib"]
+CONSUMER_SRC = os.path.join(ROOT, "tests", "consumer", "opea_consumer.cpp")
+CONSUMER_BIN = os.path.join(ROOT, "tests", "consumer", "opea_consumer")
 
 
 def _headers():
@@ -67,7 +71,23 @@ def build(jobs: int = 8, force: bool = False, verbose: bool = True) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_consumer(verbose)
     return LIB
+
+
+def build_consumer(verbose: bool = True) -> str:
+    """The C++ consumer test program, linked against the library by a relative rpath."""
+    rel = os.path.relpath(PKG, os.path.dirname(CONSUMER_BIN))
+    cmd = [HIPCC, "--offload-arch=gfx950", "-O2", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
+           "-packed-fp32-ops", f"-I{ROOT}/include", CONSUMER_SRC, f"-L{PKG}", "-lqumasun_hip",
+           f"-Wl,-rpath,$ORIGIN/{rel}", "-o", CONSUMER_BIN + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"consumer build failed:\n$ {' '.join(cmd)}\n{r.stdout}{r.stderr}")
+    os.replace(CONSUMER_BIN + ".tmp", CONSUMER_BIN)
+    if verbose:
+        print(f"built {CONSUMER_BIN}")
+    return CONSUMER_BIN
 
 
 if __name__ == "__main__":

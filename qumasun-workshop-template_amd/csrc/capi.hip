@@ -43,6 +43,23 @@ int activate(const qm_ctx *ctx) {
     return QM_OK;
 }
 
+thread_local int g_internal_capture = 0;
+
+int check_unpinned(const qm_ctx *ctx, const char *who) {
+    QM_CHECK(ctx->capture_pins == 0, QM_ERR_STATE,
+             std::string(who) + ": a graph captured an H.psi apply of this context and holds the addresses of its "
+             "tables and work buffers; destroy that graph and call qm_release_captures first");
+    return QM_OK;
+}
+
+// Whether st is capturing a graph for the caller (not one of the library's own).
+static int caller_capturing(hipStream_t st, bool *out) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    QM_HIP(hipStreamIsCapturing(st, &cs));
+    *out = cs != hipStreamCaptureStatusNone && g_internal_capture == 0;
+    return QM_OK;
+}
+
 template <typename T>
 static int upload(T **dst, const T *src, size_t n) {
     if (*dst) {
@@ -82,8 +99,8 @@ static void free_projectors(Projectors &p) {
     dfree(p.d_cov_off);
     dfree(p.d_cov_vox);
     dfree(p.d_atom_order);
-    dfree(p.d_tab16_64);
-    dfree(p.d_tab16_32);
+    dfree(p.d_tabp_64);
+    dfree(p.d_tabp_32);
     dfree(p.d_chunk_atom);
     dfree(p.d_chunk_v0);
     dfree(p.d_atom_chunk_off);
@@ -109,10 +126,9 @@ static inline int wrap_host(int i, int n) {
 
 // e^{ik.r} on every voxel for one k-point (mCreateBlochExp, ..._kpoint.h:3537-3618):
 // phase = k_x h_x ix + k_y h_y iy + k_z h_z iz on the unwrapped coordinates.
-static int build_phase(qm_ctx *ctx, KPoint &kp) {
+static int build_phase(const qm_ctx *ctx, const Projectors &p, KPoint &kp) {
     dfree(kp.d_phase64);
     dfree(kp.d_phase32);
-    const Projectors &p = ctx->proj;
     if (kp.gamma || p.vtot == 0) return QM_OK;
     std::vector<double> ph(2 * p.vtot);
     std::vector<float> ph32(2 * p.vtot);
@@ -149,6 +165,145 @@ int side_stream(qm_ctx *ctx) {
     return QM_OK;
 }
 
+// Builds the device projector set p from validated host arrays: voxel maps,
+// tables (column-major and voxel-major padded to p.mpad), the projection work
+// list and the owner-computes scatter map.  On failure p holds whatever was
+// uploaded so far (the caller frees it).
+static int build_projectors(const qm_ctx *ctx, Projectors &p, int natoms, const int *nproj, const int64_t *vox_off,
+                            const int *vox_xyz, const double *table, const double *dtable, const int64_t *tab_off,
+                            const double *energy, const int *proj_off) {
+    p.natoms = natoms;
+    p.nproj.assign(nproj, nproj + natoms);
+    p.vox_off.assign(vox_off, vox_off + natoms + 1);
+    p.tab_off.assign(tab_off, tab_off + natoms + 1);
+    p.proj_off.assign(proj_off, proj_off + natoms + 1);
+    for (int a = 0; a < natoms; ++a) p.max_m = std::max(p.max_m, nproj[a]);
+    p.mpad = p.max_m > 16 ? 32 : 16;
+    p.vtot = vox_off[natoms];
+    p.ttot = tab_off[natoms];
+    p.ptot = proj_off[natoms];
+    p.vox_xyz.assign(vox_xyz, vox_xyz + 3 * p.vtot);
+    p.energy.assign(energy, energy + p.ptot);
+    p.has_grad = dtable != nullptr;
+
+    // wrapped grid index and owner atom of every voxel
+    std::vector<int> grid(p.vtot), atom(p.vtot);
+    for (int a = 0; a < natoms; ++a)
+        for (int64_t v = vox_off[a]; v < vox_off[a + 1]; ++v) {
+            grid[v] = wrap_host(vox_xyz[3 * v], ctx->nx) +
+                      ctx->nx * (wrap_host(vox_xyz[3 * v + 1], ctx->ny) +
+                                 ctx->ny * wrap_host(vox_xyz[3 * v + 2], ctx->nz));
+            atom[v] = a;
+        }
+    // covered-point map for the owner-computes scatter: voxels sorted by grid point
+    std::vector<int> order(p.vtot);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return grid[x] < grid[y]; });
+    std::vector<int> cov_grid, cov_off;
+    cov_off.reserve(p.vtot + 1);
+    for (int64_t i = 0; i < p.vtot; ++i) {
+        if (i == 0 || grid[order[i]] != grid[order[i - 1]]) {
+            cov_grid.push_back(grid[order[i]]);
+            cov_off.push_back((int)i);
+        }
+    }
+    cov_off.push_back((int)p.vtot);
+    p.ncov = (int)cov_grid.size();
+    // heaviest atoms first in the projection launch
+    std::vector<int> aord(natoms);
+    std::iota(aord.begin(), aord.end(), 0);
+    std::stable_sort(aord.begin(), aord.end(), [&](int x, int y) {
+        return (vox_off[x + 1] - vox_off[x]) * nproj[x] > (vox_off[y + 1] - vox_off[y]) * nproj[y];
+    });
+
+    std::vector<float> t32(table, table + p.ttot);
+    QM_TRY(upload(&p.d_nproj, nproj, natoms));
+    QM_TRY(upload(&p.d_proj_off, proj_off, natoms + 1));
+    QM_TRY(upload(&p.d_vox_off, vox_off, natoms + 1));
+    QM_TRY(upload(&p.d_tab_off, tab_off, natoms + 1));
+    QM_TRY(upload(&p.d_vox_grid, grid.data(), p.vtot));
+    QM_TRY(upload(&p.d_vox_atom, atom.data(), p.vtot));
+    QM_TRY(upload(&p.d_energy, energy, p.ptot));
+    QM_TRY(upload(&p.d_table64, table, p.ttot));
+    QM_TRY(upload(&p.d_table32, t32.data(), p.ttot));
+    if (dtable) {
+        std::vector<float> d32(dtable, dtable + 3 * p.ttot);
+        QM_TRY(upload(&p.d_dtable64, dtable, 3 * p.ttot));
+        QM_TRY(upload(&p.d_dtable32, d32.data(), 3 * p.ttot));
+    }
+    QM_TRY(upload(&p.d_cov_grid, cov_grid.data(), cov_grid.size()));
+    QM_TRY(upload(&p.d_cov_off, cov_off.data(), cov_off.size()));
+    QM_TRY(upload(&p.d_cov_vox, order.data(), order.size()));
+    QM_TRY(upload(&p.d_atom_order, aord.data(), aord.size()));
+    // voxel-major tables padded to mpad projectors, and the projection work list
+    {
+        const int mp = p.mpad;
+        std::vector<double> tp((size_t)p.vtot * mp, 0.0);
+        std::vector<float> tpf((size_t)p.vtot * mp, 0.0f);
+        std::vector<int> catom, cv0, acoff(natoms + 1, 0), patom(p.ptot);
+        for (int a = 0; a < natoms; ++a) {
+            const int64_t V = vox_off[a + 1] - vox_off[a];
+            for (int w = 0; w < nproj[a]; ++w) {
+                patom[proj_off[a] + w] = a;
+                for (int64_t v = 0; v < V; ++v) {
+                    const double val = table[tab_off[a] + w * V + v];
+                    tp[(size_t)(vox_off[a] + v) * mp + w] = val;
+                    tpf[(size_t)(vox_off[a] + v) * mp + w] = (float)val;
+                }
+            }
+            for (int64_t v0 = 0; v0 < V; v0 += kProjChunk) {
+                catom.push_back(a);
+                cv0.push_back((int)v0);
+            }
+            acoff[a + 1] = (int)catom.size();
+        }
+        p.nchunk = (int)catom.size();
+        QM_TRY(upload(&p.d_tabp_64, tp.data(), tp.size()));
+        QM_TRY(upload(&p.d_tabp_32, tpf.data(), tpf.size()));
+        QM_TRY(upload(&p.d_chunk_atom, catom.data(), catom.size()));
+        QM_TRY(upload(&p.d_chunk_v0, cv0.data(), cv0.size()));
+        QM_TRY(upload(&p.d_atom_chunk_off, acoff.data(), acoff.size()));
+        QM_TRY(upload(&p.d_proj_atom, patom.data(), patom.size()));
+    }
+    // per scatter workgroup: distinct atoms of its entries, and every entry's slot
+    {
+        const int nb = (p.ncov + kScatterBlock - 1) / kScatterBlock;
+        std::vector<int> boff(nb + 1, 0), batoms, slot(p.vtot, -1);
+        for (int b = 0; b < nb; ++b) {
+            std::vector<int> list;
+            const int i0 = b * kScatterBlock, i1 = std::min(p.ncov, i0 + kScatterBlock);
+            for (int i = i0; i < i1; ++i)
+                for (int e = cov_off[i]; e < cov_off[i + 1]; ++e) {
+                    const int a = atom[order[e]];
+                    int s = -1;
+                    for (int q = 0; q < (int)list.size(); ++q)
+                        if (list[q] == a) s = q;
+                    if (s < 0 && (int)list.size() < kScatterSlots) {
+                        list.push_back(a);
+                        s = (int)list.size() - 1;
+                    }
+                    slot[e] = s;
+                }
+            batoms.insert(batoms.end(), list.begin(), list.end());
+            boff[b + 1] = (int)batoms.size();
+        }
+        if (batoms.empty()) batoms.push_back(0);
+        p.nsblock = nb;
+        QM_TRY(upload(&p.d_block_atom_off, boff.data(), boff.size()));
+        QM_TRY(upload(&p.d_block_atoms, batoms.data(), batoms.size()));
+        QM_TRY(upload(&p.d_cov_slot, slot.data(), slot.size()));
+    }
+    return QM_OK;
+}
+
+// The local part (kinetic + V_loc, overwrite) with the context's kinetic term
+template <typename T>
+int launch_local(qm_ctx *ctx, T *out, const T *in, int nb, int ncomp, const double k[3], hipStream_t st) {
+    if (ctx->kinetic == QM_KINETIC_FFT) return launch_fft_local<T>(ctx, out, in, nb, ncomp, k, st);
+    if constexpr (sizeof(T) == 8) return launch_fd8_local_f64(ctx, out, in, nb, ncomp, k, st);
+    else return launch_fd8_local_f32(ctx, out, in, nb, ncomp, k, st);
+}
+
 // part: 3 = local + nonlocal (overwrite), 1 = local only (overwrite), 2 = nonlocal only (accumulate)
 template <typename T>
 int hpsi_apply(qm_ctx *ctx, T *hpsi, const T *psi, int nb, int ncomp, int ik, void *stream, int part = 3) {
@@ -160,6 +315,9 @@ int hpsi_apply(qm_ctx *ctx, T *hpsi, const T *psi, int nb, int ncomp, int ik, vo
     if (nb == 0) return QM_OK;
     QM_TRY(activate(ctx));
     hipStream_t st = (hipStream_t)stream;
+    bool capturing = false;
+    QM_TRY(caller_capturing(st, &capturing));
+    if (capturing) ctx->capture_pins = 1;
     const KPoint &kp = ctx->kpts[ik];
     const bool nl = ctx->proj.natoms > 0 && ctx->proj.ptot > 0;
     if (part == 3 && nl) {
@@ -179,14 +337,13 @@ int hpsi_apply(qm_ctx *ctx, T *hpsi, const T *psi, int nb, int ncomp, int ik, vo
             if constexpr (sizeof(T) == 8) QM_TRY(nonlocal_project_f64(ctx, psi, nb, ncomp, kp, ctx->side));
             else QM_TRY(nonlocal_project_f32(ctx, psi, nb, ncomp, kp, ctx->side));
             QM_HIP(hipEventRecord(ctx->ev_join, ctx->side));
-            if constexpr (sizeof(T) == 8) QM_TRY(launch_fd8_local_f64(ctx, hpsi, psi, nb, ncomp, kp.k, st));
-            else QM_TRY(launch_fd8_local_f32(ctx, hpsi, psi, nb, ncomp, kp.k, st));
+            QM_TRY(launch_local<T>(ctx, hpsi, psi, nb, ncomp, kp.k, st));
             QM_HIP(hipStreamWaitEvent(st, ctx->ev_join, 0));
             if constexpr (sizeof(T) == 8) QM_TRY(nonlocal_scatter_f64(ctx, hpsi, nb, ncomp, kp, st));
             else QM_TRY(nonlocal_scatter_f32(ctx, hpsi, nb, ncomp, kp, st));
             return QM_OK;
         };
-        if (ctx->stencil_force >= 0) return full();
+        if (ctx->stencil_force >= 0 || ctx->kinetic == QM_KINETIC_FFT) return full();
         // The stencil's tile shape for full applies is tuned on full applies:
         // the shape that runs fastest alone is not always the one that leaves
         // the projection the most room beside it.  Every shape gives the same
@@ -239,15 +396,21 @@ int hpsi_apply(qm_ctx *ctx, T *hpsi, const T *psi, int nb, int ncomp, int ik, vo
         ctx->stencil_force = cand;
         return full();
     }
+    if (part & 1) QM_TRY(launch_local<T>(ctx, hpsi, psi, nb, ncomp, kp.k, st));
     if constexpr (sizeof(T) == 8) {
-        if (part & 1) QM_TRY(launch_fd8_local_f64(ctx, hpsi, psi, nb, ncomp, kp.k, st));
         if (part & 2) QM_TRY(launch_nonlocal_f64(ctx, hpsi, psi, nb, ncomp, kp, st));
     } else {
-        if (part & 1) QM_TRY(launch_fd8_local_f32(ctx, hpsi, psi, nb, ncomp, kp.k, st));
         if (part & 2) QM_TRY(launch_nonlocal_f32(ctx, hpsi, psi, nb, ncomp, kp, st));
     }
     return QM_OK;
 }
+
+template <typename T>
+int hpsi_part(qm_ctx *ctx, T *out, const T *in, int nb, int ncomp, int ik, hipStream_t st, int part) {
+    return hpsi_apply<T>(ctx, out, in, nb, ncomp, ik, (void *)st, part);
+}
+template int hpsi_part<double>(qm_ctx *, double *, const double *, int, int, int, hipStream_t, int);
+template int hpsi_part<float>(qm_ctx *, float *, const float *, int, int, int, hipStream_t, int);
 
 template <typename T>
 int band_density_impl(qm_ctx *ctx, double *rho, const T *psi, const double *occ, int nb, int ncomp,
@@ -354,6 +517,8 @@ void qm_destroy(qm_ctx *ctx) {
     ctx->gemm_part.release();
     ctx->nl_part.release();
     ctx->eig_work.release();
+    ctx->td_work.release();
+    fft_destroy(ctx);
     if (ctx->blas) rocblas_destroy_handle((rocblas_handle)ctx->blas);
     if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
     if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
@@ -365,8 +530,17 @@ int qm_set_vloc(qm_ctx *ctx, const double *vloc) {
     QM_CHECK(ctx && vloc, QM_ERR_ARG, "qm_set_vloc: NULL argument");
     QM_TRY(activate(ctx));
     std::vector<float> v32(vloc, vloc + ctx->ng);
-    QM_TRY(upload(&ctx->d_vloc64, vloc, ctx->ng));
-    QM_TRY(upload(&ctx->d_vloc32, v32.data(), ctx->ng));
+    if (ctx->d_vloc64 && ctx->d_vloc32) {
+        // same size every time: overwrite in place (a captured apply keeps
+        // reading the same buffers, and sees the new values); the copy is
+        // ordered after the work queued on the null stream, and waits for it
+        QM_HIP(hipDeviceSynchronize());
+        QM_HIP(hipMemcpy(ctx->d_vloc64, vloc, sizeof(double) * ctx->ng, hipMemcpyHostToDevice));
+        QM_HIP(hipMemcpy(ctx->d_vloc32, v32.data(), sizeof(float) * ctx->ng, hipMemcpyHostToDevice));
+    } else {
+        QM_TRY(upload(&ctx->d_vloc64, vloc, ctx->ng));
+        QM_TRY(upload(&ctx->d_vloc32, v32.data(), ctx->ng));
+    }
     ctx->has_vloc = true;
     return QM_OK;
 }
@@ -376,165 +550,85 @@ int qm_set_projectors(qm_ctx *ctx, int natoms, const int *nproj, const int64_t *
                       const int *proj_off) {
     QM_CHECK(ctx, QM_ERR_ARG, "qm_set_projectors: NULL context");
     QM_CHECK(natoms >= 0, QM_ERR_ARG, "qm_set_projectors: natoms < 0");
+    QM_TRY(check_unpinned(ctx, "qm_set_projectors"));
     QM_TRY(activate(ctx));
-    QM_HIP(hipDeviceSynchronize());
-    free_projectors(ctx->proj);
-    Projectors &p = ctx->proj;
-    if (natoms == 0) {
-        for (auto &k : ctx->kpts) QM_TRY(build_phase(ctx, k));
-        return QM_OK;
-    }
-    QM_CHECK(nproj && vox_off && vox_xyz && table && tab_off && energy && proj_off, QM_ERR_ARG,
-             "qm_set_projectors: NULL array");
-    QM_CHECK(vox_off[0] == 0 && tab_off[0] == 0 && proj_off[0] == 0, QM_ERR_ARG,
-             "qm_set_projectors: offsets must start at 0");
-    p.natoms = natoms;
-    p.nproj.assign(nproj, nproj + natoms);
-    p.vox_off.assign(vox_off, vox_off + natoms + 1);
-    p.tab_off.assign(tab_off, tab_off + natoms + 1);
-    p.proj_off.assign(proj_off, proj_off + natoms + 1);
-    for (int a = 0; a < natoms; ++a) {
-        const int64_t V = vox_off[a + 1] - vox_off[a];
-        QM_CHECK(nproj[a] >= 0 && nproj[a] <= kMaxProj, QM_ERR_ARG, "qm_set_projectors: nproj must be in [0, 16]");
-        QM_CHECK(V >= 0 && V < ((int64_t)1 << 31), QM_ERR_ARG, "qm_set_projectors: bad voxel range");
-        QM_CHECK(tab_off[a + 1] - tab_off[a] == V * nproj[a], QM_ERR_ARG,
-                 "qm_set_projectors: tab_off inconsistent with nproj * voxels");
-        QM_CHECK(proj_off[a + 1] - proj_off[a] == nproj[a], QM_ERR_ARG,
-                 "qm_set_projectors: proj_off inconsistent with nproj");
-        p.max_m = std::max(p.max_m, nproj[a]);
-    }
-    p.vtot = vox_off[natoms];
-    p.ttot = tab_off[natoms];
-    p.ptot = proj_off[natoms];
-    QM_CHECK(p.vtot < ((int64_t)1 << 31), QM_ERR_ARG, "qm_set_projectors: too many voxels");
-    p.vox_xyz.assign(vox_xyz, vox_xyz + 3 * p.vtot);
-    p.energy.assign(energy, energy + p.ptot);
-    p.has_grad = dtable != nullptr;
-
-    // wrapped grid index and owner atom of every voxel
-    std::vector<int> grid(p.vtot), atom(p.vtot);
-    for (int a = 0; a < natoms; ++a)
-        for (int64_t v = vox_off[a]; v < vox_off[a + 1]; ++v) {
-            grid[v] = wrap_host(vox_xyz[3 * v], ctx->nx) +
-                      ctx->nx * (wrap_host(vox_xyz[3 * v + 1], ctx->ny) +
-                                 ctx->ny * wrap_host(vox_xyz[3 * v + 2], ctx->nz));
-            atom[v] = a;
+    // Every argument is validated and the whole new set is built on the side;
+    // the context's set and phases change only when everything succeeded.
+    Projectors np;
+    std::vector<KPoint> nk(ctx->kpts.size());
+    struct Cleanup {  // frees the new set unless it was handed over
+        Projectors *p;
+        std::vector<KPoint> *k;
+        ~Cleanup() {
+            if (p) free_projectors(*p);
+            if (k)
+                for (auto &kp : *k) {
+                    dfree(kp.d_phase64);
+                    dfree(kp.d_phase32);
+                }
         }
-    // covered-point map for the owner-computes scatter: voxels sorted by grid point
-    std::vector<int> order(p.vtot);
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return grid[x] < grid[y]; });
-    std::vector<int> cov_grid, cov_off;
-    cov_off.reserve(p.vtot + 1);
-    for (int64_t i = 0; i < p.vtot; ++i) {
-        if (i == 0 || grid[order[i]] != grid[order[i - 1]]) {
-            cov_grid.push_back(grid[order[i]]);
-            cov_off.push_back((int)i);
-        }
+    } cleanup{&np, &nk};
+    for (size_t i = 0; i < nk.size(); ++i) {
+        for (int d = 0; d < 3; ++d) nk[i].k[d] = ctx->kpts[i].k[d];
+        nk[i].gamma = ctx->kpts[i].gamma;
     }
-    cov_off.push_back((int)p.vtot);
-    p.ncov = (int)cov_grid.size();
-    // heaviest atoms first in the projection launch
-    std::vector<int> aord(natoms);
-    std::iota(aord.begin(), aord.end(), 0);
-    std::stable_sort(aord.begin(), aord.end(), [&](int x, int y) {
-        return (vox_off[x + 1] - vox_off[x]) * nproj[x] > (vox_off[y + 1] - vox_off[y]) * nproj[y];
-    });
-
-    std::vector<float> t32(table, table + p.ttot);
-    QM_TRY(upload(&p.d_nproj, nproj, natoms));
-    QM_TRY(upload(&p.d_proj_off, proj_off, natoms + 1));
-    QM_TRY(upload(&p.d_vox_off, vox_off, natoms + 1));
-    QM_TRY(upload(&p.d_tab_off, tab_off, natoms + 1));
-    QM_TRY(upload(&p.d_vox_grid, grid.data(), p.vtot));
-    QM_TRY(upload(&p.d_vox_atom, atom.data(), p.vtot));
-    QM_TRY(upload(&p.d_energy, energy, p.ptot));
-    QM_TRY(upload(&p.d_table64, table, p.ttot));
-    QM_TRY(upload(&p.d_table32, t32.data(), p.ttot));
-    if (dtable) {
-        std::vector<float> d32(dtable, dtable + 3 * p.ttot);
-        QM_TRY(upload(&p.d_dtable64, dtable, 3 * p.ttot));
-        QM_TRY(upload(&p.d_dtable32, d32.data(), 3 * p.ttot));
-    }
-    QM_TRY(upload(&p.d_cov_grid, cov_grid.data(), cov_grid.size()));
-    QM_TRY(upload(&p.d_cov_off, cov_off.data(), cov_off.size()));
-    QM_TRY(upload(&p.d_cov_vox, order.data(), order.size()));
-    QM_TRY(upload(&p.d_atom_order, aord.data(), aord.size()));
-    // voxel-major, 16-padded tables and the projection work list
-    {
-        std::vector<double> t16((size_t)p.vtot * kMaxProj, 0.0);
-        std::vector<float> t16f((size_t)p.vtot * kMaxProj, 0.0f);
-        std::vector<int> catom, cv0, acoff(natoms + 1, 0), patom(p.ptot);
+    if (natoms > 0) {
+        QM_CHECK(nproj && vox_off && vox_xyz && table && tab_off && energy && proj_off, QM_ERR_ARG,
+                 "qm_set_projectors: NULL array");
+        QM_CHECK(vox_off[0] == 0 && tab_off[0] == 0 && proj_off[0] == 0, QM_ERR_ARG,
+                 "qm_set_projectors: offsets must start at 0");
         for (int a = 0; a < natoms; ++a) {
             const int64_t V = vox_off[a + 1] - vox_off[a];
-            for (int w = 0; w < nproj[a]; ++w) {
-                patom[proj_off[a] + w] = a;
-                for (int64_t v = 0; v < V; ++v) {
-                    const double val = table[tab_off[a] + w * V + v];
-                    t16[(size_t)(vox_off[a] + v) * kMaxProj + w] = val;
-                    t16f[(size_t)(vox_off[a] + v) * kMaxProj + w] = (float)val;
-                }
-            }
-            for (int64_t v0 = 0; v0 < V; v0 += kProjChunk) {
-                catom.push_back(a);
-                cv0.push_back((int)v0);
-            }
-            acoff[a + 1] = (int)catom.size();
+            QM_CHECK(nproj[a] >= 0 && nproj[a] <= kMaxProj, QM_ERR_ARG, "qm_set_projectors: nproj must be in [0, 32]");
+            QM_CHECK(V >= 0 && V < ((int64_t)1 << 31), QM_ERR_ARG, "qm_set_projectors: bad voxel range");
+            QM_CHECK(tab_off[a + 1] - tab_off[a] == V * nproj[a], QM_ERR_ARG,
+                     "qm_set_projectors: tab_off inconsistent with nproj * voxels");
+            QM_CHECK(proj_off[a + 1] - proj_off[a] == nproj[a], QM_ERR_ARG,
+                     "qm_set_projectors: proj_off inconsistent with nproj");
         }
-        p.nchunk = (int)catom.size();
-        QM_TRY(upload(&p.d_tab16_64, t16.data(), t16.size()));
-        QM_TRY(upload(&p.d_tab16_32, t16f.data(), t16f.size()));
-        QM_TRY(upload(&p.d_chunk_atom, catom.data(), catom.size()));
-        QM_TRY(upload(&p.d_chunk_v0, cv0.data(), cv0.size()));
-        QM_TRY(upload(&p.d_atom_chunk_off, acoff.data(), acoff.size()));
-        QM_TRY(upload(&p.d_proj_atom, patom.data(), patom.size()));
+        QM_CHECK(vox_off[natoms] < ((int64_t)1 << 31), QM_ERR_ARG, "qm_set_projectors: too many voxels");
+        QM_TRY(build_projectors(ctx, np, natoms, nproj, vox_off, vox_xyz, table, dtable, tab_off, energy, proj_off));
     }
-    // per scatter workgroup: distinct atoms of its entries, and every entry's slot
-    {
-        const int nb = (p.ncov + kScatterBlock - 1) / kScatterBlock;
-        std::vector<int> boff(nb + 1, 0), batoms, slot(p.vtot, -1);
-        for (int b = 0; b < nb; ++b) {
-            std::vector<int> list;
-            const int i0 = b * kScatterBlock, i1 = std::min(p.ncov, i0 + kScatterBlock);
-            for (int i = i0; i < i1; ++i)
-                for (int e = cov_off[i]; e < cov_off[i + 1]; ++e) {
-                    const int a = atom[order[e]];
-                    int s = -1;
-                    for (int q = 0; q < (int)list.size(); ++q)
-                        if (list[q] == a) s = q;
-                    if (s < 0 && (int)list.size() < kScatterSlots) {
-                        list.push_back(a);
-                        s = (int)list.size() - 1;
-                    }
-                    slot[e] = s;
-                }
-            batoms.insert(batoms.end(), list.begin(), list.end());
-            boff[b + 1] = (int)batoms.size();
-        }
-        if (batoms.empty()) batoms.push_back(0);
-        p.nsblock = nb;
-        QM_TRY(upload(&p.d_block_atom_off, boff.data(), boff.size()));
-        QM_TRY(upload(&p.d_block_atoms, batoms.data(), batoms.size()));
-        QM_TRY(upload(&p.d_cov_slot, slot.data(), slot.size()));
-    }
-    for (auto &k : ctx->kpts) QM_TRY(build_phase(ctx, k));
+    for (auto &k : nk) QM_TRY(build_phase(ctx, np, k));
+    // applies queued before this call may still read the old tables
+    QM_HIP(hipDeviceSynchronize());
+    free_projectors(ctx->proj);
+    free_kpoints(ctx);
+    ctx->proj = np;
+    ctx->kpts = nk;
+    cleanup.p = nullptr;
+    cleanup.k = nullptr;
     return QM_OK;
 }
 
 int qm_set_kpoints(qm_ctx *ctx, int nk, const double *kvec) {
     QM_CHECK(ctx && kvec && nk >= 1, QM_ERR_ARG, "qm_set_kpoints: bad argument");
+    QM_TRY(check_unpinned(ctx, "qm_set_kpoints"));
     QM_TRY(activate(ctx));
-    QM_HIP(hipDeviceSynchronize());
-    free_kpoints(ctx);
-    ctx->kpts.resize(nk);
+    std::vector<KPoint> kp(nk);
+    struct Cleanup {  // frees the new phases unless they were handed over
+        std::vector<KPoint> *k;
+        ~Cleanup() {
+            if (k)
+                for (auto &x : *k) {
+                    dfree(x.d_phase64);
+                    dfree(x.d_phase32);
+                }
+        }
+    } cleanup{&kp};
     for (int i = 0; i < nk; ++i) {
-        KPoint &k = ctx->kpts[i];
+        KPoint &k = kp[i];
         for (int d = 0; d < 3; ++d) k.k[d] = kvec[3 * i + d];
         // the reference treats a k-point as Gamma only when flagged so
         // (m_is_gamma_point_list); here: exactly zero k
         k.gamma = (k.k[0] == 0.0 && k.k[1] == 0.0 && k.k[2] == 0.0);
-        QM_TRY(build_phase(ctx, k));
+        QM_TRY(build_phase(ctx, ctx->proj, k));
     }
+    // applies queued before this call may still read the old phases
+    QM_HIP(hipDeviceSynchronize());
+    free_kpoints(ctx);
+    ctx->kpts = kp;
+    cleanup.k = nullptr;
     return QM_OK;
 }
 
@@ -600,6 +694,12 @@ int qm_nonlocal_forces_f64(qm_ctx *ctx, const double *d_psi, const double *occ, 
 int qm_nonlocal_forces_f32(qm_ctx *ctx, const float *d_psi, const double *occ, int nb, int ncomp, int ik,
                            double *forces_out, double *e_nl, void *stream) {
     return forces<float>(ctx, d_psi, occ, nb, ncomp, ik, forces_out, e_nl, stream);
+}
+
+int qm_release_captures(qm_ctx *ctx) {
+    QM_CHECK(ctx, QM_ERR_ARG, "qm_release_captures: NULL context");
+    ctx->capture_pins = 0;
+    return QM_OK;
 }
 
 int qm_stencil_num_candidates(void) { return stencil_num_candidates(); }

@@ -1,0 +1,310 @@
+// Spectral (FFT) kinetic term on gfx950: the reference's default kinetic
+// operator in mHamiltonianMatrix_ddm_bundle (m_algorithm_Laplacian = 1,
+// qumasun_base1.matrix.h:171-228; its hipFFT many-plan, gyield/gy_fft_executor_hip.h:188-248):
+//   H psi = IFFT( |G + k|^2 / 2 . FFT psi ) / N + V_loc psi
+// with G = 2 pi / L_d times the wrapped index (i > n/2 -> i - n, so the n/2
+// Nyquist index is +n/2), OperateKineticInKspace_bundle (qumasun_base1.matrix.h:96-130).
+// And the kinetic step of the split-operator propagator,
+//   psi <- IFFT( e^{-i dt |G + k|^2 / 2} FFT psi ) / N
+// (EvolveKineticInKspace / mEvolveK_FFT, qumasun_symplectic.kinetic_fft.h:14-60, 65-140).
+//
+// GPU structure: rocFFT batched 3-D transforms straight from and to the SoA
+// band layout.  Complex bands are read as a planar (re, im) array pair with a
+// batch distance of 2 Ng, so there is no gather into an interleaved copy as in
+// the reference's GatherExchangeD2Z; real Gamma-point bands use a real-to-
+// Hermitian transform (half the spectrum).  The k-space factor (and 1/N) is one
+// elementwise kernel, and V_loc psi is added by one fused kernel after the
+// inverse transform.  Plans are cached per (precision, ncomp, band count).
+#include <rocfft/rocfft.h>
+
+#include <cmath>
+#include <map>
+#include <mutex>
+
+#include "qm_internal.h"
+
+namespace qm {
+
+namespace {
+
+#define QM_FFT(call)                                                                               \
+    do {                                                                                           \
+        rocfft_status s_ = (call);                                                                 \
+        if (s_ != rocfft_status_success) {                                                         \
+            ::qm::set_error(std::string(#call) + ": rocFFT status " + std::to_string((int)s_) +    \
+                            " at " + __FILE__ + ":" + std::to_string(__LINE__));                   \
+            return QM_ERR_HIP;                                                                     \
+        }                                                                                          \
+    } while (0)
+
+std::once_flag g_rocfft_once;
+
+// One direction of a cached transform pair.
+struct FftPlan {
+    rocfft_plan plan = nullptr;
+    rocfft_execution_info info = nullptr;
+    size_t work = 0;
+};
+
+struct FftPair {
+    FftPlan fwd, inv;
+};
+
+}  // namespace
+
+// Transforms and buffers of one context (opaque in qm_ctx).
+struct FftState {
+    std::map<int64_t, FftPair> plans;
+    DevBuf spec;  // spectra of the band block (complex interleaved)
+    DevBuf work;  // rocFFT work buffer
+};
+
+namespace {
+
+template <typename T>
+struct C2 {
+    T x, y;
+};
+
+// spec *= f(|G + k|^2 / 2) for every band's spectrum (nxe x ny x nz complex,
+// x fastest; nxe = nx, or nx/2 + 1 for a real band's half spectrum).
+//   mode 0: f = e2 * scale (kinetic operator)
+//   mode 1: f = e^{-i dt e2} * scale (kinetic propagator)
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void kspace_kernel(C2<T> *__restrict__ spec, int64_t per_band, int64_t total,
+                                                     int nxe, int nx, int ny, int nz, double gx, double gy, double gz,
+                                                     double kx, double ky, double kz, double scale, double dt) {
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (int64_t)gridDim.x * 256) {
+        const int64_t i = e % per_band;
+        const int ix = (int)(i % nxe);
+        const int64_t r = i / nxe;
+        const int iy = (int)(r % ny), iz = (int)(r / ny);
+        const double qx = gx * (double)(ix * 2 > nx ? ix - nx : ix) + kx;
+        const double qy = gy * (double)(iy * 2 > ny ? iy - ny : iy) + ky;
+        const double qz = gz * (double)(iz * 2 > nz ? iz - nz : iz) + kz;
+        const double e2 = 0.5 * (qx * qx + qy * qy + qz * qz);
+        C2<T> v = spec[e];
+        if constexpr (MODE == 0) {
+            const T f = (T)(e2 * scale);
+            v.x *= f;
+            v.y *= f;
+        } else {
+            double sn, cs;
+            sincos(-dt * e2, &sn, &cs);
+            const T c = (T)(cs * scale), s = (T)(sn * scale);
+            const T re = v.x, im = v.y;
+            v.x = c * re - s * im;
+            v.y = c * im + s * re;
+        }
+        spec[e] = v;
+    }
+}
+
+// out[col][i] += V[i] psi[col][i] over ncol columns of Ng
+template <typename T>
+__global__ __launch_bounds__(256) void vpsi_add_kernel(T *__restrict__ out, const T *__restrict__ psi,
+                                                       const T *__restrict__ v, int64_t ng, int64_t total) {
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (int64_t)gridDim.x * 256)
+        out[e] += v[e % ng] * psi[e];
+}
+
+inline unsigned grid_of(int64_t n) {
+    const int64_t b = (n + 255) / 256;
+    return (unsigned)std::min<int64_t>(std::max<int64_t>(b, 1), 1 << 20);
+}
+
+int make_plan(FftPlan &p, rocfft_transform_type type, rocfft_precision prec, const size_t len[3], size_t batch,
+              rocfft_array_type in_t, rocfft_array_type out_t, const size_t in_str[3], size_t in_dist,
+              const size_t out_str[3], size_t out_dist) {
+    rocfft_plan_description d = nullptr;
+    QM_FFT(rocfft_plan_description_create(&d));
+    rocfft_status st = rocfft_plan_description_set_data_layout(d, in_t, out_t, nullptr, nullptr, 3, in_str, in_dist,
+                                                               3, out_str, out_dist);
+    if (st == rocfft_status_success)
+        st = rocfft_plan_create(&p.plan, rocfft_placement_notinplace, type, prec, 3, len, batch, d);
+    (void)rocfft_plan_description_destroy(d);
+    QM_FFT(st);
+    QM_FFT(rocfft_plan_get_work_buffer_size(p.plan, &p.work));
+    QM_FFT(rocfft_execution_info_create(&p.info));
+    return QM_OK;
+}
+
+void destroy_pair(FftPair &q) {
+    for (FftPlan *p : {&q.fwd, &q.inv}) {
+        if (p->info) (void)rocfft_execution_info_destroy(p->info);
+        if (p->plan) (void)rocfft_plan_destroy(p->plan);
+        p->info = nullptr;
+        p->plan = nullptr;
+    }
+}
+
+// The forward / inverse pair for nb bands of this shape: complex bands go
+// planar SoA -> interleaved spectrum -> planar SoA; real bands go real ->
+// Hermitian half spectrum -> real.
+template <typename T>
+int get_pair(qm_ctx *ctx, int nb, int ncomp, FftPair **out) {
+    std::call_once(g_rocfft_once, [] { (void)rocfft_setup(); });
+    if (!ctx->fft) ctx->fft = new FftState();
+    FftState &f = *ctx->fft;
+    const int64_t key = ((int64_t)nb * 2 + (ncomp - 1)) * 2 + (sizeof(T) == 8 ? 1 : 0);
+    auto it = f.plans.find(key);
+    if (it != f.plans.end()) {
+        *out = &it->second;
+        return QM_OK;
+    }
+    const rocfft_precision prec = sizeof(T) == 8 ? rocfft_precision_double : rocfft_precision_single;
+    const size_t nx = ctx->nx, ny = ctx->ny, nz = ctx->nz, ng = (size_t)ctx->ng;
+    const size_t len[3] = {nx, ny, nz};  // fastest first
+    const size_t str[3] = {1, nx, nx * ny};
+    FftPair q;
+    int rc = QM_OK;
+    if (ncomp == 2) {
+        rc = make_plan(q.fwd, rocfft_transform_type_complex_forward, prec, len, nb, rocfft_array_type_complex_planar,
+                       rocfft_array_type_complex_interleaved, str, 2 * ng, str, ng);
+        if (rc == QM_OK)
+            rc = make_plan(q.inv, rocfft_transform_type_complex_inverse, prec, len, nb,
+                           rocfft_array_type_complex_interleaved, rocfft_array_type_complex_planar, str, ng, str,
+                           2 * ng);
+    } else {
+        const size_t nxh = nx / 2 + 1;
+        const size_t hstr[3] = {1, nxh, nxh * ny};
+        rc = make_plan(q.fwd, rocfft_transform_type_real_forward, prec, len, nb, rocfft_array_type_real,
+                       rocfft_array_type_hermitian_interleaved, str, ng, hstr, nxh * ny * nz);
+        if (rc == QM_OK)
+            rc = make_plan(q.inv, rocfft_transform_type_real_inverse, prec, len, nb,
+                           rocfft_array_type_hermitian_interleaved, rocfft_array_type_real, hstr, nxh * ny * nz, str,
+                           ng);
+    }
+    if (rc != QM_OK) {
+        destroy_pair(q);
+        return rc;
+    }
+    *out = &(f.plans[key] = q);
+    return QM_OK;
+}
+
+int execute(FftState &f, FftPlan &p, void *in[], void *out[], hipStream_t st) {
+    if (p.work > 0) {
+        QM_TRY(f.work.reserve(p.work));
+        QM_FFT(rocfft_execution_info_set_work_buffer(p.info, f.work.p, p.work));
+    }
+    QM_FFT(rocfft_execution_info_set_stream(p.info, st));
+    QM_FFT(rocfft_execute(p.plan, in, out, p.info));
+    return QM_OK;
+}
+
+// spec = FFT(psi) (nb bands, ncomp 1 or 2); returns the spectrum and its size per band
+template <typename T>
+int forward(qm_ctx *ctx, const T *psi, int nb, int ncomp, FftPair **pair, C2<T> **spec, int64_t *per_band,
+            hipStream_t st) {
+    QM_TRY(get_pair<T>(ctx, nb, ncomp, pair));
+    FftState &f = *ctx->fft;
+    const int64_t pb = ncomp == 2 ? ctx->ng : (int64_t)(ctx->nx / 2 + 1) * ctx->ny * ctx->nz;
+    QM_CHECK(ctx->capture_pins == 0 || f.spec.bytes >= sizeof(C2<T>) * (size_t)pb * nb, QM_ERR_STATE,
+             "FFT kinetic: larger spectrum buffer needed while a graph holds an apply of this context; destroy it "
+             "and call qm_release_captures first");
+    QM_TRY(f.spec.reserve(sizeof(C2<T>) * (size_t)pb * nb));
+    C2<T> *s = f.spec.as<C2<T>>();
+    void *in[2] = {const_cast<T *>(psi), const_cast<T *>(psi) + ctx->ng};
+    void *out[1] = {s};
+    QM_TRY(execute(f, (*pair)->fwd, in, out, st));
+    *spec = s;
+    *per_band = pb;
+    return QM_OK;
+}
+
+template <typename T>
+int inverse(qm_ctx *ctx, FftPair *pair, C2<T> *spec, T *dst, hipStream_t st) {
+    void *in[1] = {spec};
+    void *out[2] = {dst, dst + ctx->ng};
+    return execute(*ctx->fft, pair->inv, in, out, st);
+}
+
+template <typename T, int MODE>
+void launch_kspace(qm_ctx *ctx, C2<T> *spec, int64_t per_band, int nb, int ncomp, const double k[3], double dt,
+                   hipStream_t st) {
+    const int nxe = ncomp == 2 ? ctx->nx : ctx->nx / 2 + 1;
+    const double gx = 2.0 * M_PI / (ctx->hx * ctx->nx), gy = 2.0 * M_PI / (ctx->hy * ctx->ny),
+                 gz = 2.0 * M_PI / (ctx->hz * ctx->nz);
+    const int64_t total = per_band * nb;
+    hipLaunchKernelGGL((kspace_kernel<T, MODE>), dim3(grid_of(total)), dim3(256), 0, st, spec, per_band, total, nxe,
+                       ctx->nx, ctx->ny, ctx->nz, gx, gy, gz, k[0], k[1], k[2], 1.0 / (double)ctx->ng, dt);
+}
+
+}  // namespace
+
+void fft_destroy(qm_ctx *ctx) {
+    if (!ctx->fft) return;
+    for (auto &kv : ctx->fft->plans) destroy_pair(kv.second);
+    ctx->fft->spec.release();
+    ctx->fft->work.release();
+    delete ctx->fft;
+    ctx->fft = nullptr;
+}
+
+// out = IFFT(|G + k|^2 / 2 FFT psi) / N + V_loc psi (overwrites out)
+template <typename T>
+int launch_fft_local(qm_ctx *ctx, T *out, const T *psi, int nb, int ncomp, const double k[3], hipStream_t st) {
+    const bool kt = k[0] != 0.0 || k[1] != 0.0 || k[2] != 0.0;
+    QM_CHECK(ncomp == 2 || !kt, QM_ERR_ARG, "real (ncomp=1) bands need a Gamma k-point");
+    FftPair *pair = nullptr;
+    C2<T> *spec = nullptr;
+    int64_t pb = 0;
+    QM_TRY(forward<T>(ctx, psi, nb, ncomp, &pair, &spec, &pb, st));
+    launch_kspace<T, 0>(ctx, spec, pb, nb, ncomp, k, 0.0, st);
+    QM_HIP(hipGetLastError());
+    QM_TRY(inverse<T>(ctx, pair, spec, out, st));
+    const T *v = sizeof(T) == 8 ? (const T *)ctx->d_vloc64 : (const T *)ctx->d_vloc32;
+    const int64_t total = (int64_t)nb * ncomp * ctx->ng;
+    hipLaunchKernelGGL(vpsi_add_kernel<T>, dim3(grid_of(total)), dim3(256), 0, st, out, psi, v, ctx->ng, total);
+    QM_HIP(hipGetLastError());
+    return QM_OK;
+}
+template int launch_fft_local<double>(qm_ctx *, double *, const double *, int, int, const double *, hipStream_t);
+template int launch_fft_local<float>(qm_ctx *, float *, const float *, int, int, const double *, hipStream_t);
+
+// psi <- IFFT(e^{-i dt |G + k|^2 / 2} FFT psi) / N, SoA complex bands
+template <typename T>
+int fft_kinetic_step(qm_ctx *ctx, T *psi, int nb, const double k[3], double dt, hipStream_t st) {
+    FftPair *pair = nullptr;
+    C2<T> *spec = nullptr;
+    int64_t pb = 0;
+    QM_TRY(forward<T>(ctx, psi, nb, 2, &pair, &spec, &pb, st));
+    launch_kspace<T, 1>(ctx, spec, pb, nb, 2, k, dt, st);
+    QM_HIP(hipGetLastError());
+    return inverse<T>(ctx, pair, spec, psi, st);
+}
+
+}  // namespace qm
+
+using namespace qm;
+
+extern "C" {
+
+int qm_set_kinetic(qm_ctx *ctx, int mode) {
+    QM_CHECK(ctx, QM_ERR_ARG, "qm_set_kinetic: NULL context");
+    QM_CHECK(mode == QM_KINETIC_FD8 || mode == QM_KINETIC_FFT, QM_ERR_ARG,
+             "qm_set_kinetic: mode must be QM_KINETIC_FD8 or QM_KINETIC_FFT");
+    ctx->kinetic = mode;
+    return QM_OK;
+}
+
+int qm_td_kinetic_fft_f64(qm_ctx *ctx, double *d_psi, int nb, int ik, double dt, void *stream) {
+    QM_CHECK(ctx && d_psi, QM_ERR_ARG, "qm_td_kinetic_fft: NULL argument");
+    QM_CHECK(nb >= 0 && ik >= 0 && ik < (int)ctx->kpts.size(), QM_ERR_ARG, "qm_td_kinetic_fft: bad nb or k index");
+    QM_CHECK(std::isfinite(dt), QM_ERR_ARG, "qm_td_kinetic_fft: dt is not finite");
+    if (nb == 0) return QM_OK;
+    QM_TRY(activate(ctx));
+    return fft_kinetic_step<double>(ctx, d_psi, nb, ctx->kpts[ik].k, dt, (hipStream_t)stream);
+}
+
+int qm_td_kinetic_fft_f32(qm_ctx *ctx, float *d_psi, int nb, int ik, double dt, void *stream) {
+    QM_CHECK(ctx && d_psi, QM_ERR_ARG, "qm_td_kinetic_fft: NULL argument");
+    QM_CHECK(nb >= 0 && ik >= 0 && ik < (int)ctx->kpts.size(), QM_ERR_ARG, "qm_td_kinetic_fft: bad nb or k index");
+    QM_CHECK(std::isfinite(dt), QM_ERR_ARG, "qm_td_kinetic_fft: dt is not finite");
+    if (nb == 0) return QM_OK;
+    QM_TRY(activate(ctx));
+    return fft_kinetic_step<float>(ctx, d_psi, nb, ctx->kpts[ik].k, dt, (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -9,8 +9,11 @@
 //                     normalised by the residual norms; complex blocks packed from
 //                     the real Gram of the SoA columns (AoSComplexFrom2by2Real,
 //                     matrix_2x2real_to_complex.h:8-56)
-//   H c = l S c       rocSOLVER sygvd / hegvd (the lapack_ZHEGVD(_emu) step,
-//                     lobpcg_z_multi_mpi.h:2254-2263, 2492-2511)
+//   H c = l S c       the Cholesky route of lapack_ZHEGVD_emu (w_zhegvd.h:227-272;
+//                     lobpcg_z_multi_mpi.h:2254-2263, 2492-2511): real n <= 512 by
+//                     the own Cholesky, trtri + GEMMs and tridiagonal reduction
+//                     (eigen.hip) with rocSOLVER stedc / ormtr; complex and larger
+//                     real problems by rocSOLVER potrf / trsm / syevd or heevd
 //   recurrence        S' = Cb^H S Cb, H' = Cb^H H Cb with Cb = [C | C_P] (rocBLAS;
 //                     NextMatrix_z_blasmp, 1104-1436); P' normalised by its norm
 //   update            [X' P'] = [X R P] U, [AX' AP'] = [AX AR AP] U (MFMA, T)
@@ -407,6 +410,7 @@ struct qm_lobpcg {
     int fallbacks = 0;  // subspace-solve fallbacks (P dropped or SVQB) in the last solve
     int rdrops = 0;     // R orthonormalisations that dropped near-dependent directions
     double cond64 = 1.0e-14, cond32 = 1.0e-4;  // minimum squared Cholesky pivot ratio of S
+    double limit_normx = qm::kLimitNormX;       // rollback when max |1 - |x'|^2| exceeds this
     double rdrop64 = 1.0e-13, rdrop32 = 1.0e-6; // R directions kept above this eigenvalue ratio
     void *XRn = nullptr, *Mr = nullptr, *Wr = nullptr, *Ur = nullptr;  // R orthonormalisation
     double *X64 = nullptr, *AX64 = nullptr;  // FP64 Rayleigh-quotient refinement of the mixed solve
@@ -789,7 +793,9 @@ int capture_segment(SegGraph &g, F &fn, hipStream_t cs) {
         (void)hipGetLastError();
         return QM_OK;
     }
+    ++g_internal_capture;  // this graph is destroyed before the call returns: no pin
     const int rc = fn(cs);
+    --g_internal_capture;
     hipStream_t st = cs;
     hipGraph_t graph = nullptr;
     const hipError_t ec = hipStreamEndCapture(st, &graph);
@@ -1082,6 +1088,9 @@ int solve_impl(qm_lobpcg *s, T *x_user, double *eigenvalues, double *residual_no
             QM_TRY(update<T>(Z, Y, (const T *)s->U, c * n3, 2 * nc, ng, st));
             QM_TRY(update<T>(AZ, AY, (const T *)s->U, c * n3, 2 * nc, ng, st));
         }
+        // the residual norms of the current x, returned if the update is rolled back
+        QM_HIP(hipMemcpyAsync(s->rnorm + 2 * Nall, s->rnorm + rn_off, sizeof(double) * N, hipMemcpyDeviceToDevice,
+                              st));
         // R' = AX' - X' L, |R'|^2, |X'|^2
         QM_TRY(band_reduce<T>(s->red_part, 1, s->rnorm, s->xnorm, AR, W[0], W[2], s->evals, N, c, ng, st));
         rn_off = 0;
@@ -1092,10 +1101,12 @@ int solve_impl(qm_lobpcg *s, T *x_user, double *eigenvalues, double *residual_no
                  "qm_lobpcg_solve: subspace eigensolve failed (rocSOLVER info " + std::to_string(*s->hinfo) + ")");
         double maxdev = 0.0;
         for (int n = 0; n < N; ++n) maxdev = std::max(maxdev, std::fabs(1.0 - h_norm[n]));
-        if (maxdev > kLimitNormX) {
+        if (maxdev > s->limit_normx) {
             // rollback and escape (lobpcg_z_multi_mpi.h:2611-2666): P = R and leave the
             // loop.  The reference rescales the old x by the norms of the rejected
-            // update (2639); the old x is renormalised with its own norms here.
+            // update (2639); the old x is renormalised with its own norms here, and
+            // its Ritz values and residual norms are the ones returned.
+            QM_HIP(hipMemcpyAsync(s->rnorm, s->rnorm + 2 * Nall, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
             QM_HIP(hipMemcpyAsync(P, R, s->tsz * (size_t)nc * ng, hipMemcpyDeviceToDevice, st));
             s->have_p = true;
             QM_TRY(band_reduce<T>(s->red_part, 2, s->xnorm, nullptr, nullptr, X, nullptr, nullptr, N, c, ng, st));
@@ -1131,7 +1142,7 @@ int solve_impl(qm_lobpcg *s, T *x_user, double *eigenvalues, double *residual_no
         s->have_p = iter_max > 0 ? hp : (!first_time && s->have_p);
     }
     if constexpr (sizeof(T) == 4) {
-        if (iter_max > 0 && !rolled_back) {
+        if (iter_max > 0) {  // also after a rollback: the returned (old) vectors
             // mixed precision: band energies as FP64 Rayleigh quotients of the returned
             // FP32 vectors with the FP64 operator, and FP64 residual norms
             // (second order in the vector error, so the FP32 operator's rounding drops out)
@@ -1320,15 +1331,22 @@ int qm_lobpcg_solve_locking(qm_lobpcg *s, void *d_x, double *eigenvalues, double
 
 int qm_lobpcg_set_conditioning(qm_lobpcg *s, double min_pivot_ratio_fp64, double min_pivot_ratio_fp32) {
     QM_CHECK(s, QM_ERR_ARG, "qm_lobpcg_set_conditioning: NULL solver");
-    QM_CHECK(min_pivot_ratio_fp64 >= 0.0 && min_pivot_ratio_fp32 >= 0.0 && min_pivot_ratio_fp64 < 1.0 &&
-                 min_pivot_ratio_fp32 < 1.0,
-             QM_ERR_ARG, "qm_lobpcg_set_conditioning: ratios must be in [0, 1)");
+    QM_CHECK(min_pivot_ratio_fp64 >= 0.0 && min_pivot_ratio_fp32 >= 0.0 && min_pivot_ratio_fp64 <= 2.0 &&
+                 min_pivot_ratio_fp32 <= 2.0,
+             QM_ERR_ARG, "qm_lobpcg_set_conditioning: ratios must be in [0, 2]");
     s->cond64 = min_pivot_ratio_fp64;
     s->cond32 = min_pivot_ratio_fp32;
     return QM_OK;
 }
 
 int qm_lobpcg_fallbacks(qm_lobpcg *s) { return s ? s->fallbacks : QM_ERR_ARG; }
+
+int qm_lobpcg_set_rollback_limit(qm_lobpcg *s, double limit) {
+    QM_CHECK(s, QM_ERR_ARG, "qm_lobpcg_set_rollback_limit: NULL solver");
+    QM_CHECK(limit >= 0.0, QM_ERR_ARG, "qm_lobpcg_set_rollback_limit: limit must be >= 0");
+    s->limit_normx = limit;
+    return QM_OK;
+}
 
 int qm_lobpcg_set_graphs(qm_lobpcg *s, int enable) {
     QM_CHECK(s, QM_ERR_ARG, "qm_lobpcg_set_graphs: NULL solver");

@@ -50,9 +50,10 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// part[(item * 16 + w) * ncol + col] = <p_w | e^{ik.r} psi>_col summed over the
+// part[(item * 16 MT + w) * ncol + col] = <p_w | e^{ik.r} psi>_col summed over the
 // item's voxels (col = real SoA column n * NC + c), for the GT 16-column tiles
-// of column group g (logical block = item * groups + g).  Each wave takes kProjChunk / 4 voxels in
+// of column group g (logical block = item * groups + g) and the MT 16-row
+// projector tiles of the table (rows padded to 16 MT).  Each wave takes kProjChunk / 4 voxels in
 // sub-chunks of 64.  Per sub-chunk it reads its block of the voxel-major
 // projector table once, as the MFMA A operand of all GT tiles; per tile the
 // 64 x 16 block of (phase-rotated) psi values is gathered with lanes along
@@ -61,14 +62,15 @@ __device__ __forceinline__ double wave_sum(double v) {
 // barrier is needed until the final reduction over the four waves.
 constexpr int PJ_VPW = kProjChunk / 4;  // voxels per wave
 constexpr int PJ_SUB = 64;              // voxels per LDS sub-chunk
-template <typename T, int NC, bool PH, int GT>
+template <typename T, int NC, bool PH, int GT, int MT>
 __global__ __launch_bounds__(256) void nl_project_kernel(double *__restrict__ part, const T *__restrict__ psi,
                                                          int64_t ng, int ncol, const int *__restrict__ chunk_atom,
                                                          const int *__restrict__ chunk_v0,
                                                          const int64_t *__restrict__ vox_off,
-                                                         const int *__restrict__ vox_grid, const T *__restrict__ tab16,
+                                                         const int *__restrict__ vox_grid, const T *__restrict__ tabp,
                                                          const T *__restrict__ phase, int groups, int nblk,
                                                          int per_xcd) {
+    constexpr int MPAD = 16 * MT;
     using M = Mfma<T>;
     using acc_t = typename M::acc_t;
     __shared__ T stage[4][PJ_SUB][16 + 1];
@@ -83,18 +85,21 @@ __global__ __launch_bounds__(256) void nl_project_kernel(double *__restrict__ pa
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int j = lane & 15, g = lane >> 4;
     const int vb = chunk_v0[item] + wave * PJ_VPW;
-    acc_t acc[GT];
+    acc_t acc[GT][MT];
 #pragma unroll
-    for (int t = 0; t < GT; ++t) acc[t] = acc_t{0, 0, 0, 0};
+    for (int t = 0; t < GT; ++t)
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) acc[t][mt] = acc_t{0, 0, 0, 0};
     for (int sc = 0; sc < PJ_VPW / PJ_SUB; ++sc) {
         const int vs = vb + sc * PJ_SUB;
         if (vs >= V) break;  // wave-uniform: the rest of the wave's voxels are past the sphere
-        // A[w = j][k = g] of the 16 k-steps of this sub-chunk
-        T av[PJ_SUB / 4];
+        // A[w = 16 mt + j][k = g] of the 16 k-steps of this sub-chunk
+        T av[MT][PJ_SUB / 4];
 #pragma unroll
         for (int s = 0; s < PJ_SUB / 4; ++s) {
             const int v = vs + 4 * s + g;
-            av[s] = v < V ? tab16[(vg0 + v) * kMaxProj + j] : T(0);
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) av[mt][s] = v < V ? tabp[(vg0 + v) * MPAD + 16 * mt + j] : T(0);
         }
         // the lane's voxel for the gathers
         const int v = vs + lane;
@@ -141,8 +146,11 @@ __global__ __launch_bounds__(256) void nl_project_kernel(double *__restrict__ pa
                 // compiler from moving them across
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 #pragma unroll
-                for (int s = 0; s < PJ_SUB / 4; ++s)
-                    acc[t] = M::mma(av[s], stage[wave][4 * s + g][j], acc[t]);  // B[k = g][col = j]
+                for (int s = 0; s < PJ_SUB / 4; ++s) {
+                    const T b = stage[wave][4 * s + g][j];  // B[k = g][col = j]
+#pragma unroll
+                    for (int mt = 0; mt < MT; ++mt) acc[t][mt] = M::mma(av[mt][s], b, acc[t][mt]);
+                }
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             }
         }
@@ -153,23 +161,26 @@ __global__ __launch_bounds__(256) void nl_project_kernel(double *__restrict__ pa
         const int cb = c0 + 16 * t;
         if (cb >= ncol) break;  // uniform over the workgroup
 #pragma unroll
-        for (int r = 0; r < 4; ++r) red[wave][M::row(g, r) * 16 + j] = (double)acc[t][r];
-        __syncthreads();
-        {
-            const int e = threadIdx.x;  // 256 = 16 projectors x 16 columns
-            const int w = e >> 4, cl = e & 15;
-            if (cb + cl < ncol) {
-                const double sum = (red[0][e] + red[1][e]) + (red[2][e] + red[3][e]);
-                part[((int64_t)item * kMaxProj + w) * ncol + cb + cl] = sum;
+        for (int mt = 0; mt < MT; ++mt) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) red[wave][M::row(g, r) * 16 + j] = (double)acc[t][mt][r];
+            __syncthreads();
+            {
+                const int e = threadIdx.x;  // 256 = 16 projectors x 16 columns
+                const int w = e >> 4, cl = e & 15;
+                if (cb + cl < ncol) {
+                    const double sum = (red[0][e] + red[1][e]) + (red[2][e] + red[3][e]);
+                    part[((int64_t)item * MPAD + 16 * mt + w) * ncol + cb + cl] = sum;
+                }
             }
+            __syncthreads();
         }
-        __syncthreads();
     }
 }
 
 // coef[p * ncol + col] = E_p dV sum over the atom's chunks (fixed order, FP64)
 template <typename T>
-__global__ void nl_coef_kernel(T *__restrict__ coef, const double *__restrict__ part, int ncol, int ptot,
+__global__ void nl_coef_kernel(T *__restrict__ coef, const double *__restrict__ part, int ncol, int ptot, int mpad,
                                const int *__restrict__ proj_atom, const int *__restrict__ proj_off,
                                const int *__restrict__ atom_chunk_off, const double *__restrict__ energy, double dV) {
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -179,7 +190,7 @@ __global__ void nl_coef_kernel(T *__restrict__ coef, const double *__restrict__ 
     const int w = p - proj_off[a];
     double s = 0.0;
     for (int it = atom_chunk_off[a]; it < atom_chunk_off[a + 1]; ++it)
-        s += part[((int64_t)it * kMaxProj + w) * ncol + col];
+        s += part[((int64_t)it * mpad + w) * ncol + col];
     coef[e] = (T)(s * dV * energy[p]);
 }
 
@@ -188,16 +199,17 @@ __global__ void nl_coef_kernel(T *__restrict__ coef, const double *__restrict__ 
 // g (logical block = point block * groups + g).  The coefficients of the (up to kScatterSlots) atoms a workgroup
 // touches are staged in LDS; each thread reads its point's entry list and
 // table rows once for the whole group.  MP: projector slots per entry (9 when
-// every atom has at most 9 projectors, l <= 2, else kMaxProj); table rows
-// keep their kMaxProj padding.
+// every atom has at most 9 projectors, l <= 2, else the table padding 16 or
+// 32); table rows are MS = 16 (MP <= 16) or 32 values apart.
 template <typename T, int NC, bool PH, int NBS, int GT, int MP>
 __global__ __launch_bounds__(kScatterBlock) void nl_scatter_kernel(
     T *__restrict__ hpsi, const T *__restrict__ coef, int64_t ng, int ncol, int ncov,
     const int *__restrict__ cov_grid, const int *__restrict__ cov_off, const int *__restrict__ cov_vox,
     const int *__restrict__ cov_slot, const int *__restrict__ block_atom_off, const int *__restrict__ block_atoms,
     const int *__restrict__ vox_atom, const int *__restrict__ nproj, const int *__restrict__ proj_off,
-    const T *__restrict__ tab16, const T *__restrict__ phase, int groups, int nblk, int per_xcd) {
+    const T *__restrict__ tabp, const T *__restrict__ phase, int groups, int nblk, int per_xcd) {
     constexpr int NG = NBS * GT;  // columns of the group
+    constexpr int MS = MP > 16 ? 32 : 16;
     __shared__ T cf_s[kScatterSlots][MP][NG];
     const int lb = xcd_block(nblk, per_xcd);
     if (lb < 0) return;  // the whole workgroup
@@ -222,7 +234,7 @@ __global__ __launch_bounds__(kScatterBlock) void nl_scatter_kernel(
     int vid = cov_vox[e0], sl = cov_slot[e0];
     T pw[MP];
 #pragma unroll
-    for (int w = 0; w < MP; ++w) pw[w] = tab16[(int64_t)vid * kMaxProj + w];
+    for (int w = 0; w < MP; ++w) pw[w] = tabp[(int64_t)vid * MS + w];
     T cs = T(1), sn = T(0);
     if constexpr (PH) {
         cs = phase[2 * (int64_t)vid];
@@ -277,7 +289,7 @@ __global__ __launch_bounds__(kScatterBlock) void nl_scatter_kernel(
         vid = cov_vox[e];
         sl = cov_slot[e];
 #pragma unroll
-        for (int w = 0; w < MP; ++w) pw[w] = tab16[(int64_t)vid * kMaxProj + w];
+        for (int w = 0; w < MP; ++w) pw[w] = tabp[(int64_t)vid * MS + w];
         if constexpr (PH) {
             cs = phase[2 * (int64_t)vid];
             sn = phase[2 * (int64_t)vid + 1];
@@ -288,8 +300,10 @@ __global__ __launch_bounds__(kScatterBlock) void nl_scatter_kernel(
         if (cg + q < ncol) hpsi[(int64_t)(cg + q) * ng + g] = hv[q];
 }
 
-// Per (atom, band group): E_nl and F = 2 sum occ E Re(conj(<p|psi>) <dp/dr|psi>).
-// out[(blockIdx.y * natoms + a) * 4 + {0: E, 1..3: F}]
+// Per (atom, band group, group of MM projectors): E_nl and
+// F = 2 sum occ E Re(conj(<p|psi>) <dp/dr|psi>) over the projectors
+// MM blockIdx.z .. MM blockIdx.z + MM - 1 of the atom.
+// out[((blockIdx.z * gridDim.y + blockIdx.y) * natoms + a) * 4 + {0: E, 1..3: F}]
 template <typename T, int MM, int NBG, int NC, bool PH>
 __global__ __launch_bounds__(NL_THREADS) void nl_force_kernel(
     double *__restrict__ out, const T *__restrict__ psi, int64_t ng, int nb, int natoms, const double *__restrict__ occ,
@@ -303,9 +317,10 @@ __global__ __launch_bounds__(NL_THREADS) void nl_force_kernel(
     const int n0 = blockIdx.y * NBG;
     const int64_t v0 = vox_off[a];
     const int V = (int)(vox_off[a + 1] - v0);
-    const int M = nproj[a];
-    const T *tabs[4] = {table + tab_off[a], dtable + tab_off[a], dtable + ttot + tab_off[a],
-                        dtable + 2 * ttot + tab_off[a]};
+    const int w0 = blockIdx.z * MM;
+    const int M = min(nproj[a] - w0, MM);  // projectors of this block (may be <= 0)
+    const int64_t t0 = tab_off[a] + (int64_t)w0 * V;
+    const T *tabs[4] = {table + t0, dtable + t0, dtable + ttot + t0, dtable + 2 * ttot + t0};
 
     T acc[4][MM][NBG][NC];
 #pragma unroll
@@ -373,12 +388,12 @@ __global__ __launch_bounds__(NL_THREADS) void nl_force_kernel(
                         for (int k = 0; k < NL_THREADS / 64; ++k) t += red[k][((q * MM + w) * NBG + b) * NC + c];
                         s[q][c] = t * dV;
                     }
-                const double eo = energy[proj_off[a] + w] * o;
+                const double eo = energy[proj_off[a] + w0 + w] * o;
                 e += (s[0][0] * s[0][0] + s[0][1] * s[0][1]) * eo;
                 for (int d = 0; d < 3; ++d) f[d] += 2.0 * (s[0][0] * s[d + 1][0] + s[0][1] * s[d + 1][1]) * eo;
             }
         }
-        double *o4 = out + ((int64_t)blockIdx.y * natoms + a) * 4;
+        double *o4 = out + (((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * natoms + a) * 4;
         o4[0] = e;
         o4[1] = f[0];
         o4[2] = f[1];
@@ -406,60 +421,64 @@ template <>
 const float *phase_of<float>(const KPoint &k) { return k.d_phase32; }
 
 template <typename T>
-const T *tab16_of(const Projectors &p);
+const T *tabp_of(const Projectors &p);
 template <>
-const double *tab16_of<double>(const Projectors &p) { return p.d_tab16_64; }
+const double *tabp_of<double>(const Projectors &p) { return p.d_tabp_64; }
 template <>
-const float *tab16_of<float>(const Projectors &p) { return p.d_tab16_32; }
+const float *tabp_of<float>(const Projectors &p) { return p.d_tabp_32; }
 
 constexpr int64_t kProjMinBlocks = 1024;  // 4 workgroups per CU
 
-template <typename T, int NC, bool PH, int GT>
-void launch_project(qm_ctx *ctx, const T *psi, int ncol, const T *phase, hipStream_t st) {
+template <typename T, int NC, bool PH, int GT, int MT>
+void launch_project(qm_ctx *ctx, double *part, const T *psi, int ncol, const T *phase, hipStream_t st) {
     const Projectors &p = ctx->proj;
     const int groups = (ncol + 16 * GT - 1) / (16 * GT);
     const int64_t nblk = (int64_t)p.nchunk * groups;
     const int per_xcd = (int)((nblk + 7) / 8);
-    hipLaunchKernelGGL((nl_project_kernel<T, NC, PH, GT>), dim3(8 * per_xcd), dim3(256), 0, st,
-                       ctx->nl_part.as<double>(), psi, ctx->ng, ncol, p.d_chunk_atom, p.d_chunk_v0, p.d_vox_off,
-                       p.d_vox_grid, tab16_of<T>(p), phase, groups, (int)nblk, per_xcd);
+    hipLaunchKernelGGL((nl_project_kernel<T, NC, PH, GT, MT>), dim3(8 * per_xcd), dim3(256), 0, st, part, psi, ctx->ng,
+                       ncol, p.d_chunk_atom, p.d_chunk_v0, p.d_vox_off, p.d_vox_grid, tabp_of<T>(p), phase, groups,
+                       (int)nblk, per_xcd);
 }
 
+// Projection coefficients of nb bands into coef (ptot x ncol, T) through the
+// per-chunk partial sums part (nchunk x mpad x ncol, FP64).  Both buffers
+// are written in full before they are read, so they need no clearing.
 template <typename T, int NC, bool PH>
-int nonlocal_project(qm_ctx *ctx, const T *psi, int nb, const T *phase, hipStream_t st) {
+int nonlocal_project(qm_ctx *ctx, T *coef, double *part, const T *psi, int nb, const T *phase, hipStream_t st) {
     const Projectors &p = ctx->proj;
     const int ncol = nb * NC;
-    QM_TRY(ctx->coef.reserve(sizeof(T) * (size_t)p.ptot * ncol));
-    QM_TRY(ctx->nl_part.reserve(sizeof(double) * (size_t)p.nchunk * kMaxProj * ncol));
-    T *coef = ctx->coef.as<T>();
     // 4 tiles of 16 columns per projection workgroup share each table read;
     // the columns beyond split over more workgroups.  A launch that would then
     // have fewer than kProjMinBlocks workgroups (a small grid or few bands:
     // 64^3 x 128 has 640) takes 1 tile per workgroup instead, for four times
     // the workgroups in flight on this latency-bound gather (2 tiles measured
     // 1-2 % slower there).  Each column's sum runs over the same voxels in the
-    // same order either way.
+    // same order either way.  Tables padded to 32 projectors (two MFMA row
+    // tiles per table read) take at most 2 column tiles, for the registers.
     const int64_t g4 = (ncol + 63) / 64;
     QM_CHECK((int64_t)p.nchunk * g4 * 4 < (int64_t)INT32_MAX / 16, QM_ERR_ARG,
              "nonlocal apply: too many bands per call");
-    if ((int64_t)p.nchunk * g4 >= kProjMinBlocks) launch_project<T, NC, PH, 4>(ctx, psi, ncol, phase, st);
-    else launch_project<T, NC, PH, 1>(ctx, psi, ncol, phase, st);
+    const bool wide = (int64_t)p.nchunk * g4 >= kProjMinBlocks;
+    if (p.mpad == 16) {
+        if (wide) launch_project<T, NC, PH, 4, 1>(ctx, part, psi, ncol, phase, st);
+        else launch_project<T, NC, PH, 1, 1>(ctx, part, psi, ncol, phase, st);
+    } else {
+        if (wide) launch_project<T, NC, PH, 2, 2>(ctx, part, psi, ncol, phase, st);
+        else launch_project<T, NC, PH, 1, 2>(ctx, part, psi, ncol, phase, st);
+    }
     QM_HIP(hipGetLastError());
     const int64_t ncoef = (int64_t)p.ptot * ncol;
-    hipLaunchKernelGGL((nl_coef_kernel<T>), dim3((unsigned)((ncoef + 255) / 256)), dim3(256), 0, st, coef,
-                       ctx->nl_part.as<double>(), ncol, p.ptot, p.d_proj_atom, p.d_proj_off, p.d_atom_chunk_off,
-                       p.d_energy, ctx->hx * ctx->hy * ctx->hz);
+    hipLaunchKernelGGL((nl_coef_kernel<T>), dim3((unsigned)((ncoef + 255) / 256)), dim3(256), 0, st, coef, part, ncol,
+                       p.ptot, p.mpad, p.d_proj_atom, p.d_proj_off, p.d_atom_chunk_off, p.d_energy,
+                       ctx->hx * ctx->hy * ctx->hz);
     QM_HIP(hipGetLastError());
     return QM_OK;
 }
 
 template <typename T, int NC, bool PH>
-int nonlocal_scatter(qm_ctx *ctx, T *hpsi, int nb, const T *phase, hipStream_t st) {
+int nonlocal_scatter(qm_ctx *ctx, T *hpsi, const T *coef, int nb, const T *phase, hipStream_t st) {
     const Projectors &p = ctx->proj;
     const int ncol = nb * NC;
-    QM_CHECK(ctx->coef.bytes >= sizeof(T) * (size_t)p.ptot * ncol, QM_ERR_STATE,
-             "nonlocal scatter: no projection coefficients for this band count");
-    const T *coef = ctx->coef.as<T>();
     // columns per scatter tile (16; 64 per thread measured 1.9x slower on
     // 64^3 x 128) and tiles per workgroup (the staged coefficients, entry lists
     // and table rows serve all of them)
@@ -473,44 +492,68 @@ int nonlocal_scatter(qm_ctx *ctx, T *hpsi, int nb, const T *phase, hipStream_t s
         hipLaunchKernelGGL((nl_scatter_kernel<T, NC, PH, NBS, SGT, 9>), dim3(8 * per_xcd), dim3(kScatterBlock), 0, st,
                            hpsi, coef, ctx->ng, ncol, p.ncov, p.d_cov_grid, p.d_cov_off, p.d_cov_vox, p.d_cov_slot,
                            p.d_block_atom_off, p.d_block_atoms, p.d_vox_atom, p.d_nproj, p.d_proj_off,
-                           tab16_of<T>(p), phase, sgroups, (int)nblk, per_xcd);
-    else
-        hipLaunchKernelGGL((nl_scatter_kernel<T, NC, PH, NBS, SGT, kMaxProj>), dim3(8 * per_xcd), dim3(kScatterBlock),
+                           tabp_of<T>(p), phase, sgroups, (int)nblk, per_xcd);
+    else if (p.max_m <= 16)
+        hipLaunchKernelGGL((nl_scatter_kernel<T, NC, PH, NBS, SGT, 16>), dim3(8 * per_xcd), dim3(kScatterBlock),
                            0, st, hpsi, coef, ctx->ng, ncol, p.ncov, p.d_cov_grid, p.d_cov_off, p.d_cov_vox,
                            p.d_cov_slot, p.d_block_atom_off, p.d_block_atoms, p.d_vox_atom, p.d_nproj, p.d_proj_off,
-                           tab16_of<T>(p), phase, sgroups, (int)nblk, per_xcd);
+                           tabp_of<T>(p), phase, sgroups, (int)nblk, per_xcd);
+    else
+        hipLaunchKernelGGL((nl_scatter_kernel<T, NC, PH, NBS, SGT, 32>), dim3(8 * per_xcd), dim3(kScatterBlock),
+                           0, st, hpsi, coef, ctx->ng, ncol, p.ncov, p.d_cov_grid, p.d_cov_off, p.d_cov_vox,
+                           p.d_cov_slot, p.d_block_atom_off, p.d_block_atoms, p.d_vox_atom, p.d_nproj, p.d_proj_off,
+                           tabp_of<T>(p), phase, sgroups, (int)nblk, per_xcd);
     QM_HIP(hipGetLastError());
     return QM_OK;
 }
 
-// which: 1 = projection coefficients, 2 = scatter, 3 = both
+// which: 1 = projection coefficients, 2 = scatter, 3 = both.  coef / part are
+// the work buffers (nonlocal_work_bytes); NULL takes the context's own, grown
+// as needed.
 template <typename T>
 int launch_nonlocal(qm_ctx *ctx, T *hpsi, const T *psi, int nb, int ncomp, const KPoint &kp, hipStream_t st,
-                    int which = 3) {
+                    int which = 3, T *coef = nullptr, double *part = nullptr) {
     const Projectors &p = ctx->proj;
     if (p.natoms == 0 || p.ptot == 0) return QM_OK;
     const bool ph = !kp.gamma;
     QM_CHECK(ncomp == 2 || !ph, QM_ERR_ARG, "real (ncomp=1) bands need a Gamma k-point");
     const T *phase = phase_of<T>(kp);
+    size_t cb = 0, pb = 0;
+    nonlocal_work_bytes(ctx, nb, ncomp, sizeof(T), &cb, &pb);
+    if (!coef && ctx->capture_pins) {  // a caller's graph holds the current work buffers
+        QM_CHECK(ctx->coef.bytes >= cb && ((which & 1) == 0 || ctx->nl_part.bytes >= pb), QM_ERR_STATE,
+                 "H.psi apply: larger nonlocal work buffers needed (more bands, or FP64 after FP32) while a "
+                 "graph holds an apply of this context; destroy it and call qm_release_captures first");
+    }
+    if (!coef) {
+        if (which & 1) QM_TRY(ctx->coef.reserve(cb));
+        QM_CHECK(ctx->coef.bytes >= cb, QM_ERR_STATE,
+                 "nonlocal scatter: no projection coefficients for this band count");
+        coef = ctx->coef.as<T>();
+    }
+    if ((which & 1) && !part) {
+        QM_TRY(ctx->nl_part.reserve(pb));
+        part = ctx->nl_part.as<double>();
+    }
     if (which & 1) {
-        if (ncomp == 1) QM_TRY((nonlocal_project<T, 1, false>(ctx, psi, nb, phase, st)));
-        else if (!ph) QM_TRY((nonlocal_project<T, 2, false>(ctx, psi, nb, phase, st)));
-        else QM_TRY((nonlocal_project<T, 2, true>(ctx, psi, nb, phase, st)));
+        if (ncomp == 1) QM_TRY((nonlocal_project<T, 1, false>(ctx, coef, part, psi, nb, phase, st)));
+        else if (!ph) QM_TRY((nonlocal_project<T, 2, false>(ctx, coef, part, psi, nb, phase, st)));
+        else QM_TRY((nonlocal_project<T, 2, true>(ctx, coef, part, psi, nb, phase, st)));
     }
     if (which & 2) {
-        if (ncomp == 1) QM_TRY((nonlocal_scatter<T, 1, false>(ctx, hpsi, nb, phase, st)));
-        else if (!ph) QM_TRY((nonlocal_scatter<T, 2, false>(ctx, hpsi, nb, phase, st)));
-        else QM_TRY((nonlocal_scatter<T, 2, true>(ctx, hpsi, nb, phase, st)));
+        if (ncomp == 1) QM_TRY((nonlocal_scatter<T, 1, false>(ctx, hpsi, coef, nb, phase, st)));
+        else if (!ph) QM_TRY((nonlocal_scatter<T, 2, false>(ctx, hpsi, coef, nb, phase, st)));
+        else QM_TRY((nonlocal_scatter<T, 2, true>(ctx, hpsi, coef, nb, phase, st)));
     }
     return QM_OK;
 }
 
 template <typename T, int MM, int NC, bool PH>
-void launch_force(qm_ctx *ctx, double *out, const T *psi, int nb, const double *d_occ, const T *phase,
+void launch_force(qm_ctx *ctx, double *out, const T *psi, int nb, const double *d_occ, const T *phase, int nz,
                   hipStream_t st) {
     constexpr int NBG = 1;
     const Projectors &p = ctx->proj;
-    dim3 grid(p.natoms, (nb + NBG - 1) / NBG);
+    dim3 grid(p.natoms, (nb + NBG - 1) / NBG, nz);
     hipLaunchKernelGGL((nl_force_kernel<T, MM, NBG, NC, PH>), grid, dim3(NL_THREADS), 0, st, out, psi, ctx->ng, nb,
                        p.natoms, d_occ, p.d_nproj, p.d_proj_off, p.d_vox_off, p.d_tab_off, p.ttot, p.d_vox_grid,
                        table_of<T>(p), dtable_of<T>(p), phase, p.d_energy, ctx->hx * ctx->hy * ctx->hz);
@@ -524,7 +567,10 @@ int nonlocal_forces(qm_ctx *ctx, const T *psi, const double *occ, int nb, int nc
     QM_CHECK(ncomp == 1 || ncomp == 2, QM_ERR_ARG, "ncomp must be 1 or 2");
     const bool ph = !kp.gamma;
     QM_CHECK(ncomp == 2 || !ph, QM_ERR_ARG, "real (ncomp=1) bands need a Gamma k-point");
-    const int ngrp = nb;  // one band per force block (see launch_force)
+    // one band per force block (see launch_force); projectors in groups of 16
+    // over blockIdx.z when an atom has more than 16
+    const int nz = p.max_m > 16 ? 2 : 1;
+    const int ngrp = nb * nz;
     const size_t out_bytes = sizeof(double) * (size_t)ngrp * p.natoms * 4;
     QM_TRY(ctx->fwork.reserve(out_bytes + sizeof(double) * nb));
     double *d_out = ctx->fwork.as<double>();
@@ -533,14 +579,14 @@ int nonlocal_forces(qm_ctx *ctx, const T *psi, const double *occ, int nb, int nc
     const T *phase = phase_of<T>(kp);
     const bool small = p.max_m <= 9;
     if (ncomp == 1) {
-        if (small) launch_force<T, 9, 1, false>(ctx, d_out, psi, nb, d_occ, phase, st);
-        else launch_force<T, 16, 1, false>(ctx, d_out, psi, nb, d_occ, phase, st);
+        if (small) launch_force<T, 9, 1, false>(ctx, d_out, psi, nb, d_occ, phase, nz, st);
+        else launch_force<T, 16, 1, false>(ctx, d_out, psi, nb, d_occ, phase, nz, st);
     } else if (!ph) {
-        if (small) launch_force<T, 9, 2, false>(ctx, d_out, psi, nb, d_occ, phase, st);
-        else launch_force<T, 16, 2, false>(ctx, d_out, psi, nb, d_occ, phase, st);
+        if (small) launch_force<T, 9, 2, false>(ctx, d_out, psi, nb, d_occ, phase, nz, st);
+        else launch_force<T, 16, 2, false>(ctx, d_out, psi, nb, d_occ, phase, nz, st);
     } else {
-        if (small) launch_force<T, 9, 2, true>(ctx, d_out, psi, nb, d_occ, phase, st);
-        else launch_force<T, 16, 2, true>(ctx, d_out, psi, nb, d_occ, phase, st);
+        if (small) launch_force<T, 9, 2, true>(ctx, d_out, psi, nb, d_occ, phase, nz, st);
+        else launch_force<T, 16, 2, true>(ctx, d_out, psi, nb, d_occ, phase, nz, st);
     }
     QM_HIP(hipGetLastError());
     std::vector<double> h((size_t)ngrp * p.natoms * 4);
@@ -585,6 +631,20 @@ int nonlocal_scatter_f64(qm_ctx *ctx, double *hpsi, int nb, int ncomp, const KPo
 }
 int nonlocal_scatter_f32(qm_ctx *ctx, float *hpsi, int nb, int ncomp, const KPoint &kp, hipStream_t st) {
     return launch_nonlocal<float>(ctx, hpsi, nullptr, nb, ncomp, kp, st, 2);
+}
+// Bytes of the two nonlocal work buffers for nb bands: the coefficients
+// (ptot x ncol, T) and the per-chunk partial sums (nchunk x mpad x ncol, FP64).
+void nonlocal_work_bytes(const qm_ctx *ctx, int nb, int ncomp, size_t tsize, size_t *coef_bytes,
+                         size_t *part_bytes) {
+    const Projectors &p = ctx->proj;
+    const size_t ncol = (size_t)nb * ncomp;
+    *coef_bytes = tsize * (size_t)p.ptot * ncol;
+    *part_bytes = sizeof(double) * (size_t)p.nchunk * p.mpad * ncol;
+}
+
+int nonlocal_apply_in_f32(qm_ctx *ctx, float *hpsi, const float *psi, int nb, int ncomp, const KPoint &kp,
+                          int which, float *coef, double *part, hipStream_t st) {
+    return launch_nonlocal<float>(ctx, hpsi, psi, nb, ncomp, kp, st, which, coef, part);
 }
 int nonlocal_forces_f64(qm_ctx *ctx, const double *psi, const double *occ, int nb, int ncomp, const KPoint &kp,
                         double *forces, double *e_nl, hipStream_t st) {

@@ -41,7 +41,11 @@ void set_error(const std::string &msg);
         if (s_ != QM_OK) return s_;       \
     } while (0)
 
-constexpr int kMaxProj = 16;
+// Projectors per atom: at most 32 (two radial projectors for each l <= 3, as
+// in W.upf: 2 x (1 + 3 + 5 + 7)).  Per-projector-set padding (Projectors::mpad)
+// is 16 when every atom has at most 16, else 32: the MFMA row tiles of the
+// projection and the table rows of the scatter are 16 wide.
+constexpr int kMaxProj = 32;
 
 // Device buffer with grow-only reallocation.
 struct DevBuf {
@@ -61,6 +65,7 @@ struct Projectors {
     int64_t vtot = 0;      // sum_a V_a
     int64_t ttot = 0;      // sum_a V_a M_a
     int max_m = 0;
+    int mpad = 16;         // padded projector count of the voxel-major tables (16 or 32)
     bool has_grad = false;
     std::vector<int> nproj, proj_off;
     std::vector<int64_t> vox_off, tab_off;
@@ -81,9 +86,9 @@ struct Projectors {
     int *d_cov_vox = nullptr;            // [vtot] global voxel ids grouped by covered point
     // atom work list for the projection launch, longest atoms first
     int *d_atom_order = nullptr;
-    // voxel-major tables padded to 16 projectors: tab16[v * 16 + w] (MFMA operand layout)
-    double *d_tab16_64 = nullptr;
-    float *d_tab16_32 = nullptr;
+    // voxel-major tables padded to mpad projectors: tabp[v * mpad + w] (MFMA operand layout)
+    double *d_tabp_64 = nullptr;
+    float *d_tabp_32 = nullptr;
     // projection work items: chunks of kProjChunk voxels of one atom
     int nchunk = 0;
     int *d_chunk_atom = nullptr, *d_chunk_v0 = nullptr;
@@ -99,6 +104,8 @@ struct Projectors {
 constexpr int kProjChunk = 512;   // voxels per projection work item (4 waves x 128)
 constexpr int kScatterBlock = 256;  // covered points per scatter workgroup
 constexpr int kScatterSlots = 8;    // atoms whose coefficients a scatter workgroup stages in LDS
+
+struct FftState;  // rocFFT plans and buffers of the spectral kinetic term (kinetic_fft.hip)
 
 struct KPoint {
     double k[3] = {0, 0, 0};
@@ -127,15 +134,32 @@ struct qm_ctx {
     int stencil_force = -1;                 // >= 0: use this candidate, no tuning
     void *blas = nullptr;                   // rocblas_handle of qm_sygvd_f64, created on first use
     qm::DevBuf eig_work;                    // qm_sygvd_f64 workspace
+    qm::DevBuf td_work;                     // two band blocks of the Taylor propagator
     // second stream of a full H.psi apply: the nonlocal projection runs on it
     // beside the stencil, joined by events (created on first use)
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // Set when an H.psi apply on this context was recorded into a caller's
+    // graph (stream capture): the graph holds the addresses of the tables,
+    // phases and work buffers, so calls that would free or move them refuse
+    // (QM_ERR_STATE) until qm_release_captures.
+    int capture_pins = 0;
+    int kinetic = QM_KINETIC_FD8;  // kinetic term of every apply (qm_set_kinetic)
+    qm::FftState *fft = nullptr;   // created on the first spectral apply
 };
 
 namespace qm {
 
 int activate(const qm_ctx *ctx);
+// H.psi of capi.hip (part: 3 = full, overwrite; 1 = local only, overwrite; 2 = nonlocal, accumulate)
+template <typename T>
+int hpsi_part(qm_ctx *ctx, T *out, const T *in, int nb, int ncomp, int ik, hipStream_t st, int part);
+// QM_ERR_STATE (with a message naming `who`) while a caller's graph holds an
+// apply of this context
+int check_unpinned(const qm_ctx *ctx, const char *who);
+// Nonzero while the library captures a graph of its own (the LOBPCG segment
+// graphs, destroyed before their call returns): applies recorded then do not pin.
+extern thread_local int g_internal_capture;
 
 // stencil.hip
 int launch_fd8_local_f64(qm_ctx *ctx, double *out, const double *in, int nb, int ncomp,
@@ -164,6 +188,11 @@ int chol_transform(void *blas, int n, const double *U, double *C, double *Ui, do
 int chol_back(void *blas, int n, int nev, const double *Ui, const double *Z, int ldz, double *Zout, int ldo);
 constexpr int kTrdPanelCols = 96;  // VW workspace columns (n x 96) of sytrd_lower
 
+// kinetic_fft.hip: out = IFFT(|G+k|^2/2 FFT psi)/N + V_loc psi (overwrites); context teardown
+template <typename T>
+int launch_fft_local(qm_ctx *ctx, T *out, const T *psi, int nb, int ncomp, const double k[3], hipStream_t st);
+void fft_destroy(qm_ctx *ctx);
+
 // nonlocal.hip
 int launch_nonlocal_f64(qm_ctx *ctx, double *hpsi, const double *psi, int nb, int ncomp,
                         const KPoint &kp, hipStream_t st);
@@ -175,6 +204,12 @@ int nonlocal_project_f64(qm_ctx *ctx, const double *psi, int nb, int ncomp, cons
 int nonlocal_project_f32(qm_ctx *ctx, const float *psi, int nb, int ncomp, const KPoint &kp, hipStream_t st);
 int nonlocal_scatter_f64(qm_ctx *ctx, double *hpsi, int nb, int ncomp, const KPoint &kp, hipStream_t st);
 int nonlocal_scatter_f32(qm_ctx *ctx, float *hpsi, int nb, int ncomp, const KPoint &kp, hipStream_t st);
+// Bytes of the nonlocal work buffers for nb bands (coefficients, per-chunk partials)
+void nonlocal_work_bytes(const qm_ctx *ctx, int nb, int ncomp, size_t tsize, size_t *coef_bytes,
+                         size_t *part_bytes);
+// launch_nonlocal_f32 with caller-provided work buffers (which: 1 project, 2 scatter, 3 both)
+int nonlocal_apply_in_f32(qm_ctx *ctx, float *hpsi, const float *psi, int nb, int ncomp, const KPoint &kp,
+                          int which, float *coef, double *part, hipStream_t st);
 int nonlocal_forces_f64(qm_ctx *ctx, const double *psi, const double *occ, int nb, int ncomp,
                         const KPoint &kp, double *forces, double *e_nl, hipStream_t st);
 int nonlocal_forces_f32(qm_ctx *ctx, const float *psi, const double *occ, int nb, int ncomp,

@@ -750,8 +750,14 @@ bool feasible_any(const qm_ctx *ctx, int cand, int ncomp) {
     return ncomp == 1 ? feasible<T, 1>(ctx, cand) : feasible<T, 2>(ctx, cand);
 }
 
+// Tuning key of a call shape.  Band counts share a key within each power-of-two
+// bucket (33..64, 65..128, ...): the fastest shape depends on the band count
+// through the launch size only, and locking, which shrinks the iterated block
+// a few bands at a time, then does not re-tune at every new count.
 int64_t tune_key(int precision, int ncomp, bool kt, int nb) {
-    return ((((int64_t)nb * 2 + (ncomp - 1)) * 2 + (kt ? 1 : 0)) << 8) | precision;
+    int64_t bucket = 1;
+    while (bucket < nb) bucket <<= 1;
+    return (((bucket * 2 + (ncomp - 1)) * 2 + (kt ? 1 : 0)) << 8) | precision;
 }
 
 // Times every candidate on the caller's buffers (each run overwrites out with

@@ -64,6 +64,12 @@ class Hamiltonian:
         ptrs = [(_ptr(a) if a is not None else None) for a in arrs]
         _lib.check(self.lib.qm_set_projectors(self.ctx, s.natoms, *ptrs))
 
+    def set_kinetic(self, mode):
+        """Kinetic term of later applies: "fd8" (8th-order finite differences,
+        the default) or "fft" (spectral, the reference's default in
+        mHamiltonianMatrix_ddm_bundle)."""
+        _lib.check(self.lib.qm_set_kinetic(self.ctx, {"fd8": 0, "fft": 1}[mode]))
+
     def set_kpoints(self, kpoints):
         k = np.ascontiguousarray(kpoints, dtype=np.float64).reshape(-1, 3)
         _lib.check(self.lib.qm_set_kpoints(self.ctx, k.shape[0], _ptr(k)))

@@ -99,6 +99,10 @@ class LOBPCG:
     def set_conditioning(self, min_pivot_ratio_fp64=1e-14, min_pivot_ratio_fp32=1e-4):
         _lib.check(self.lib.qm_lobpcg_set_conditioning(self.h, min_pivot_ratio_fp64, min_pivot_ratio_fp32))
 
+    def set_rollback_limit(self, limit=1.0e-3):
+        """|1 - |x'|^2| above which an update is rolled back (0: the first update of a call)."""
+        _lib.check(self.lib.qm_lobpcg_set_rollback_limit(self.h, float(limit)))
+
     @property
     def fallbacks(self):
         return self.lib.qm_lobpcg_fallbacks(self.h)

@@ -10,8 +10,14 @@ closed forms:
 * KB projector p_{lm}(r) = r^l exp(-r^2 / (2 * 0.7^2)) Y_lm(r_hat) for |r| < r_c,
   with real Y_lm in the reference convention (RealSphericalHarmonics.h:148-162:
   no Condon-Shortley sign; order m = -l..l, SetBlockYlm.h:666-740), written here
-  as solid-harmonic polynomials S_lm(x, y, z) = r^l Y_lm.
-* KB energies D_l = (+4, -2, +1) Ha for l = 0, 1, 2.
+  as solid-harmonic polynomials S_lm(x, y, z) = r^l Y_lm, for l <= 3.
+* With nradial = 2 each l has a second radial projector
+  r^l (r^2 / 0.7^2) exp(-r^2 / (2 * 0.7^2)) Y_lm; projectors are ordered by l,
+  then radial index, then m, the reference's order of radial projectors k and
+  m = -l_k..l_k (PseudoPotNonlocal_SvF2_kpoint.h:940-945, 960-966).  l <= 3 with
+  two radial projectors gives 32 per atom, the shape of the reference's W.upf.
+* KB energies D_l = (+4, -2, +1, -0.5) Ha for l = 0..3 (first radial projector)
+  and (-1.5, +0.75, -0.4, +0.25) Ha for the second.
 * Projector support: the voxels of the atom's cutoff sphere, stored with
   UNWRAPPED integer coordinates (the Bloch phase uses them,
   PseudoPotNonlocal_SvF2_kpoint.h:3578-3596) and ordered z, y, x so runs along x
@@ -34,7 +40,8 @@ import numpy as np
 VLOC_DEPTH = -14.0
 VLOC_SIGMA = 0.8
 KB_SIGMA = 0.7
-KB_ENERGY = (4.0, -2.0, 1.0)
+KB_ENERGY = (4.0, -2.0, 1.0, -0.5)
+KB_ENERGY2 = (-1.5, 0.75, -0.4, 0.25)  # second radial projector of each l
 _VLOC_RCUT = 10.0 * VLOC_SIGMA  # exp(-50) ~ 2e-22 of the well depth is dropped
 
 
@@ -52,6 +59,7 @@ class System:
     ncomp: int             # 1 = real Gamma-point bands, 2 = SoA complex bands
     kpoints: np.ndarray    # (nk, 3) 1/bohr
     seed: int
+    nradial: int = 1             # radial KB projectors per l (1 or 2)
     vloc: np.ndarray = None
     nproj: np.ndarray = None     # (natoms,) int32
     proj_off: np.ndarray = None  # (natoms+1,) int32
@@ -140,6 +148,11 @@ _C1 = math.sqrt(3.0 / (4.0 * math.pi))
 _C2a = math.sqrt(15.0 / (4.0 * math.pi))   # xy, yz, xz
 _C2b = math.sqrt(5.0 / (16.0 * math.pi))   # 2z^2 - x^2 - y^2
 _C2c = math.sqrt(15.0 / (16.0 * math.pi))  # x^2 - y^2
+_C3a = 0.25 * math.sqrt(35.0 / (2.0 * math.pi))  # 3x^2 y - y^3, x^3 - 3 x y^2
+_C3b = 0.5 * math.sqrt(105.0 / math.pi)          # xyz
+_C3c = 0.25 * math.sqrt(21.0 / (2.0 * math.pi))  # y (4z^2 - x^2 - y^2), x (4z^2 - x^2 - y^2)
+_C3d = 0.25 * math.sqrt(7.0 / math.pi)           # z (2z^2 - 3x^2 - 3y^2)
+_C3e = 0.25 * math.sqrt(105.0 / math.pi)         # z (x^2 - y^2)
 
 
 def solid_harmonics(lmax, x, y, z):
@@ -159,8 +172,24 @@ def solid_harmonics(lmax, x, y, z):
                (-2 * _C2b * x, -2 * _C2b * y, 4 * _C2b * z),
                (_C2a * z, zero, _C2a * x),
                (2 * _C2c * x, -2 * _C2c * y, zero)]
-    if lmax > 2:
-        raise ValueError("synthetic KB projectors are defined for l <= 2")
+    if lmax >= 3:
+        x2, y2, z2 = x * x, y * y, z * z
+        S += [_C3a * (3 * x2 * y - y2 * y),
+              _C3b * x * y * z,
+              _C3c * y * (4 * z2 - x2 - y2),
+              _C3d * z * (2 * z2 - 3 * x2 - 3 * y2),
+              _C3c * x * (4 * z2 - x2 - y2),
+              _C3e * z * (x2 - y2),
+              _C3a * (x2 * x - 3 * x * y2)]
+        dS += [(_C3a * 6 * x * y, _C3a * (3 * x2 - 3 * y2), zero),
+               (_C3b * y * z, _C3b * x * z, _C3b * x * y),
+               (-2 * _C3c * x * y, _C3c * (4 * z2 - x2 - 3 * y2), 8 * _C3c * y * z),
+               (-6 * _C3d * x * z, -6 * _C3d * y * z, _C3d * (6 * z2 - 3 * x2 - 3 * y2)),
+               (_C3c * (4 * z2 - 3 * x2 - y2), -2 * _C3c * x * y, 8 * _C3c * x * z),
+               (2 * _C3e * x * z, -2 * _C3e * y * z, _C3e * (x2 - y2)),
+               (_C3a * (3 * x2 - 3 * y2), -6 * _C3a * x * y, zero)]
+    if lmax > 3:
+        raise ValueError("synthetic KB projectors are defined for l <= 3")
     return S, dS
 
 
@@ -177,40 +206,50 @@ def sphere_voxels(R, rc, h):
     return np.stack([ix[kx], iy[ky], iz[kz]], axis=1).astype(np.int32)
 
 
-def projector_values(R, lmax, vox, h):
-    """Table (M, V) and gradient tables (3, M, V) of the KB projectors at voxels."""
+def projector_values(R, lmax, vox, h, nradial=1):
+    """Table (M, V) and gradient tables (3, M, V) of the KB projectors at voxels,
+    M = nradial (lmax + 1)^2, ordered l, radial index, m."""
     hx, hy, hz = h
     x = vox[:, 0] * hx - R[0]
     y = vox[:, 1] * hy - R[1]
     z = vox[:, 2] * hz - R[2]
     r2 = x * x + y * y + z * z
+    inv_s2 = 1.0 / KB_SIGMA ** 2
     g = np.exp(-r2 / (2.0 * KB_SIGMA ** 2))
     S, dS = solid_harmonics(lmax, x, y, z)
-    inv_s2 = 1.0 / KB_SIGMA ** 2
-    tab = np.stack([s * g for s in S])
-    dtab = np.stack([
-        np.stack([(d[0] - s * x * inv_s2) * g for s, d in zip(S, dS)]),
-        np.stack([(d[1] - s * y * inv_s2) * g for s, d in zip(S, dS)]),
-        np.stack([(d[2] - s * z * inv_s2) * g for s, d in zip(S, dS)]),
-    ])
+    xyz = (x, y, z)
+    tabs, dtabs = [], []
+    for l in range(lmax + 1):
+        for k in range(nradial):
+            # radial factor u g with u = 1 or r^2 / sigma^2; grad(S u g) = (dS u + S du - S u r / sigma^2) g
+            u = np.ones_like(r2) if k == 0 else r2 * inv_s2
+            for w in range(l * l, (l + 1) * (l + 1)):
+                s_, d_ = S[w], dS[w]
+                tabs.append(s_ * u * g)
+                dtabs.append(np.stack([(d_[c] * u + (0.0 if k == 0 else s_ * 2.0 * xyz[c] * inv_s2)
+                                        - s_ * u * xyz[c] * inv_s2) * g for c in range(3)]))
+    tab = np.stack(tabs)
+    dtab = np.stack(dtabs, axis=1)  # (3, M, V)
     return tab, dtab
 
 
 def build_projectors(s: System):
     nat = s.natoms
     h = s.h
-    nproj = np.array([(l + 1) ** 2 for l in s.lmax], dtype=np.int32)
+    nr = int(s.nradial)
+    nproj = np.array([nr * (l + 1) ** 2 for l in s.lmax], dtype=np.int32)
     vox_list, tabs, dtabs, energies = [], [], [], []
     vox_off = np.zeros(nat + 1, dtype=np.int64)
     tab_off = np.zeros(nat + 1, dtype=np.int64)
     proj_off = np.zeros(nat + 1, dtype=np.int32)
     for a in range(nat):
         vox = sphere_voxels(s.positions[a], float(s.rc[a]), h)
-        tab, dtab = projector_values(s.positions[a], int(s.lmax[a]), vox, h)
+        tab, dtab = projector_values(s.positions[a], int(s.lmax[a]), vox, h, nr)
         vox_list.append(vox)
         tabs.append(tab.ravel())          # (M, V) row-major == column-major V x M
         dtabs.append(dtab.reshape(3, -1))
-        energies.append(np.concatenate([[KB_ENERGY[l]] * (2 * l + 1) for l in range(int(s.lmax[a]) + 1)]))
+        energies.append(np.concatenate([[(KB_ENERGY, KB_ENERGY2)[k][l]] * (2 * l + 1)
+                                        for l in range(int(s.lmax[a]) + 1) for k in range(nr)]))
         vox_off[a + 1] = vox_off[a] + vox.shape[0]
         tab_off[a + 1] = tab_off[a] + tab.size
         proj_off[a + 1] = proj_off[a] + nproj[a]
@@ -234,7 +273,7 @@ def retabulate(s: System, positions) -> System:
     tabs, dtabs = [], []
     for a in range(s.natoms):
         vox = s.vox_xyz[s.vox_off[a]:s.vox_off[a + 1]]
-        tab, dtab = projector_values(t.positions[a], int(s.lmax[a]), vox, h)
+        tab, dtab = projector_values(t.positions[a], int(s.lmax[a]), vox, h, int(s.nradial))
         tabs.append(tab.ravel())
         dtabs.append(dtab.reshape(3, -1))
     t.table = np.ascontiguousarray(np.concatenate(tabs)) if s.natoms else s.table
@@ -315,11 +354,22 @@ def config(index: int, *, nbands=None, grid=None, build=True) -> System:
     return s.build() if build else s
 
 
-def small_system(n=12, L=4.0, natoms=2, lmax=2, rc=1.4, nbands=4, ncomp=1, k_red=(0.0, 0.0, 0.0), seed=7):
+def small_system(n=12, L=4.0, natoms=2, lmax=2, rc=1.4, nbands=4, ncomp=1, k_red=(0.0, 0.0, 0.0), seed=7,
+                 nradial=1):
     """Tiny periodic system for brute-force checks (dense matrices on n^3 grids)."""
     rng = np.random.default_rng(seed)
     pos = rng.uniform(0.0, L, size=(natoms, 3))
     kp = np.array([k_red]) * (2.0 * math.pi / L)
     s = System(f"small{n}", n, n, n, (L, L, L), pos, np.full(natoms, lmax, np.int32),
-               np.full(natoms, rc), nbands, ncomp, kp, seed)
+               np.full(natoms, rc), nbands, ncomp, kp, seed, nradial)
+    return s.build()
+
+
+def wlike(index: int = 0, *, nbands=None, grid=None) -> System:
+    """One of the synthetic workloads with the projector shape of the reference's
+    W.upf: two radial projectors for each l <= 3 (32 per atom) on every atom."""
+    s = config(index, nbands=nbands, grid=grid, build=False)
+    s.name += " W-like 32 projectors"
+    s.lmax = np.full(s.natoms, 3, np.int32)
+    s.nradial = 2
     return s.build()

@@ -97,7 +97,7 @@ def test_lobpcg_complex_kpoint_matches_oracle(qm, orc, torch_cuda):
 
 
 def test_lobpcg_mixed_complex_close_to_fp64(qm, torch_cuda):
-    """Complex k-point: mixed FP32/FP64 vs FP64 on the bands the FP64 run has
+    """Complex k-point toy (2 atoms, 24^3): mixed FP32/FP64 vs FP64 on the bands the FP64 run has
     converged (|H x - e x| < 1e-5): cluster-mean eigenvalues within 1e-6 Ha."""
     s = qm.system.small_system(n=24, L=5.0, natoms=2, rc=1.6, nbands=8, ncomp=2, k_red=(0.25, 0.25, 0.25), seed=9)
     X0 = s.initial_bands()
@@ -109,6 +109,125 @@ def test_lobpcg_mixed_complex_close_to_fp64(qm, torch_cuda):
     print("mixed - fp64 (Ha):", ev32 - ev64, "converged:", conv)
     assert conv[:4].all()
     assert err[conv].max() <= 1e-6
+
+
+@pytest.mark.parametrize("ik", [1, 3], ids=["k=(1/4,0,0)", "k=(1/4,1/4,1/4)"])
+def test_lobpcg_mixed_complex_c2_cell(qm, torch_cuda, ik):
+    """The C2 cell (54-atom bcc 3x3x3, a = 18) on a 40^3 grid with 64 complex
+    bands at a k-point: mixed FP32/FP64 vs FP64 cluster means within 1e-6 Ha on
+    the 54 states of the s-like band, the stated set.
+
+    64 bands take the s band whole: ARPACK on the oracle operator (k =
+    (1/4,0,0)) puts its lowest 24 states within 4 mHa, with an 8-fold degenerate
+    group at bands 6-13, so a block of 8 bands cuts through a degenerate
+    cluster and converges at the rate set by a zero gap; a block that holds the
+    whole s band has the s-p gap above it.  Inside the band, clusters split by
+    1e-8 .. 1e-3 Ha keep single Ritz vectors' residuals near 1e-4 for hundreds
+    of iterations while their eigenvalues have settled, so the set is taken as
+    converged when every FP64 cluster mean moved by < 1e-9 Ha over the last
+    call of 25 iterations."""
+    s = qm.system.config(2, nbands=64, grid=40)
+    X0 = s.initial_bands()
+    print(f"case C2 cell 40^3 64 complex bands k index {ik}: mixed vs FP64, 8 x 25 iterations", flush=True)
+    res = {}
+    for prec in (64, 32):
+        H = qm.hamiltonian.Hamiltonian(s)
+        L = qm.lobpcg.LOBPCG(H, s.nbands, precision=prec)
+        x = torch_cuda.from_numpy(X0).to("cuda", torch_cuda.float64 if prec == 64 else torch_cuda.float32)
+        evs = []
+        for c in range(8):
+            ev, rn = L.solve(x, 25, k_index=ik, first_time=(c == 0))
+            evs.append(ev)
+        res[prec] = (evs, rn, L.timings())
+        L.close()
+        H.close()
+    (ev64s, rn64, t64), (ev32s, rn32, t32) = res[64], res[32]
+    ev64, ev32 = ev64s[-1], ev32s[-1]
+    nset = 54
+    moved = qm.lobpcg.cluster_mean_errors(ev64s[-1][:nset], ev64s[-2][:nset])
+    err = qm.lobpcg.cluster_mean_errors(ev32, ev64)
+    print("FP64 sqrt residual norms of the s band:", np.array2string(np.sqrt(rn64[:nset]), precision=1))
+    print("FP64 cluster-mean change over the last call (Ha):", float(moved.max()))
+    print("per-band |mixed - fp64| (Ha):", np.array2string(np.abs(ev32 - ev64), precision=1))
+    print("cluster-mean |error| (Ha):", np.array2string(err, precision=1))
+    print("timings FP64 / mixed (ms, last call):", t64, t32)
+    assert moved.max() < 1e-9
+    assert err[:nset].max() <= 1e-6
+
+
+@pytest.mark.parametrize("cond,label", [(0.5, "P dropped"), (2.0, "S eigenbasis (SVQB)")],
+                         ids=["drop-P", "svqb"])
+def test_lobpcg_fallbacks_match_oracle(qm, orc, torch_cuda, cond, label):
+    """Subspace-solve fallbacks forced through qm_lobpcg_set_conditioning, FP64, C0:
+    - 0.5: the [X R P] overlap fails the squared-pivot test every time P is present
+      (P columns are unit but not orthogonal), so each step is redone on [X R];
+    - 2.0: every pivot test fails, so every step takes the S eigenbasis.
+    The oracle (LOBPCGRef with the same `cond`) takes the same branches;
+    eigenvalues agree to <= 1e-12 relative after 2 calls of 4 iterations."""
+    import lobpcg_ref
+    s = qm.system.config(0)
+    X0 = s.initial_bands()
+    print(f"case C0 FP64 fallback {label} (cond {cond})", flush=True)
+    Lo = lobpcg_ref.LOBPCGRef(lambda a: orc.hpsi(s, a), s.ncomp, s.dV, cond=cond)
+    Xo = X0
+    H = qm.hamiltonian.Hamiltonian(s)
+    L = qm.lobpcg.LOBPCG(H, s.nbands, precision=64)
+    L.set_conditioning(cond, 1e-4)
+    x = torch_cuda.from_numpy(X0).to("cuda", torch_cuda.float64)
+    for c in range(2):
+        Xo, ev_o, rn_o, _ = Lo.solve(Xo, 4, first_time=(c == 0))
+        ev_g, rn_g = L.solve(x, 4, first_time=(c == 0))
+        err = np.abs(ev_g - ev_o).max() / np.abs(ev_o).max()
+        print(f"call {c}: fallbacks {L.fallbacks}, max rel eigenvalue diff {err:.2e}")
+        assert L.fallbacks > 0
+        assert err <= 1e-12
+    L.close()
+    H.close()
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_lobpcg_rollback_matches_oracle(qm, orc, torch_cuda, prec):
+    """The normalisation rollback (lobpcg_z_multi_mpi.h:2611-2666) forced with a
+    rollback limit of 0: the first update of the call is rejected, the previous
+    vectors come back renormalised with their own Ritz values and residual
+    norms, P = R is kept, and the next call (limit back at 1e-3) continues from
+    there.  Against the oracle with the same limit: FP64 eigenvalues, residual
+    norms and vectors to <= 1e-12 relative; mixed (FP64 Rayleigh quotients of
+    the returned FP32 vectors) within 1e-5 relative, residual norms within 1e-3
+    relative."""
+    import lobpcg_ref
+    s = qm.system.config(0)
+    X0 = s.initial_bands()
+    print(f"case C0 rollback precision {prec}", flush=True)
+    Lo = lobpcg_ref.LOBPCGRef(lambda a: orc.hpsi(s, a), s.ncomp, s.dV)
+    H = qm.hamiltonian.Hamiltonian(s)
+    L = qm.lobpcg.LOBPCG(H, s.nbands, precision=prec)
+    dtype = torch_cuda.float64 if prec == 64 else torch_cuda.float32
+    x = torch_cuda.from_numpy(X0).to("cuda", dtype)
+    # two ordinary iterations first, so that the rolled-back call starts with P
+    Xo, _, _, _ = Lo.solve(X0, 2, first_time=True)
+    L.solve(x, 2, first_time=True)
+    Lo.limit_norm_x = 0.0
+    L.set_rollback_limit(0.0)
+    Xo, ev_o, rn_o, it_o = Lo.solve(Xo, 5, first_time=False)
+    ev_g, rn_g = L.solve(x, 5, first_time=False)
+    assert it_o == 1 and L.iterations == 1
+    tol_e, tol_r = (1e-12, 1e-12) if prec == 64 else (1e-5, 1e-3)
+    e_err = np.abs(ev_g - ev_o).max() / np.abs(ev_o).max()
+    r_err = np.abs(rn_g - rn_o).max() / np.abs(rn_o).max()
+    x_err = np.abs(x.double().cpu().numpy() - Xo).max() / np.abs(Xo).max()
+    print(f"rollback call: eigenvalues {e_err:.2e}, residual norms {r_err:.2e}, vectors {x_err:.2e}")
+    assert e_err <= tol_e and r_err <= tol_r
+    assert x_err <= (1e-12 if prec == 64 else 1e-5)
+    Lo.limit_norm_x = 1e-3
+    L.set_rollback_limit(1e-3)
+    Xo, ev_o, _, _ = Lo.solve(Xo, 4, first_time=False)
+    ev_g, _ = L.solve(x, 4, first_time=False)
+    err = np.abs(ev_g - ev_o).max() / np.abs(ev_o).max()
+    print(f"next call (P = R kept): max rel eigenvalue diff {err:.2e}")
+    assert err <= (1e-12 if prec == 64 else 1e-5)
+    L.close()
+    H.close()
 
 
 def _locked_runs(qm, orc, torch, s, X0, schedule, prec=64, ik=0):

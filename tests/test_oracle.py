@@ -86,8 +86,9 @@ def _circulant(n, coeffs, sym):
     return A
 
 
-def dense_hamiltonian(s, k):
-    """Independent dense H(k) on the grid (x fastest)."""
+def dense_hamiltonian(s, k, local=True, nonlocal_=True):
+    """Independent dense H(k) on the grid (x fastest); local=False leaves out the
+    kinetic and V_loc terms, nonlocal_=False the projectors."""
     hx, hy, hz = s.h
     n = (s.nx, s.ny, s.nz)
     h = (hx, hy, hz)
@@ -102,8 +103,10 @@ def dense_hamiltonian(s, k):
     grad = [kron3(D1[0], I[1], I[2]), kron3(I[0], D1[1], I[2]), kron3(I[0], I[1], D1[2])]
     H = -0.5 * (lap + 2j * sum(k[d] * grad[d] for d in range(3)) - float(np.dot(k, k)) * np.eye(s.ng))
     H = H + np.diag(s.vloc)
+    if not local:
+        H = np.zeros_like(H)
     # nonlocal: sum_a e^{-ik.r} P E dV P^T e^{ik.r}
-    for a in range(s.natoms):
+    for a in range(s.natoms if nonlocal_ else 0):
         v0, v1 = s.vox_off[a], s.vox_off[a + 1]
         V = v1 - v0
         M = s.nproj[a]
@@ -230,7 +233,7 @@ def _real_ylm_scipy(l, m, x, y, z):
     return math.sqrt(2.0) * cs * (Y.real if m > 0 else Y.imag)
 
 
-@pytest.mark.parametrize("l", [0, 1, 2])
+@pytest.mark.parametrize("l", [0, 1, 2, 3])
 def test_ylm_matches_scipy(orc, l):
     rng = np.random.default_rng(l)
     for _ in range(20):
@@ -240,17 +243,17 @@ def test_ylm_matches_scipy(orc, l):
 
 
 def test_ylm_orthonormal(orc):
-    # Gauss-Legendre in cos(theta) x uniform phi integrates degree <= 4 exactly
+    # Gauss-Legendre in cos(theta) x uniform phi integrates products up to l = 3 exactly
     xg, wg = np.polynomial.legendre.leggauss(12)
     phis = np.arange(24) * 2 * math.pi / 24
-    lm = [(l, m) for l in range(3) for m in range(-l, l + 1)]
-    G = np.zeros((9, 9))
+    lm = [(l, m) for l in range(4) for m in range(-l, l + 1)]
+    G = np.zeros((16, 16))
     for ct, w in zip(xg, wg):
         st = math.sqrt(1 - ct * ct)
         for ph in phis:
             v = np.array([orc.ylm(l, m, st * math.cos(ph), st * math.sin(ph), ct) for l, m in lm])
             G += w * (2 * math.pi / 24) * np.outer(v, v)
-    np.testing.assert_allclose(G, np.eye(9), atol=1e-13)
+    np.testing.assert_allclose(G, np.eye(16), atol=1e-13)
 
 
 def test_ylm_origin(orc):
@@ -259,17 +262,26 @@ def test_ylm_origin(orc):
     assert orc.ylm(2, -2, 0, 0, 0) == 0.0
 
 
-@pytest.mark.parametrize("cfg", [0, 3])
+@pytest.mark.parametrize("cfg", [0, 3, "wlike"])
 def test_projector_tables_match_oracle(qm, orc, cfg):
-    s = qm.system.config(cfg, grid=48 if cfg == 3 else None)
+    """Generator tables (solid-harmonic polynomials) against the oracle's
+    (Y_lm from the associated Legendre form, pinned to scipy above), including
+    the W-like set: l <= 3 with two radial projectors, 32 per atom, in the
+    order l, radial index, m."""
+    if cfg == "wlike":
+        s = qm.system.wlike(0, nbands=2)
+        assert (s.nproj == 32).all()
+    else:
+        s = qm.system.config(cfg, grid=48 if cfg == 3 else None)
     t = orc.projector_table(s)
     np.testing.assert_allclose(s.table, t, rtol=0, atol=1e-14)
 
 
-def test_projector_gradient_tables(qm, orc):
+@pytest.mark.parametrize("which", ["C4", "wlike"])
+def test_projector_gradient_tables(qm, orc, which):
     # analytic d p / d r of the generator vs central differences of the oracle
     # tables in the atom position (d p(r - R)/d R = -d p/d r), same voxels.
-    s = qm.system.config(4, grid=32)
+    s = qm.system.config(4, grid=32) if which == "C4" else qm.system.wlike(4, nbands=2, grid=32)
     h = 1e-5
     for d in range(3):
         dp = s.positions.copy()
@@ -352,3 +364,110 @@ def test_nonlocal_forces_finite_difference(qm, orc, ncomp, kred):
                 vals.append(orc.nonlocal_forces(s2, psi, occ, k=k, ncomp=ncomp)[0])
             fd = -(vals[0] - vals[1]) / (2 * h)
             assert abs(fd - f[a, d]) < 1e-6 * max(1.0, np.abs(f).max()), (a, d, fd, f[a, d])
+
+
+# --------------------------------------------------------------------------
+# real-time propagators
+
+def _soa(z):
+    out = np.empty((2 * z.shape[0], z.shape[1]))
+    out[0::2], out[1::2] = z.real, z.imag
+    return out
+
+
+@pytest.mark.parametrize("op", ["full", "local", "nonlocal"])
+def test_td_taylor_matches_exponential(qm, orc, op):
+    """The oracle's Taylor propagator against exp(-i dt Op) of the dense H(k)
+    (scipy expm) on an 8^3 k-point system: the error is the truncated tail,
+    O(dt^5) for order 4, so halving dt divides it by ~32.  A wrong sign or
+    factor in any of the four terms leaves an O(dt^n) error with n <= 4
+    (ratio <= 16); order 8 shrinks the error well below order 4."""
+    import scipy.linalg
+    s = qm.system.small_system(n=8, L=4.0, natoms=2, lmax=2, rc=1.4, nbands=2, ncomp=2,
+                               k_red=(0.25, 0.1, -0.3), seed=11)
+    k = s.kpoints[0]
+    Hd = dense_hamiltonian(s, k, local=op != "nonlocal", nonlocal_=op != "local")
+    rng = np.random.default_rng(5)
+    z = rng.standard_normal((2, s.ng)) + 1j * rng.standard_normal((2, s.ng))
+    dt0 = 0.2 / np.abs(np.linalg.eigvalsh(Hd)).max()
+    errs = []
+    for dt in (dt0, dt0 / 2):
+        want = z @ scipy.linalg.expm(-1j * dt * Hd).T
+        got = orc.td_taylor(s, _soa(z), dt, order=4, op=op, k=k)
+        errs.append(np.abs(got[0::2] + 1j * got[1::2] - want).max() / np.abs(want).max())
+    ratio = errs[0] / errs[1]
+    print(op, "errors", errs, "ratio", ratio)
+    assert 24.0 < ratio < 40.0
+    assert errs[0] < 1e-4
+    want = z @ scipy.linalg.expm(-1j * dt0 * Hd).T
+    got8 = orc.td_taylor(s, _soa(z), dt0, order=8, op=op, k=k)
+    e8 = np.abs(got8[0::2] + 1j * got8[1::2] - want).max() / np.abs(want).max()
+    assert e8 < 1e-3 * errs[0]
+
+
+def test_td_expv_matches_exponential(qm, orc):
+    """e^{-i V dt} psi against the exponential of the diagonal V_loc (sign pinned)."""
+    s = qm.system.small_system(n=8, L=4.0, natoms=2, nbands=2, ncomp=2, seed=3)
+    rng = np.random.default_rng(6)
+    z = rng.standard_normal((2, s.ng)) + 1j * rng.standard_normal((2, s.ng))
+    dt = 0.37
+    got = orc.td_expv(s, _soa(z), dt)
+    want = z * np.exp(-1j * dt * s.vloc)[None, :]
+    np.testing.assert_allclose(got[0::2] + 1j * got[1::2], want, rtol=0, atol=1e-14 * np.abs(want).max())
+
+
+# --------------------------------------------------------------------------
+# spectral (FFT) kinetic term
+
+@pytest.mark.parametrize("dims,kred", [((8, 8, 8), (0.0, 0.0, 0.0)), ((8, 6, 10), (0.25, -0.1, 0.4)),
+                                       ((7, 9, 6), (0.5, 0.5, -0.5))])
+def test_fft_kinetic_plane_waves(qm, orc, dims, kred):
+    """Every grid plane wave e^{2 pi i m.r / L} is an eigenvector of the
+    spectral kinetic term with eigenvalue |G_m + k|^2 / 2, G_m = 2 pi m / L and m
+    the wrapped index (i > n/2 -> i - n): the Nyquist index n/2 of an even
+    axis stays +n/2, which matters at k != 0 (the reference's
+    OperateKineticInKspace_bundle convention)."""
+    nx, ny, nz = dims
+    L = (3.0, 2.5, 3.5)
+    s = qm.system.System("pw", nx, ny, nz, L, np.zeros((0, 3)), np.zeros(0, np.int32), np.zeros(0), 1, 2,
+                         np.array([kred]) * (2.0 * np.pi / np.array(L)), 1).build()
+    k = s.kpoints[0]
+    z_ax = [np.arange(n) for n in dims]
+    for m in [(0, 0, 0), (1, 0, 0), (0, -1, 2), (nx // 2, ny // 2, nz // 2), (-1, ny // 2, 1)]:
+        ph = [np.exp(2j * np.pi * m[d] * z_ax[d] / dims[d]) for d in range(3)]
+        pw = (ph[2][:, None, None] * ph[1][None, :, None] * ph[0][None, None, :]).ravel()
+        psi = np.stack([pw.real, pw.imag])
+        out = orc.fft_kinetic(s, psi, k=k, ncomp=2)
+        got = out[0] + 1j * out[1]
+        mw = [(m[d] % dims[d]) for d in range(3)]
+        mw = [mm - dims[d] if mm * 2 > dims[d] else mm for d, mm in enumerate(mw)]
+        e = 0.5 * sum((2 * np.pi * mw[d] / L[d] + k[d]) ** 2 for d in range(3))
+        np.testing.assert_allclose(got, e * pw, rtol=0, atol=1e-11 * max(1.0, e))
+
+
+def test_fft_kinetic_real_matches_complex(qm, orc):
+    s = qm.system.small_system(n=12, natoms=0)
+    rng = np.random.default_rng(2)
+    re = rng.standard_normal((3, s.ng))
+    out_r = orc.fft_kinetic(s, re, ncomp=1)
+    cpx = np.zeros((6, s.ng))
+    cpx[0::2] = re
+    out_c = orc.fft_kinetic(s, cpx, ncomp=2)
+    np.testing.assert_allclose(out_r, out_c[0::2], rtol=0, atol=1e-12 * np.abs(out_r).max())
+    np.testing.assert_allclose(out_c[1::2], 0.0, atol=1e-12 * np.abs(out_r).max())
+
+
+def test_td_kinetic_fft_matches_plane_wave_phase(qm, orc):
+    """The kinetic propagator multiplies each plane wave by e^{-i dt |G + k|^2 / 2}."""
+    s = qm.system.small_system(n=8, L=3.0, natoms=0, nbands=1, ncomp=2, k_red=(0.25, 0.0, -0.5))
+    k = s.kpoints[0]
+    n = 8
+    ax = np.arange(n)
+    m = (3, -2, 4)
+    ph = [np.exp(2j * np.pi * m[d] * ax / n) for d in range(3)]
+    pw = (ph[2][:, None, None] * ph[1][None, :, None] * ph[0][None, None, :]).ravel()
+    dt = 0.7
+    out = orc.td_kinetic_fft(s, np.stack([pw.real, pw.imag]), dt, k=k)
+    mw = [(mm % n) - n if (mm % n) * 2 > n else (mm % n) for mm in m]
+    e = 0.5 * sum((2 * np.pi * mw[d] / 3.0 + k[d]) ** 2 for d in range(3))
+    np.testing.assert_allclose(out[0] + 1j * out[1], np.exp(-1j * dt * e) * pw, rtol=0, atol=1e-12)
