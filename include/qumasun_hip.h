@@ -88,16 +88,16 @@ int qm_set_kpoints(qm_ctx *ctx, int nk, const double *kvec);
  *                  with OperateKineticInKspace_bundle (96-130): G = 2 pi / L_d times the
  *                  wrapped index i (i > n/2 -> i - n; the Nyquist index n/2 stays +n/2).
  * Spectral applies hold the band block's spectrum and rocFFT's work buffer on
- * the context (grown on demand).  The stencil autotuner only concerns
- * QM_KINETIC_FD8. */
+ * the context (grown on demand); qm_hpsi_apply_f32_noalloc refuses them.  The
+ * stencil autotuner only concerns QM_KINETIC_FD8. */
 int qm_set_kinetic(qm_ctx *ctx, int mode);
 
 /* ---------------------------------------------------------------- H.psi */
 
-/* Stream capture.  An apply (qm_hpsi_apply_*, qm_local_apply_*, qm_nonlocal_add_*)
- * may be recorded into a caller's HIP graph.  The graph then holds the
- * addresses of this context's projector tables, phases, V_loc and its
- * nonlocal work buffers.  From the
+/* Stream capture.  An apply (qm_hpsi_apply_*, qm_local_apply_*, qm_nonlocal_add_*,
+ * qm_hpsi_apply_f32_noalloc) may be recorded into a caller's HIP graph.  The graph
+ * then holds the addresses of this context's projector tables, phases, V_loc
+ * and (except for the noalloc entry point) its nonlocal work buffers.  From the
  * capture on, every call that would free or move them returns QM_ERR_STATE:
  * qm_set_projectors, qm_set_kpoints, and an apply that needs larger work
  * buffers (more bands, or FP64 after FP32).  qm_set_vloc overwrites V_loc in
@@ -122,6 +122,23 @@ int qm_hpsi_apply_f64(qm_ctx *ctx, double *d_hpsi, const double *d_psi, int nb, 
  * across thread blocks. */
 int qm_hpsi_apply_f32(qm_ctx *ctx, float *d_hpsi, const float *d_psi, int nb, int ncomp, int ik,
                       void *stream);
+
+/* qm_hpsi_apply_f32 for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  It allocates nothing, creates no stream or
+ * event, does not synchronise and does no shape timing: the nonlocal
+ * coefficients and partial sums live in a fixed 64 MiB device array that the
+ * library holds per GPU until the process ends, and the stencil takes the
+ * shape already tuned for this call shape, else candidate 0 (same values).
+ * The work memory needed is 4*ptot*ncomp*nb (rounded up to 256) +
+ * 8*nchunk*P*ncomp*nb bytes (ptot projectors, nchunk = sum over atoms of
+ * ceil(V_a/512), P = 16, or 32 when an atom has more than 16 projectors);
+ * above 64 MiB the call returns QM_ERR_ARG: apply fewer bands per call.  Every call on one GPU, from any context or stream, shares that
+ * array, so calls that overlap in time on one GPU race; the array keeps the
+ * last call's contents, and nothing in it needs clearing between calls.
+ * Same contract and results as qm_hpsi_apply_f32 otherwise. */
+int qm_hpsi_apply_f32_noalloc(qm_ctx *ctx, float *d_hpsi, const float *d_psi, int nb, int ncomp, int ik,
+                              void *stream);
 
 /* The two halves of qm_hpsi_apply_*, for callers that assemble H.psi
  * themselves (the reference's deprecated per-band path does, qumasun_base1.matrix.h:262-291):

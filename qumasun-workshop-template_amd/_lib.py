@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(_PKG, "libqumasun_hip.so")
 # every entry point declared in include/qumasun_hip.h
 EXPORTS = (
     "qm_last_error", "qm_create", "qm_destroy", "qm_set_vloc", "qm_set_projectors", "qm_set_kpoints",
-    "qm_hpsi_apply_f64", "qm_hpsi_apply_f32", "qm_local_apply_f64", "qm_local_apply_f32",
+    "qm_hpsi_apply_f64", "qm_hpsi_apply_f32", "qm_hpsi_apply_f32_noalloc", "qm_local_apply_f64", "qm_local_apply_f32",
     "qm_nonlocal_add_f64", "qm_nonlocal_add_f32", "qm_band_density_f64", "qm_band_density_f32",
     "qm_nonlocal_forces_f64", "qm_nonlocal_forces_f32", "qm_gram_f64", "qm_gram_f32", "qm_update_f64",
     "qm_update_f32", "qm_lobpcg_create", "qm_lobpcg_destroy",
@@ -57,7 +57,7 @@ def load():
     L.qm_set_vloc.argtypes = [P, P]
     L.qm_set_projectors.argtypes = [P, I, P, P, P, P, P, P, P, P]
     L.qm_set_kpoints.argtypes = [P, I, P]
-    for n in ("qm_hpsi_apply_f64", "qm_hpsi_apply_f32", "qm_local_apply_f64",
+    for n in ("qm_hpsi_apply_f64", "qm_hpsi_apply_f32", "qm_hpsi_apply_f32_noalloc", "qm_local_apply_f64",
               "qm_local_apply_f32", "qm_nonlocal_add_f64", "qm_nonlocal_add_f32"):
         getattr(L, n).argtypes = [P, P, P, I, I, I, P]
     for n in ("qm_band_density_f64", "qm_band_density_f32"):

@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 
 #include <rocblas/rocblas.h>
@@ -473,6 +474,83 @@ int update_api(qm_ctx *ctx, const T *Z, int m, const T *C, int n, int64_t k, T *
     return update<T>(z, y, C, m, n, k, (hipStream_t)stream);
 }
 
+// ------------------------------------------------- work memory of the no-allocation apply
+// qm_hpsi_apply_f32_noalloc keeps the nonlocal coefficients and per-chunk
+// partial sums in this file-scope device array: one copy per GPU, held until
+// the process ends, shared by every context and stream on that GPU.  In host
+// code `scratch` names a host-side stand-in, not a device address, so host
+// code never uses it: qm_create resolves the address of each device's copy
+// with hipGetSymbolAddress once, with that device current, and keeps it in
+// g_scratch_ptr[device]; kernels receive that pointer.
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kScratchDevices] = {};
+
+// Resolves the current device's copy of `scratch` (once per device).
+static int resolve_scratch(int device) {
+    QM_CHECK(device >= 0 && device < kScratchDevices, QM_ERR_ARG, "qm_create: device index above 63");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return QM_OK;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    QM_CHECK(e == hipSuccess && ptr != nullptr, QM_ERR_HIP,
+             std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = ptr;
+    return QM_OK;
+}
+
+// H.psi in FP32 with every work buffer in `scratch`: no allocation, no stream
+// or event creation, no synchronisation, no shape timing.  The projection,
+// its coefficient sums, the stencil and the scatter run in that order on the
+// caller's stream; the stencil uses the shape already tuned for this call
+// shape (full or local applies), else candidate 0, which computes the same
+// values.
+static int hpsi_apply_noalloc(qm_ctx *ctx, float *hpsi, const float *psi, int nb, int ncomp, int ik, void *stream) {
+    QM_CHECK(ctx && hpsi && psi, QM_ERR_ARG, "qm_hpsi_apply_f32_noalloc: NULL argument");
+    QM_CHECK(nb >= 0 && (ncomp == 1 || ncomp == 2), QM_ERR_ARG, "qm_hpsi_apply_f32_noalloc: bad nb / ncomp");
+    QM_CHECK(ik >= 0 && ik < (int)ctx->kpts.size(), QM_ERR_ARG, "qm_hpsi_apply_f32_noalloc: k index out of range");
+    QM_CHECK(ctx->has_vloc, QM_ERR_STATE, "qm_hpsi_apply_f32_noalloc: call qm_set_vloc first");
+    QM_CHECK((const void *)hpsi != (const void *)psi, QM_ERR_ARG, "qm_hpsi_apply_f32_noalloc: output aliases input");
+    QM_CHECK(ctx->kinetic == QM_KINETIC_FD8, QM_ERR_STATE,
+             "qm_hpsi_apply_f32_noalloc: the spectral kinetic term needs work memory; select QM_KINETIC_FD8");
+    if (nb == 0) return QM_OK;
+    QM_TRY(activate(ctx));
+    void *base = g_scratch_ptr[ctx->device];
+    QM_CHECK(base != nullptr, QM_ERR_STATE, "qm_hpsi_apply_f32_noalloc: work memory not resolved for this device");
+    hipStream_t st = (hipStream_t)stream;
+    bool capturing = false;
+    QM_TRY(caller_capturing(st, &capturing));
+    if (capturing) ctx->capture_pins = 1;  // the graph reads the tables, phases and V_loc
+    const KPoint &kp = ctx->kpts[ik];
+    const bool nl = ctx->proj.natoms > 0 && ctx->proj.ptot > 0;
+    size_t cb = 0, pb = 0;
+    if (nl) nonlocal_work_bytes(ctx, nb, ncomp, sizeof(float), &cb, &pb);
+    const size_t cb_al = (cb + 255) & ~(size_t)255;
+    QM_CHECK(cb_al + pb <= kScratchBytes, QM_ERR_ARG,
+             "qm_hpsi_apply_f32_noalloc: " + std::to_string(cb_al + pb) +
+                 " bytes of nonlocal work memory needed, the limit is 67108864 (64 MiB): apply fewer bands per call");
+    float *coef = static_cast<float *>(base);
+    double *part = reinterpret_cast<double *>(static_cast<unsigned char *>(base) + cb_al);
+    const int kt = (std::fabs(kp.k[0]) >= 1.0e-8 || std::fabs(kp.k[1]) >= 1.0e-8 ||
+                    std::fabs(kp.k[2]) >= 1.0e-8) ? 1 : 0;
+    int cand = ctx->stencil_force;
+    if (cand < 0) cand = stencil_choice(ctx, 32, ncomp, kt | 2, nb);
+    if (cand < 0) cand = stencil_choice(ctx, 32, ncomp, kt, nb);
+    if (cand < 0) cand = 0;
+    struct Restore {  // the caller's stencil setting, on every return
+        qm_ctx *c;
+        int v;
+        ~Restore() { c->stencil_force = v; }
+    } restore{ctx, ctx->stencil_force};
+    ctx->stencil_force = cand;
+    if (nl) QM_TRY(nonlocal_apply_in_f32(ctx, nullptr, psi, nb, ncomp, kp, 1, coef, part, st));
+    QM_TRY(launch_fd8_local_f32(ctx, hpsi, psi, nb, ncomp, kp.k, st));
+    if (nl) QM_TRY(nonlocal_apply_in_f32(ctx, hpsi, nullptr, nb, ncomp, kp, 2, coef, nullptr, st));
+    return QM_OK;
+}
+
 }  // namespace qm
 
 using namespace qm;
@@ -490,6 +568,8 @@ int qm_create(qm_ctx **out, int nx, int ny, int nz, double hx, double hy, double
     int ndev = 0;
     QM_HIP(hipGetDeviceCount(&ndev));
     QM_CHECK(device >= 0 && device < ndev, QM_ERR_ARG, "qm_create: no such device");
+    QM_HIP(hipSetDevice(device));
+    QM_TRY(resolve_scratch(device));
     qm_ctx *c = new qm_ctx();
     c->device = device;
     c->nx = nx;
@@ -638,6 +718,11 @@ int qm_hpsi_apply_f64(qm_ctx *ctx, double *d_hpsi, const double *d_psi, int nb, 
 
 int qm_hpsi_apply_f32(qm_ctx *ctx, float *d_hpsi, const float *d_psi, int nb, int ncomp, int ik, void *stream) {
     return hpsi_apply<float>(ctx, d_hpsi, d_psi, nb, ncomp, ik, stream);
+}
+
+int qm_hpsi_apply_f32_noalloc(qm_ctx *ctx, float *d_hpsi, const float *d_psi, int nb, int ncomp, int ik,
+                              void *stream) {
+    return hpsi_apply_noalloc(ctx, d_hpsi, d_psi, nb, ncomp, ik, stream);
 }
 
 int qm_gram_f64(qm_ctx *ctx, const double *d_A, int m, const double *d_B, int n, int64_t k, double *d_C,
