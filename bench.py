@@ -218,6 +218,34 @@ def lobpcg_leg(qm, torch, H, s, psi64, calls, iters=20):
     return res
 
 
+def lobpcg_c2_leg(qm, torch, iters, ik=3):
+    """s/iteration of the 54-atom, 96^3, 256-complex-band solver at
+    k = (1/4,1/4,1/4), mixed and FP64, from the workload's seed (a warm-up call
+    of 2 iterations first); the eigenvalues are still converging, so only
+    times and the phase split are reported."""
+    s = qm.system.config(2)
+    H = qm.hamiltonian.Hamiltonian(s, device=0)
+    X0 = s.initial_bands()
+    res = {"workload": f"{s.name}: {s.natoms} atoms, {s.nx}^3 grid, {s.nbands} complex bands, k index {ik}"}
+    for prec in (32, 64):
+        L = qm.lobpcg.LOBPCG(H, s.nbands, precision=prec)
+        x = torch.from_numpy(X0).to(device="cuda:0", dtype=torch.float32 if prec == 32 else torch.float64)
+        L.solve(x, 2, k_index=ik, first_time=True)
+        L.solve(x, iters, k_index=ik, first_time=False)
+        t = L.timings()
+        res["fp64" if prec == 64 else "mixed"] = {
+            "s_per_iter": t["total"] / L.iterations / 1e3, "iterations": L.iterations,
+            "phase_ms_per_iter": {k: v / L.iterations for k, v in t.items() if k != "total"}}
+        print(f"[bench] C2 LOBPCG precision {prec}: {t['total'] / L.iterations:.1f} ms/iter", file=sys.stderr,
+              flush=True)
+        L.close()
+        del x
+        torch.cuda.empty_cache()
+    res["mixed_vs_fp64_speedup"] = res["fp64"]["s_per_iter"] / res["mixed"]["s_per_iter"]
+    H.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -231,6 +259,12 @@ def main():
                     help="larger workloads whose FP32 H.psi is also timed (comma list of 2, 3; empty = none)")
     ap.add_argument("--lobpcg-calls", type=int, default=10,
                     help="LOBPCG calls of 20 iterations per precision (0 skips the LOBPCG leg)")
+    ap.add_argument("--nonlocal-only", type=int, default=0, metavar="N",
+                    help="profiling mode: N FP32 nonlocal applies (projection, coefficient sums, scatter), then exit "
+                         "(no JSON line; for rocprofv3 counter runs)")
+    ap.add_argument("--c2-lobpcg-iters", type=int, default=5,
+                    help="iterations per precision of the 96^3 x 256 complex-band LOBPCG leg at one k-point "
+                         "(0 skips it)")
     ap.add_argument("--stencil-candidates", type=int, default=0, metavar="N",
                     help="profiling mode: N FP32 local applies with each stencil tile shape in turn, then exit "
                          "(no JSON line; for rocprofv3 counter runs)")
@@ -274,6 +308,15 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         del a, b
+    if args.nonlocal_only > 0:
+        for _ in range(3):
+            nonlocal_()
+        torch.cuda.synchronize()
+        for _ in range(args.nonlocal_only):
+            nonlocal_()
+        torch.cuda.synchronize()
+        print(f"[bench] {args.nonlocal_only} nonlocal applies done", file=sys.stderr, flush=True)
+        return
     if args.stencil_candidates > 0:
         names = H.stencil_candidates()
         for c, name in enumerate(names):
@@ -333,6 +376,7 @@ def main():
                      "traffic": measured_traffic(kernel_name, s.name.split()[0]) if args.config == 1 else None,
                      "bytes_per_launch": bytes_local, "ms_per_launch": ms_local},
         "parts_ms": {"local_fd8_v": ms_local, "nonlocal": ms_nl, "full": ms},
+        "nonlocal_projection_tiles": H.nonlocal_choice(32, nb),
         "stencil_shape_in_full_apply": (H.stencil_candidates()[H.stencil_choice(32, nb, full=True)]
                                         if H.stencil_choice(32, nb, full=True) >= 0 else "untuned"),
         "nonlocal_roofline": {"bound": "hbm", "achieved": nl_bytes / (ms_nl * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
@@ -350,6 +394,9 @@ def main():
     del x, y
     if args.lobpcg_calls > 0:
         out["lobpcg"] = lobpcg_leg(qm, torch, H, s, psi64, args.lobpcg_calls)
+    if args.c2_lobpcg_iters > 0:
+        print("[bench] LOBPCG on config 2 (one k-point)", file=sys.stderr, flush=True)
+        out["lobpcg_c2"] = lobpcg_c2_leg(qm, torch, args.c2_lobpcg_iters)
     other = {}
     for cfg in [int(c) for c in args.configs.split(",") if c.strip()]:
         print(f"[bench] H.psi on config {cfg}", file=sys.stderr, flush=True)

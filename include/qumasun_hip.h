@@ -129,7 +129,8 @@ int qm_hpsi_apply_f32(qm_ctx *ctx, float *d_hpsi, const float *d_psi, int nb, in
  * event, does not synchronise and does no shape timing: the nonlocal
  * coefficients and partial sums live in a fixed 64 MiB device array that the
  * library holds per GPU until the process ends, and the stencil takes the
- * shape already tuned for this call shape, else candidate 0 (same values).
+ * shape already tuned for this call shape, else candidate 0 (same values), and
+ * the nonlocal projection its tuned shape, else the size rule.
  * The work memory needed is 4*ptot*ncomp*nb (rounded up to 256) +
  * 8*nchunk*P*ncomp*nb bytes (ptot projectors, nchunk = sum over atoms of
  * ceil(V_a/512), P = 16, or 32 when an atom has more than 16 projectors);
@@ -204,6 +205,22 @@ int qm_td_kinetic_fft_f32(qm_ctx *ctx, float *d_psi, int nb, int ik, double dt, 
  * qumasun_symplectic.evolution.h:80-100, 140-165).  Asynchronous on `stream`. */
 int qm_td_expv_f64(qm_ctx *ctx, double *d_psi, int nb, double dt, void *stream);
 int qm_td_expv_f32(qm_ctx *ctx, float *d_psi, int nb, double dt, void *stream);
+
+/* ------------------------------------------------- nonlocal projection autotuner */
+
+/* Shape autotuner of the nonlocal projection (no counterpart in the reference,
+ * whose per-atom DGEMMs have one shape): column tiles of 16 per projection
+ * workgroup, 1, 2 or 4 (at most 2 when some atom has more than 16 projectors).
+ * The first projection of a (precision, ncomp, k type, padding, band-count
+ * bucket) shape times the candidates on the caller's buffers and keeps the
+ * fastest; it synchronises the stream once.  Every shape sums each column over
+ * the same voxels in the same order, so the choice never changes a result.
+ * Under stream capture, and in qm_hpsi_apply_f32_noalloc, an untuned shape
+ * takes the size rule (4 tiles when that leaves >= 1024 workgroups, else 1).
+ * qm_nonlocal_force   use `tiles` for every later projection, 0 = autotune;
+ * qm_nonlocal_choice  tiles in use for that shape (kt bit 0: a k-point), -1 if not tuned yet. */
+int qm_nonlocal_force(qm_ctx *ctx, int tiles);
+int qm_nonlocal_choice(qm_ctx *ctx, int precision, int ncomp, int kt, int nb, int *tiles);
 
 /* ---------------------------------------------------------------- adjoint */
 

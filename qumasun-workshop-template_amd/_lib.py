@@ -24,6 +24,7 @@ EXPORTS = (
     "qm_sygvd_f64", "qm_release_captures", "qm_lobpcg_set_rollback_limit",
     "qm_td_taylor_f64", "qm_td_taylor_f32", "qm_td_expv_f64", "qm_td_expv_f32",
     "qm_set_kinetic", "qm_td_kinetic_fft_f64", "qm_td_kinetic_fft_f32",
+    "qm_nonlocal_force", "qm_nonlocal_choice",
 )
 
 QM_OK = 0
@@ -80,6 +81,8 @@ def load():
     for n in ("qm_td_expv_f64", "qm_td_expv_f32"):
         getattr(L, n).argtypes = [P, P, I, D, P]
     L.qm_set_kinetic.argtypes = [P, I]
+    L.qm_nonlocal_force.argtypes = [P, I]
+    L.qm_nonlocal_choice.argtypes = [P, I, I, I, I, ctypes.POINTER(I)]
     for n in ("qm_td_kinetic_fft_f64", "qm_td_kinetic_fft_f32"):
         getattr(L, n).argtypes = [P, P, I, I, D, P]
     L.qm_lobpcg_set_graphs.argtypes = [P, I]

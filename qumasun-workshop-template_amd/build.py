@@ -2,7 +2,7 @@
 
     python qumasun-workshop-template_amd/build.py [--jobs N] [--force]
 
-Each csrc/*.hip file is compiled to an object under _build/ (rebuilt when it or
+Each csrc/*.hip (and host-only csrc/*.cpp) file is compiled to an object under _build/ (rebuilt when it or
 a header is newer), then linked with rocBLAS / rocSOLVER / rocFFT into
 qumasun-workshop-template_amd/libqumasun_hip.so.  The C++ consumer test
 (tests/consumer/opea_consumer.cpp, a QUMASUN-style caller of the C ABI) is
@@ -56,7 +56,7 @@ def _compile(src, force):
 
 def build(jobs: int = 8, force: bool = False, verbose: bool = True) -> str:
     os.makedirs(BUILD, exist_ok=True)
-    srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
+    srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(CSRC, "*.cpp")))
     with cf.ThreadPoolExecutor(max_workers=max(1, min(jobs, len(srcs)))) as ex:
         results = list(ex.map(lambda s: _compile(s, force), srcs))
     errors = [e for _, e in results if e]

@@ -172,6 +172,146 @@ __global__ __launch_bounds__(256) void gram_partial_kernel(ColBlocks A, ColBlock
         }
 }
 
+// Large Gram products: workgroup tile 128 x 128 of C = A^T B over a K range,
+// panels of KT rows of the 256 columns staged through LDS (double-buffered),
+// 8 waves in a 2 x 4 grid, each 64 x 32 outputs (4 x 2 MFMA 16x16 tiles).
+// FP32: KT = 32 (128 B per column per panel), FP32 accumulators flushed into
+// FP64 registers every GT_FLUSH panels; FP64: KT = 16.
+constexpr int GT_M = 128, GT_N = 128, GT_THREADS = 512;
+constexpr int GT_FLUSH = 8;  // FP32 panels (of 32 K values) between FP64 flushes: 256 K values
+
+template <typename T>
+struct GramTile {
+    static constexpr int KT = sizeof(T) == 4 ? 32 : 16;   // K rows per panel (128 B per column)
+    static constexpr int VEC = 16 / sizeof(T);           // values per 16-byte load
+    static constexpr int PAD = VEC;                      // LDS row padding (keeps 16-byte alignment)
+    static constexpr int LD = KT + PAD;                  // LDS stride of one column
+    static constexpr int LOADS = (GT_M + GT_N) * KT / VEC / GT_THREADS;  // 16-byte loads per thread per panel
+};
+
+// partial[chunk][j * m + i] = sum_{k in chunk} A[k][i] B[k][j] for the tile's
+// 128 x 128 outputs.  Requires ng % KT == 0 and kchunk % KT == 0 (16-byte
+// aligned columns and whole panels).
+template <typename T>
+__global__ __launch_bounds__(GT_THREADS) void gram_tile_kernel(ColBlocks A, ColBlocks B, int m, int n, int64_t ng,
+                                                               int64_t kchunk, int tiles_n, int tiles, int nblk,
+                                                               int per_xcd, double *__restrict__ partial) {
+    using P = GramTile<T>;
+    using M = Mfma<T>;
+    using acc_t = typename M::acc_t;
+    using vec_t = typename M::vec_t;
+    constexpr int KT = P::KT, VEC = P::VEC, LD = P::LD;
+    __shared__ __align__(16) T lds[2][GT_M + GT_N][LD];
+    const int lb = xcd_block(nblk, per_xcd);
+    if (lb < 0) return;
+    // tj fastest: the tiles of one row of tiles (sharing the A panel) are consecutive on an XCD
+    const int tile = lb % tiles;
+    const int ti = tile / tiles_n, tj = tile % tiles_n;
+    const int chunk = lb / tiles;
+    const int64_t k0 = (int64_t)chunk * kchunk;
+    const int64_t k1 = min(k0 + kchunk, ng);
+    const int npanel = (int)((k1 - k0) / KT);
+
+    // loader: thread t loads 16-byte pieces (VEC values) of columns; KT / VEC
+    // consecutive threads cover one column's panel rows (one 128-byte line)
+    constexpr int PER_COL = KT / VEC;
+    const T *src[P::LOADS];
+    int dcol[P::LOADS];
+    const int piece = threadIdx.x % PER_COL;
+#pragma unroll
+    for (int q = 0; q < P::LOADS; ++q) {
+        const int c = threadIdx.x / PER_COL + q * (GT_THREADS / PER_COL);  // 0 .. 255
+        dcol[q] = c;
+        const T *p = nullptr;
+        if (c < GT_M) {
+            const int i = ti * GT_M + c;
+            if (i < m) p = col_ptr<T>(A, i, ng);
+        } else {
+            const int j = tj * GT_N + (c - GT_M);
+            if (j < n) p = col_ptr<T>(B, j, ng);
+        }
+        src[q] = p ? p + k0 + piece * VEC : nullptr;
+    }
+    vec_t stage[P::LOADS];
+    auto fetch = [&](int panel) {
+#pragma unroll
+        for (int q = 0; q < P::LOADS; ++q)
+            stage[q] = src[q] ? *reinterpret_cast<const vec_t *>(src[q] + (int64_t)panel * KT) : vec_t{};
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < P::LOADS; ++q) *reinterpret_cast<vec_t *>(&lds[buf][dcol[q]][piece * VEC]) = stage[q];
+    };
+
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int li = lane & 15, g = lane >> 4;
+    const int wm = (wave >> 2) * 64;  // wave's first output row in the tile (2 x 4 waves)
+    const int wn = (wave & 3) * 32;   // wave's first output column in the tile
+    acc_t acc[4][2];
+    double acc64[4][2][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            acc[a][b] = acc_t{0, 0, 0, 0};
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc64[a][b][r] = 0.0;
+        }
+
+    if (npanel > 0) {
+        fetch(0);
+        store(0);
+    }
+    __syncthreads();
+    for (int p = 0; p < npanel; ++p) {
+        const int cur = p & 1;
+        if (p + 1 < npanel) fetch(p + 1);  // next panel's loads in flight during this panel's MFMAs
+        // K steps of 4 VEC values: lane group g holds k = 4g .. (VEC values per lane)
+#pragma unroll
+        for (int ks = 0; ks < KT; ks += 4 * VEC) {
+            vec_t av[4], bv[2];
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+                av[a] = *reinterpret_cast<const vec_t *>(&lds[cur][wm + 16 * a + li][ks + VEC * g]);
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+                bv[b] = *reinterpret_cast<const vec_t *>(&lds[cur][GT_M + wn + 16 * b + li][ks + VEC * g]);
+#pragma unroll
+            for (int c = 0; c < VEC; ++c)
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int b = 0; b < 2; ++b) acc[a][b] = M::mma(av[a][c], bv[b][c], acc[a][b]);
+        }
+        if constexpr (sizeof(T) == 4) {
+            if ((p + 1) % GT_FLUSH == 0) {
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int b = 0; b < 2; ++b) {
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) acc64[a][b][r] += (double)acc[a][b][r];
+                        acc[a][b] = acc_t{0, 0, 0, 0};
+                    }
+            }
+        }
+        if (p + 1 < npanel) store(cur ^ 1);  // the other buffer: last read one panel ago, before the barrier
+        __syncthreads();
+    }
+    double *out = partial + (int64_t)chunk * m * n;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int j = tj * GT_N + wn + 16 * b + li;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = ti * GT_M + wm + 16 * a + M::row(g, r);
+                if (i < m && j < n) out[(int64_t)j * m + i] = acc64[a][b][r] + (double)acc[a][b][r];
+            }
+        }
+}
+
 __global__ void gram_reduce_kernel(double *__restrict__ C, const double *__restrict__ partial, int64_t mn,
                                    int nchunk) {
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -243,6 +383,126 @@ __global__ __launch_bounds__(256) void update_kernel(ColBlocks Z, ColBlocks Y, c
 #pragma unroll
             for (int b = 0; b < 2; ++b)
                 if (pin[b]) yc[p0 + 16 * b + li] = acc[a][b][r];
+        }
+}
+
+// Large updates Y (Ng x n) = Z (Ng x m) C (m x n): workgroup tile of 128 output
+// columns x 128 grid points, 8 waves in a 2 x 4 grid (64 columns x 32 points
+// each, 4 x 2 MFMA tiles), K (= m) in panels of UT_KT rows: the panel's Z rows
+// (UT_KT columns of Z x 128 points) and C rows (UT_KT x 128) staged through LDS
+// (double-buffered), so each Z value is read from memory once per 128 output
+// columns instead of once per 64.  As the untiled kernel: Y^T = C^T Z^T, the
+// MFMA B operand and the result run along 16 consecutive grid points per lane
+// group.
+constexpr int UT_N = 128, UT_P = 128, UT_THREADS = 512;
+
+template <typename T>
+struct UpdTile {
+    static constexpr int KT = sizeof(T) == 4 ? 32 : 16;  // K rows (Z columns) per panel
+    static constexpr int VEC = 16 / sizeof(T);
+    static constexpr int ZLOADS = KT * UT_P / VEC / UT_THREADS;  // 16-byte Z loads per thread per panel
+    static constexpr int CLOADS = KT * UT_N / VEC / UT_THREADS;  // 16-byte C loads per thread per panel
+    static constexpr int PAD = 4;                                // LDS row padding (values)
+};
+
+// Requires ng % UT_P == 0 and m % KT == 0 (whole tiles and panels; the caller
+// checks) and 16-byte aligned column blocks.
+template <typename T>
+__global__ __launch_bounds__(UT_THREADS) void update_tile_kernel(ColBlocks Z, ColBlocks Y, const T *__restrict__ Cm,
+                                                                 int m, int n, int64_t ng, int tiles_n, int nblk,
+                                                                 int per_xcd) {
+    using P = UpdTile<T>;
+    using M = Mfma<T>;
+    using acc_t = typename M::acc_t;
+    using vec_t = typename M::vec_t;
+    constexpr int KT = P::KT, VEC = P::VEC;
+    __shared__ __align__(16) T zs[2][KT][UT_P + P::PAD];
+    __shared__ __align__(16) T cs[2][KT][UT_N + P::PAD];
+    const int lb = xcd_block(nblk, per_xcd);
+    if (lb < 0) return;
+    const int tj = lb % tiles_n;  // output-column tile fastest: neighbours share Z panels in one XCD's L2
+    const int64_t gt = lb / tiles_n;
+    const int c0 = tj * UT_N;
+    const int64_t p0 = gt * UT_P;
+    const int npanel = m / KT;
+
+    // Z loader: UT_P / VEC consecutive threads cover one Z column's 128 points
+    constexpr int ZPER = UT_P / VEC;
+    const int zpiece = threadIdx.x % ZPER;
+    // C loader: KT / VEC consecutive threads cover KT consecutive rows of one C column
+    constexpr int CPER = KT / VEC;
+    const int cpiece = threadIdx.x % CPER;
+    vec_t zst[P::ZLOADS], cst[P::CLOADS];
+    auto fetch = [&](int panel) {
+        const int k0 = panel * KT;
+#pragma unroll
+        for (int q = 0; q < P::ZLOADS; ++q) {
+            const int kk = threadIdx.x / ZPER + q * (UT_THREADS / ZPER);
+            const T *zc = col_ptr<T>(Z, k0 + kk, ng);
+            zst[q] = *reinterpret_cast<const vec_t *>(zc + p0 + zpiece * VEC);
+        }
+#pragma unroll
+        for (int q = 0; q < P::CLOADS; ++q) {
+            const int cc = threadIdx.x / CPER + q * (UT_THREADS / CPER);
+            cst[q] = (c0 + cc < n) ? *reinterpret_cast<const vec_t *>(Cm + (int64_t)(c0 + cc) * m + k0 + cpiece * VEC)
+                                   : vec_t{};
+        }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < P::ZLOADS; ++q) {
+            const int kk = threadIdx.x / ZPER + q * (UT_THREADS / ZPER);
+            *reinterpret_cast<vec_t *>(&zs[buf][kk][zpiece * VEC]) = zst[q];
+        }
+#pragma unroll
+        for (int q = 0; q < P::CLOADS; ++q) {
+            const int cc = threadIdx.x / CPER + q * (UT_THREADS / CPER);
+#pragma unroll
+            for (int v = 0; v < VEC; ++v) cs[buf][cpiece * VEC + v][cc] = cst[q][v];
+        }
+    };
+
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int li = lane & 15, g = lane >> 4;
+    const int wc = (wave >> 2) * 64;  // wave's first output column in the tile
+    const int wp = (wave & 3) * 32;   // wave's first grid point in the tile
+    acc_t acc[4][2];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
+
+    fetch(0);
+    store(0);
+    __syncthreads();
+    for (int p = 0; p < npanel; ++p) {
+        const int cur = p & 1;
+        if (p + 1 < npanel) fetch(p + 1);
+#pragma unroll
+        for (int ks = 0; ks < KT; ks += 4) {
+            T av[4], bv[2];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) av[a] = cs[cur][ks + g][wc + 16 * a + li];  // A[col][k = g]
+#pragma unroll
+            for (int b = 0; b < 2; ++b) bv[b] = zs[cur][ks + g][wp + 16 * b + li];  // B[k = g][point]
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) acc[a][b] = M::mma(av[a], bv[b], acc[a][b]);
+        }
+        if (p + 1 < npanel) store(cur ^ 1);
+        __syncthreads();
+    }
+    // D[row = out column 16a + row(g, r)][col = grid point 16b + li]
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int col = c0 + wc + 16 * a + M::row(g, r);
+            if (col >= n) continue;
+            T *yc = col_ptr_out<T>(Y, col, ng);
+#pragma unroll
+            for (int b = 0; b < 2; ++b) yc[p0 + wp + 16 * b + li] = acc[a][b][r];
         }
 }
 
@@ -327,9 +587,46 @@ int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
 
 }  // namespace
 
+// Gram products with both sides >= 128 columns over whole panels take the
+// LDS-tiled 128 x 128 kernel: each panel row is read once per workgroup tile
+// and shared by its 8 waves, and the tiles of one tile row (the same A panel)
+// run consecutively on one XCD.  K is split so that about 1024 workgroups run.
+template <typename T>
+int gram_tiled(DevBuf &partial, double *C, const ColBlocks &A, const ColBlocks &B, int m, int n, int64_t ng,
+               hipStream_t st) {
+    constexpr int KT = GramTile<T>::KT;
+    const int tiles_m = (m + GT_M - 1) / GT_M, tiles_n = (n + GT_N - 1) / GT_N;
+    const int tiles = tiles_m * tiles_n;
+    int64_t nchunk = std::max<int64_t>(1, std::min<int64_t>(256, 1024 / tiles));
+    int64_t kchunk = round_up((ng + nchunk - 1) / nchunk, (int64_t)KT * GT_FLUSH);
+    nchunk = (ng + kchunk - 1) / kchunk;
+    QM_TRY(partial.reserve(sizeof(double) * (size_t)nchunk * m * n));
+    const int64_t nblk = (int64_t)tiles * nchunk;
+    QM_CHECK(nblk < ((int64_t)1 << 30), QM_ERR_ARG, "gram: grid too large");
+    const int per_xcd = (int)((nblk + 7) / 8);
+    hipLaunchKernelGGL((gram_tile_kernel<T>), dim3((unsigned)(8 * per_xcd)), dim3(GT_THREADS), 0, st, A, B, m, n, ng,
+                       kchunk, tiles_n, tiles, (int)nblk, per_xcd, partial.as<double>());
+    QM_HIP(hipGetLastError());
+    const int64_t mn = (int64_t)m * n;
+    hipLaunchKernelGGL(gram_reduce_kernel, dim3((unsigned)((mn + 255) / 256)), dim3(256), 0, st, C,
+                       partial.as<double>(), mn, (int)nchunk);
+    QM_HIP(hipGetLastError());
+    return QM_OK;
+}
+
+// every column block starts on a 16-byte boundary (vector loads of whole columns)
+bool aligned16(const ColBlocks &c) {
+    for (int b = 0; b < c.nblk; ++b)
+        if (reinterpret_cast<uintptr_t>(c.ptr[b]) % 16 != 0) return false;
+    return true;
+}
+
 template <typename T>
 int gram(DevBuf &partial, double *C, const ColBlocks &A, const ColBlocks &B, int m, int n, int64_t ng,
          hipStream_t st) {
+    const bool al = aligned16(A) && aligned16(B);
+    if (al && m >= GT_M && n >= GT_N && ng % GramTile<T>::KT == 0)
+        return gram_tiled<T>(partial, C, A, B, m, n, ng, st);
     constexpr int KSTEP = 4 * Mfma<T>::VEC;
     constexpr int TN = kGramTN;  // 16-column MFMA tiles per wave along n
     const int tiles_m = (m + 63) / 64, tiles_n = (n + 32 * TN - 1) / (32 * TN);
@@ -342,7 +639,7 @@ int gram(DevBuf &partial, double *C, const ColBlocks &A, const ColBlocks &B, int
     kchunk = std::max<int64_t>(kchunk, 1024);
     nchunk = (ng + kchunk - 1) / kchunk;
     QM_TRY(partial.reserve(sizeof(double) * (size_t)nchunk * m * n));
-    const bool vec = (ng % Mfma<T>::VEC) == 0;
+    const bool vec = al && (ng % Mfma<T>::VEC) == 0;
     const int64_t nblk = (int64_t)tiles * nchunk;
     QM_CHECK(nblk < ((int64_t)1 << 30), QM_ERR_ARG, "gram: grid too large");
     const int per_xcd = (int)((nblk + 7) / 8);
@@ -363,6 +660,17 @@ int gram(DevBuf &partial, double *C, const ColBlocks &A, const ColBlocks &B, int
 
 template <typename T>
 int update(const ColBlocks &Z, const ColBlocks &Y, const T *Cm, int m, int n, int64_t ng, hipStream_t st) {
+    if (n >= UT_N && ng % UT_P == 0 && m % UpdTile<T>::KT == 0 && aligned16(Z) && aligned16(Y) &&
+        reinterpret_cast<uintptr_t>(Cm) % 16 == 0) {
+        const int tiles_n = (n + UT_N - 1) / UT_N;
+        const int64_t nblk = (ng / UT_P) * tiles_n;
+        QM_CHECK(nblk < ((int64_t)1 << 30), QM_ERR_ARG, "update: grid too large");
+        const int per_xcd = (int)((nblk + 7) / 8);
+        hipLaunchKernelGGL((update_tile_kernel<T>), dim3((unsigned)(8 * per_xcd)), dim3(UT_THREADS), 0, st, Z, Y, Cm,
+                           m, n, ng, tiles_n, (int)nblk, per_xcd);
+        QM_HIP(hipGetLastError());
+        return QM_OK;
+    }
     const int tiles_n = (n + 63) / 64;
     const int64_t gtiles = (ng + 127) / 128;
     const int64_t nblk = gtiles * tiles_n;

@@ -539,12 +539,16 @@ static int hpsi_apply_noalloc(qm_ctx *ctx, float *hpsi, const float *psi, int nb
     if (cand < 0) cand = stencil_choice(ctx, 32, ncomp, kt | 2, nb);
     if (cand < 0) cand = stencil_choice(ctx, 32, ncomp, kt, nb);
     if (cand < 0) cand = 0;
-    struct Restore {  // the caller's stencil setting, on every return
+    struct Restore {  // the caller's stencil setting and shape timing, on every return
         qm_ctx *c;
         int v;
-        ~Restore() { c->stencil_force = v; }
+        ~Restore() {
+            c->stencil_force = v;
+            c->nl_notime = false;
+        }
     } restore{ctx, ctx->stencil_force};
     ctx->stencil_force = cand;
+    ctx->nl_notime = true;  // an untuned projection shape takes the size rule
     if (nl) QM_TRY(nonlocal_apply_in_f32(ctx, nullptr, psi, nb, ncomp, kp, 1, coef, part, st));
     QM_TRY(launch_fd8_local_f32(ctx, hpsi, psi, nb, ncomp, kp.k, st));
     if (nl) QM_TRY(nonlocal_apply_in_f32(ctx, hpsi, nullptr, nb, ncomp, kp, 2, coef, nullptr, st));
@@ -779,6 +783,22 @@ int qm_nonlocal_forces_f64(qm_ctx *ctx, const double *d_psi, const double *occ, 
 int qm_nonlocal_forces_f32(qm_ctx *ctx, const float *d_psi, const double *occ, int nb, int ncomp, int ik,
                            double *forces_out, double *e_nl, void *stream) {
     return forces<float>(ctx, d_psi, occ, nb, ncomp, ik, forces_out, e_nl, stream);
+}
+
+int qm_nonlocal_force(qm_ctx *ctx, int tiles) {
+    QM_CHECK(ctx, QM_ERR_ARG, "qm_nonlocal_force: NULL context");
+    QM_CHECK(tiles == 0 || tiles == 1 || tiles == 2 || tiles == 4, QM_ERR_ARG,
+             "qm_nonlocal_force: tiles must be 0 (autotune), 1, 2 or 4");
+    ctx->nl_force = tiles;
+    return QM_OK;
+}
+
+int qm_nonlocal_choice(qm_ctx *ctx, int precision, int ncomp, int kt, int nb, int *tiles) {
+    QM_CHECK(ctx && tiles, QM_ERR_ARG, "qm_nonlocal_choice: NULL argument");
+    QM_CHECK(precision == 32 || precision == 64, QM_ERR_ARG, "qm_nonlocal_choice: precision must be 32 or 64");
+    QM_CHECK(ncomp == 1 || ncomp == 2, QM_ERR_ARG, "qm_nonlocal_choice: ncomp must be 1 or 2");
+    *tiles = nonlocal_choice(ctx, precision, ncomp, kt, nb);
+    return QM_OK;
 }
 
 int qm_release_captures(qm_ctx *ctx) {

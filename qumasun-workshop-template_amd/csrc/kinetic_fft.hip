@@ -8,13 +8,13 @@
 //   psi <- IFFT( e^{-i dt |G + k|^2 / 2} FFT psi ) / N
 // (EvolveKineticInKspace / mEvolveK_FFT, qumasun_symplectic.kinetic_fft.h:14-60, 65-140).
 //
-// GPU structure: rocFFT batched 3-D transforms straight from and to the SoA
-// band layout.  Complex bands are read as a planar (re, im) array pair with a
-// batch distance of 2 Ng, so there is no gather into an interleaved copy as in
-// the reference's GatherExchangeD2Z; real Gamma-point bands use a real-to-
-// Hermitian transform (half the spectrum).  The k-space factor (and 1/N) is one
-// elementwise kernel, and V_loc psi is added by one fused kernel after the
-// inverse transform.  Plans are cached per (precision, ncomp, band count).
+// GPU structure: rocFFT batched 3-D transforms.  Complex bands are packed
+// from the SoA (re, im) arrays into one interleaved block (the reference's
+// GatherExchangeD2Z, without the DDM exchange) and transformed in place; the
+// unpack after the inverse transform is fused with V_loc psi.  Real
+// Gamma-point bands go through a real-to-Hermitian transform (half the
+// spectrum) straight from the band block.  The k-space factor (and 1/N) is one
+// elementwise kernel.  Plans are cached per (precision, ncomp, band count).
 #include <rocfft/rocfft.h>
 
 #include <cmath>
@@ -100,6 +100,35 @@ __global__ __launch_bounds__(256) void kspace_kernel(C2<T> *__restrict__ spec, i
     }
 }
 
+// spec[n][i] = (psi_re, psi_im) of SoA complex band n
+template <typename T>
+__global__ __launch_bounds__(256) void pack_kernel(C2<T> *__restrict__ spec, const T *__restrict__ psi, int64_t ng,
+                                                   int64_t total) {
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (int64_t)gridDim.x * 256) {
+        const int64_t n = e / ng, i = e - n * ng;
+        spec[e] = C2<T>{psi[2 * n * ng + i], psi[(2 * n + 1) * ng + i]};
+    }
+}
+
+// SoA complex out from the interleaved block: out = spec (+ V psi when ADDV)
+template <typename T, bool ADDV>
+__global__ __launch_bounds__(256) void unpack_kernel(T *__restrict__ out, const C2<T> *__restrict__ spec,
+                                                     const T *__restrict__ psi, const T *__restrict__ v, int64_t ng,
+                                                     int64_t total) {
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (int64_t)gridDim.x * 256) {
+        const int64_t n = e / ng, i = e - n * ng;
+        const int64_t re = 2 * n * ng + i, im = re + ng;
+        C2<T> z = spec[e];
+        if constexpr (ADDV) {
+            const T vi = v[i];
+            z.x += vi * psi[re];
+            z.y += vi * psi[im];
+        }
+        out[re] = z.x;
+        out[im] = z.y;
+    }
+}
+
 // out[col][i] += V[i] psi[col][i] over ncol columns of Ng
 template <typename T>
 __global__ __launch_bounds__(256) void vpsi_add_kernel(T *__restrict__ out, const T *__restrict__ psi,
@@ -113,15 +142,14 @@ inline unsigned grid_of(int64_t n) {
     return (unsigned)std::min<int64_t>(std::max<int64_t>(b, 1), 1 << 20);
 }
 
-int make_plan(FftPlan &p, rocfft_transform_type type, rocfft_precision prec, const size_t len[3], size_t batch,
-              rocfft_array_type in_t, rocfft_array_type out_t, const size_t in_str[3], size_t in_dist,
-              const size_t out_str[3], size_t out_dist) {
+int make_plan(FftPlan &p, rocfft_result_placement place, rocfft_transform_type type, rocfft_precision prec,
+              const size_t len[3], size_t batch, rocfft_array_type in_t, rocfft_array_type out_t,
+              const size_t in_str[3], size_t in_dist, const size_t out_str[3], size_t out_dist) {
     rocfft_plan_description d = nullptr;
     QM_FFT(rocfft_plan_description_create(&d));
     rocfft_status st = rocfft_plan_description_set_data_layout(d, in_t, out_t, nullptr, nullptr, 3, in_str, in_dist,
                                                                3, out_str, out_dist);
-    if (st == rocfft_status_success)
-        st = rocfft_plan_create(&p.plan, rocfft_placement_notinplace, type, prec, 3, len, batch, d);
+    if (st == rocfft_status_success) st = rocfft_plan_create(&p.plan, place, type, prec, 3, len, batch, d);
     (void)rocfft_plan_description_destroy(d);
     QM_FFT(st);
     QM_FFT(rocfft_plan_get_work_buffer_size(p.plan, &p.work));
@@ -138,9 +166,9 @@ void destroy_pair(FftPair &q) {
     }
 }
 
-// The forward / inverse pair for nb bands of this shape: complex bands go
-// planar SoA -> interleaved spectrum -> planar SoA; real bands go real ->
-// Hermitian half spectrum -> real.
+// The forward / inverse pair for nb bands of this shape: complex bands are
+// transformed in place in the packed interleaved block; real bands go real ->
+// Hermitian half spectrum -> real, out of place.
 template <typename T>
 int get_pair(qm_ctx *ctx, int nb, int ncomp, FftPair **out) {
     std::call_once(g_rocfft_once, [] { (void)rocfft_setup(); });
@@ -159,19 +187,19 @@ int get_pair(qm_ctx *ctx, int nb, int ncomp, FftPair **out) {
     FftPair q;
     int rc = QM_OK;
     if (ncomp == 2) {
-        rc = make_plan(q.fwd, rocfft_transform_type_complex_forward, prec, len, nb, rocfft_array_type_complex_planar,
-                       rocfft_array_type_complex_interleaved, str, 2 * ng, str, ng);
+        rc = make_plan(q.fwd, rocfft_placement_inplace, rocfft_transform_type_complex_forward, prec, len, nb,
+                       rocfft_array_type_complex_interleaved, rocfft_array_type_complex_interleaved, str, ng, str, ng);
         if (rc == QM_OK)
-            rc = make_plan(q.inv, rocfft_transform_type_complex_inverse, prec, len, nb,
-                           rocfft_array_type_complex_interleaved, rocfft_array_type_complex_planar, str, ng, str,
-                           2 * ng);
+            rc = make_plan(q.inv, rocfft_placement_inplace, rocfft_transform_type_complex_inverse, prec, len, nb,
+                           rocfft_array_type_complex_interleaved, rocfft_array_type_complex_interleaved, str, ng, str,
+                           ng);
     } else {
         const size_t nxh = nx / 2 + 1;
         const size_t hstr[3] = {1, nxh, nxh * ny};
-        rc = make_plan(q.fwd, rocfft_transform_type_real_forward, prec, len, nb, rocfft_array_type_real,
-                       rocfft_array_type_hermitian_interleaved, str, ng, hstr, nxh * ny * nz);
+        rc = make_plan(q.fwd, rocfft_placement_notinplace, rocfft_transform_type_real_forward, prec, len, nb,
+                       rocfft_array_type_real, rocfft_array_type_hermitian_interleaved, str, ng, hstr, nxh * ny * nz);
         if (rc == QM_OK)
-            rc = make_plan(q.inv, rocfft_transform_type_real_inverse, prec, len, nb,
+            rc = make_plan(q.inv, rocfft_placement_notinplace, rocfft_transform_type_real_inverse, prec, len, nb,
                            rocfft_array_type_hermitian_interleaved, rocfft_array_type_real, hstr, nxh * ny * nz, str,
                            ng);
     }
@@ -205,18 +233,44 @@ int forward(qm_ctx *ctx, const T *psi, int nb, int ncomp, FftPair **pair, C2<T> 
              "and call qm_release_captures first");
     QM_TRY(f.spec.reserve(sizeof(C2<T>) * (size_t)pb * nb));
     C2<T> *s = f.spec.as<C2<T>>();
-    void *in[2] = {const_cast<T *>(psi), const_cast<T *>(psi) + ctx->ng};
-    void *out[1] = {s};
-    QM_TRY(execute(f, (*pair)->fwd, in, out, st));
+    if (ncomp == 2) {  // pack, then transform in place
+        const int64_t total = (int64_t)nb * ctx->ng;
+        hipLaunchKernelGGL(pack_kernel<T>, dim3(grid_of(total)), dim3(256), 0, st, s, psi, ctx->ng, total);
+        QM_HIP(hipGetLastError());
+        void *io[1] = {s};
+        QM_TRY(execute(f, (*pair)->fwd, io, nullptr, st));
+    } else {
+        void *in[1] = {const_cast<T *>(psi)};
+        void *out[1] = {s};
+        QM_TRY(execute(f, (*pair)->fwd, in, out, st));
+    }
     *spec = s;
     *per_band = pb;
     return QM_OK;
 }
 
+// Inverse transform into dst: complex bands in place and unpacked into the SoA
+// block dst (plus V_loc psi when psi is given); real bands out of place into
+// dst (V_loc psi added by the caller).
 template <typename T>
-int inverse(qm_ctx *ctx, FftPair *pair, C2<T> *spec, T *dst, hipStream_t st) {
+int inverse(qm_ctx *ctx, FftPair *pair, C2<T> *spec, T *dst, int nb, int ncomp, const T *psi, hipStream_t st) {
+    if (ncomp == 2) {
+        void *io[1] = {spec};
+        QM_TRY(execute(*ctx->fft, pair->inv, io, nullptr, st));
+        const int64_t total = (int64_t)nb * ctx->ng;
+        if (psi) {
+            const T *v = sizeof(T) == 8 ? (const T *)ctx->d_vloc64 : (const T *)ctx->d_vloc32;
+            hipLaunchKernelGGL((unpack_kernel<T, true>), dim3(grid_of(total)), dim3(256), 0, st, dst, spec, psi, v,
+                               ctx->ng, total);
+        } else {
+            hipLaunchKernelGGL((unpack_kernel<T, false>), dim3(grid_of(total)), dim3(256), 0, st, dst, spec,
+                               (const T *)nullptr, (const T *)nullptr, ctx->ng, total);
+        }
+        QM_HIP(hipGetLastError());
+        return QM_OK;
+    }
     void *in[1] = {spec};
-    void *out[2] = {dst, dst + ctx->ng};
+    void *out[1] = {dst};
     return execute(*ctx->fft, pair->inv, in, out, st);
 }
 
@@ -253,11 +307,13 @@ int launch_fft_local(qm_ctx *ctx, T *out, const T *psi, int nb, int ncomp, const
     QM_TRY(forward<T>(ctx, psi, nb, ncomp, &pair, &spec, &pb, st));
     launch_kspace<T, 0>(ctx, spec, pb, nb, ncomp, k, 0.0, st);
     QM_HIP(hipGetLastError());
-    QM_TRY(inverse<T>(ctx, pair, spec, out, st));
-    const T *v = sizeof(T) == 8 ? (const T *)ctx->d_vloc64 : (const T *)ctx->d_vloc32;
-    const int64_t total = (int64_t)nb * ncomp * ctx->ng;
-    hipLaunchKernelGGL(vpsi_add_kernel<T>, dim3(grid_of(total)), dim3(256), 0, st, out, psi, v, ctx->ng, total);
-    QM_HIP(hipGetLastError());
+    QM_TRY(inverse<T>(ctx, pair, spec, out, nb, ncomp, psi, st));
+    if (ncomp == 1) {
+        const T *v = sizeof(T) == 8 ? (const T *)ctx->d_vloc64 : (const T *)ctx->d_vloc32;
+        const int64_t total = (int64_t)nb * ctx->ng;
+        hipLaunchKernelGGL(vpsi_add_kernel<T>, dim3(grid_of(total)), dim3(256), 0, st, out, psi, v, ctx->ng, total);
+        QM_HIP(hipGetLastError());
+    }
     return QM_OK;
 }
 template int launch_fft_local<double>(qm_ctx *, double *, const double *, int, int, const double *, hipStream_t);
@@ -272,7 +328,7 @@ int fft_kinetic_step(qm_ctx *ctx, T *psi, int nb, const double k[3], double dt, 
     QM_TRY(forward<T>(ctx, psi, nb, 2, &pair, &spec, &pb, st));
     launch_kspace<T, 1>(ctx, spec, pb, nb, 2, k, dt, st);
     QM_HIP(hipGetLastError());
-    return inverse<T>(ctx, pair, spec, psi, st);
+    return inverse<T>(ctx, pair, spec, psi, nb, 2, nullptr, st);
 }
 
 }  // namespace qm

@@ -429,6 +429,14 @@ const float *tabp_of<float>(const Projectors &p) { return p.d_tabp_32; }
 
 constexpr int64_t kProjMinBlocks = 1024;  // 4 workgroups per CU
 
+// Tuning key of a projection shape; band counts share a key within a
+// power-of-two bucket (as the stencil's).
+int64_t nl_tune_key(int precision, int nc, bool ph, int mpad, int nb) {
+    int64_t bucket = 1;
+    while (bucket < nb) bucket <<= 1;
+    return ((((bucket * 4 + (nc - 1) * 2 + (ph ? 1 : 0)) * 64) + mpad) << 8) | precision;
+}
+
 template <typename T, int NC, bool PH, int GT, int MT>
 void launch_project(qm_ctx *ctx, double *part, const T *psi, int ncol, const T *phase, hipStream_t st) {
     const Projectors &p = ctx->proj;
@@ -453,19 +461,68 @@ int nonlocal_project(qm_ctx *ctx, T *coef, double *part, const T *psi, int nb, c
     // 64^3 x 128 has 640) takes 1 tile per workgroup instead, for four times
     // the workgroups in flight on this latency-bound gather (2 tiles measured
     // 1-2 % slower there).  Each column's sum runs over the same voxels in the
-    // same order either way.  Tables padded to 32 projectors (two MFMA row
-    // tiles per table read) take at most 2 column tiles, for the registers.
+    // same order either way, so every shape gives the same values.  The first
+    // projection of a shape (precision, ncomp, phases, padding, band-count
+    // bucket) times the candidates on the caller's buffers (the projection only
+    // writes its own partial sums) and keeps the fastest; tables padded to 32
+    // projectors (two MFMA row tiles per table read) take at most 2 column
+    // tiles, for the registers.
     const int64_t g4 = (ncol + 63) / 64;
     QM_CHECK((int64_t)p.nchunk * g4 * 4 < (int64_t)INT32_MAX / 16, QM_ERR_ARG,
              "nonlocal apply: too many bands per call");
-    const bool wide = (int64_t)p.nchunk * g4 >= kProjMinBlocks;
-    if (p.mpad == 16) {
-        if (wide) launch_project<T, NC, PH, 4, 1>(ctx, part, psi, ncol, phase, st);
-        else launch_project<T, NC, PH, 1, 1>(ctx, part, psi, ncol, phase, st);
-    } else {
-        if (wide) launch_project<T, NC, PH, 2, 2>(ctx, part, psi, ncol, phase, st);
-        else launch_project<T, NC, PH, 1, 2>(ctx, part, psi, ncol, phase, st);
+    auto launch = [&](int gt) {
+        if (p.mpad == 16) {
+            if (gt == 4) launch_project<T, NC, PH, 4, 1>(ctx, part, psi, ncol, phase, st);
+            else if (gt == 2) launch_project<T, NC, PH, 2, 1>(ctx, part, psi, ncol, phase, st);
+            else launch_project<T, NC, PH, 1, 1>(ctx, part, psi, ncol, phase, st);
+        } else {
+            if (gt >= 2) launch_project<T, NC, PH, 2, 2>(ctx, part, psi, ncol, phase, st);
+            else launch_project<T, NC, PH, 1, 2>(ctx, part, psi, ncol, phase, st);
+        }
+    };
+    const int max_gt = p.mpad == 16 ? 4 : 2;
+    int gt = ctx->nl_force > 0 ? std::min(ctx->nl_force, max_gt) : 0;
+    if (gt == 0) {
+        const int64_t key = nl_tune_key(sizeof(T) == 8 ? 64 : 32, NC, PH, p.mpad, nb);
+        auto it = ctx->nl_choice.find(key);
+        if (it != ctx->nl_choice.end()) {
+            gt = it->second;
+        } else {
+            hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+            QM_HIP(hipStreamIsCapturing(st, &cs));
+            if (cs != hipStreamCaptureStatusNone || g_internal_capture || ctx->nl_notime) {
+                gt = (int64_t)p.nchunk * g4 >= kProjMinBlocks ? max_gt : 1;  // no timing inside a capture
+            } else {
+                hipEvent_t ev[2] = {nullptr, nullptr};
+                QM_HIP(hipEventCreate(&ev[0]));
+                if (hipEventCreate(&ev[1]) != hipSuccess) {
+                    (void)hipEventDestroy(ev[0]);
+                    QM_CHECK(false, QM_ERR_HIP, "nonlocal autotune: hipEventCreate failed");
+                }
+                float best = 1e30f;
+                int status = QM_OK;
+                for (int cand = 1; cand <= max_gt && status == QM_OK; cand *= 2) {
+                    launch(cand);  // warm-up
+                    float ms = 0.0f;
+                    if (hipEventRecord(ev[0], st) != hipSuccess) status = QM_ERR_HIP;
+                    for (int rep = 0; rep < 3 && status == QM_OK; ++rep) launch(cand);
+                    if (status == QM_OK && hipEventRecord(ev[1], st) != hipSuccess) status = QM_ERR_HIP;
+                    if (status == QM_OK && hipEventSynchronize(ev[1]) != hipSuccess) status = QM_ERR_HIP;
+                    if (status == QM_OK && hipEventElapsedTime(&ms, ev[0], ev[1]) != hipSuccess) status = QM_ERR_HIP;
+                    if (status == QM_OK && hipGetLastError() != hipSuccess) status = QM_ERR_HIP;
+                    if (status == QM_OK && ms < best) {
+                        best = ms;
+                        gt = cand;
+                    }
+                }
+                (void)hipEventDestroy(ev[0]);
+                (void)hipEventDestroy(ev[1]);
+                QM_CHECK(status == QM_OK, QM_ERR_HIP, "nonlocal autotune: HIP failure");
+                ctx->nl_choice[key] = gt;
+            }
+        }
     }
+    launch(gt);
     QM_HIP(hipGetLastError());
     const int64_t ncoef = (int64_t)p.ptot * ncol;
     hipLaunchKernelGGL((nl_coef_kernel<T>), dim3((unsigned)((ncoef + 255) / 256)), dim3(256), 0, st, coef, part, ncol,
@@ -640,6 +697,13 @@ void nonlocal_work_bytes(const qm_ctx *ctx, int nb, int ncomp, size_t tsize, siz
     const size_t ncol = (size_t)nb * ncomp;
     *coef_bytes = tsize * (size_t)p.ptot * ncol;
     *part_bytes = sizeof(double) * (size_t)p.nchunk * p.mpad * ncol;
+}
+
+int nonlocal_choice(const qm_ctx *ctx, int precision, int ncomp, int kt, int nb) {
+    if (ctx->nl_force > 0) return ctx->nl_force;
+    const int64_t key = nl_tune_key(precision, ncomp, (kt & 1) != 0 && ncomp == 2, ctx->proj.mpad, nb);
+    auto it = ctx->nl_choice.find(key);
+    return it == ctx->nl_choice.end() ? -1 : it->second;
 }
 
 int nonlocal_apply_in_f32(qm_ctx *ctx, float *hpsi, const float *psi, int nb, int ncomp, const KPoint &kp,

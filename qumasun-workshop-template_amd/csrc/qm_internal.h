@@ -132,6 +132,9 @@ struct qm_ctx {
     qm::DevBuf nl_part;   // per-chunk projection partials (FP64)
     std::map<int64_t, int> stencil_choice;  // autotuned stencil candidate per shape
     int stencil_force = -1;                 // >= 0: use this candidate, no tuning
+    std::map<int64_t, int> nl_choice;       // autotuned column tiles per projection workgroup per shape
+    int nl_force = 0;                       // > 0: use this many column tiles, no tuning
+    bool nl_notime = false;                 // untuned projection shapes take the size rule (no timing)
     void *blas = nullptr;                   // rocblas_handle of qm_sygvd_f64, created on first use
     qm::DevBuf eig_work;                    // qm_sygvd_f64 workspace
     qm::DevBuf td_work;                     // two band blocks of the Taylor propagator
@@ -204,6 +207,8 @@ int nonlocal_project_f64(qm_ctx *ctx, const double *psi, int nb, int ncomp, cons
 int nonlocal_project_f32(qm_ctx *ctx, const float *psi, int nb, int ncomp, const KPoint &kp, hipStream_t st);
 int nonlocal_scatter_f64(qm_ctx *ctx, double *hpsi, int nb, int ncomp, const KPoint &kp, hipStream_t st);
 int nonlocal_scatter_f32(qm_ctx *ctx, float *hpsi, int nb, int ncomp, const KPoint &kp, hipStream_t st);
+// Autotuned column tiles per projection workgroup for a shape (kt bit 0: a k-point), -1 if untuned
+int nonlocal_choice(const qm_ctx *ctx, int precision, int ncomp, int kt, int nb);
 // Bytes of the nonlocal work buffers for nb bands (coefficients, per-chunk partials)
 void nonlocal_work_bytes(const qm_ctx *ctx, int nb, int ncomp, size_t tsize, size_t *coef_bytes,
                          size_t *part_bytes);

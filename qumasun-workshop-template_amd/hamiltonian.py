@@ -119,6 +119,18 @@ class Hamiltonian:
                                               ctypes.byref(out)))
         return out.value
 
+    # -- nonlocal projection shape autotuner ---------------------------------
+    def nonlocal_force(self, tiles):
+        """Column tiles (1, 2, 4) per projection workgroup for later applies (0: autotune)."""
+        _lib.check(self.lib.qm_nonlocal_force(self.ctx, tiles))
+
+    def nonlocal_choice(self, precision, nb, ncomp=None, kt=False):
+        """Column tiles in use for this projection shape (-1 if not tuned yet)."""
+        ncomp = self.sys.ncomp if ncomp is None else ncomp
+        out = ctypes.c_int()
+        _lib.check(self.lib.qm_nonlocal_choice(self.ctx, precision, ncomp, int(kt), nb, ctypes.byref(out)))
+        return out.value
+
     # -- dense products (the blas_DGEMM_t / blas_DGEMM_n sub-boundary) ------
     def gram(self, A, B):
         """C = A^T B (FP64) for column-major A (k x m) and B (k x n) given as

@@ -9,7 +9,12 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [(16, 16, 4096), (24, 48, 32768), (7, 5, 1000), (96, 64, 262144), (130, 70, 20160), (64, 192, 3)]
+SHAPES = [(16, 16, 4096), (24, 48, 32768), (7, 5, 1000), (96, 64, 262144), (130, 70, 20160), (64, 192, 3),
+          # both sides >= 128 columns and K a multiple of the panel: the LDS-tiled kernel
+          # (the 64^3 x 128 LOBPCG block, partial tiles, a K split over few tiles)
+          (384, 256, 262144), (130, 200, 20160), (512, 384, 110592),
+          # the same sizes with K off the panel size: the untiled kernel
+          (130, 200, 20161)]
 
 
 @pytest.mark.parametrize("m,n,k", SHAPES)
@@ -33,7 +38,32 @@ def test_gram(qm, torch_cuda, m, n, k, prec):
     H.close()
 
 
-@pytest.mark.parametrize("m,n,k", [(48, 16, 32768), (72, 48, 20160), (9, 130, 1000), (384, 256, 262144)])
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_gram_unaligned_columns(qm, torch_cuda, prec):
+    """Column blocks that do not start on a 16-byte boundary take the scalar-load
+    path (A as a view starting one element into its buffer)."""
+    torch = torch_cuda
+    H = qm.hamiltonian.Hamiltonian(qm.system.small_system(n=8, natoms=0))
+    m, n, k = 160, 144, 4096
+    rng = np.random.default_rng(7)
+    dt = torch.float64 if prec == "f64" else torch.float32
+    buf = torch.from_numpy(rng.standard_normal(m * k + 1)).to("cuda", dt)
+    a = buf[1:].view(m, k)
+    b = torch.from_numpy(rng.standard_normal((n, k))).to("cuda", dt)
+    print(f"case gram unaligned {prec}", flush=True)
+    C = H.gram(a, b).cpu().numpy()
+    Ar, Br = a.double().cpu().numpy(), b.double().cpu().numpy()
+    scale = np.abs(Ar) @ np.abs(Br).T
+    tol = 1e-13 if prec == "f64" else 1e-6
+    assert (np.abs(C - Ar @ Br.T) <= tol * scale).all()
+    H.close()
+
+
+@pytest.mark.parametrize("m,n,k", [(48, 16, 32768), (72, 48, 20160), (9, 130, 1000), (384, 256, 262144),
+                                   # LDS-tiled kernel: n >= 128, K (= m) a multiple of the panel, whole point tiles
+                                   (96, 200, 20480), (512, 384, 110592),
+                                   # untiled: m off the panel size
+                                   (100, 200, 20480)])
 @pytest.mark.parametrize("prec", ["f64", "f32"])
 def test_update(qm, torch_cuda, m, n, k, prec):
     torch = torch_cuda

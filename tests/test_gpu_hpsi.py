@@ -363,3 +363,36 @@ def test_hpsi_graph_capture_replays(qm, torch_cuda, label, build, dtype_name):
         torch.cuda.synchronize()
         print(f"case {label} replay {rep}", flush=True)
         assert torch.equal(y, want), (label, rep)
+
+
+@pytest.mark.parametrize("label,build,ik", [
+    ("C0 32^3 70 real bands", lambda qm: qm.system.config(0, nbands=70), 0),
+    ("C2 48^3 24 complex bands k", lambda qm: qm.system.config(2, nbands=24, grid=48), 3),
+    ("C0 32^3 W-like 40 bands (32 projectors)", lambda qm: qm.system.wlike(0, nbands=40), 0),
+], ids=["C0-real", "C2-k", "W-like"])
+def test_nonlocal_shapes_bit_identical(qm, torch_cuda, label, build, ik):
+    """Every projection shape the nonlocal autotuner may pick (1, 2 or 4 column
+    tiles per workgroup) gives the same bits, and the autotuner records its
+    choice for the shape."""
+    torch = torch_cuda
+    s = build(qm)
+    H = qm.hamiltonian.Hamiltonian(s)
+    x = torch.from_numpy(s.initial_bands()).to("cuda", torch.float32)
+    outs = []
+    for tiles in (1, 2, 4):
+        print(f"case {label}: {tiles} column tiles", flush=True)
+        H.nonlocal_force(tiles)
+        y = torch.full_like(x, float("nan"))
+        H.apply(y, x, k_index=ik)
+        outs.append(y)
+    torch.cuda.synchronize()
+    for y in outs[1:]:
+        assert torch.equal(y, outs[0])
+    H.nonlocal_force(0)
+    y = torch.full_like(x, float("nan"))
+    H.apply(y, x, k_index=ik)
+    torch.cuda.synchronize()
+    assert torch.equal(y, outs[0])
+    kt = bool(np.any(np.abs(s.kpoints[ik]) >= 1e-8))
+    assert H.nonlocal_choice(32, s.nbands, kt=kt) in (1, 2, 4)
+    H.close()
