@@ -361,6 +361,45 @@ int qm_lobpcg_graph_replays(qm_lobpcg *s);
  * t[0] H.psi, t[1] Gram, t[2] subspace eigensolve, t[3] updates, t[4] other, t[5] total. */
 int qm_lobpcg_timings(qm_lobpcg *s, double *t6);
 
+/* ------------------------------------------------- UPF pseudopotentials (host) */
+
+/* A norm-conserving UPF v2 pseudopotential's nonlocal part, loaded on the host.
+ * Replaces the projector fields LoadPseudoPotential_UPF_ONCV fills in
+ * PseudoPot_MBK (upf_oncv_loader.h:355-530, pp_struct.h:5-96): number_of_proj
+ * radial projectors beta_k with angular momentum l_k <= 3 and cutoff radius,
+ * KB energies D_kk / 2 (Ry -> Ha), and z_valence.  The radial functions
+ * f_k = beta_k / r are resampled as the reference does: onto a 1000-point
+ * logarithmic grid from the uniform UPF mesh (UniformGrid::GetValue), then
+ * evaluated by RadialGrid2::GetValueByXi, zero beyond the largest cutoff of
+ * that l.  At most 32 projectors per atom (sum of 2 l_k + 1).  The mesh must be
+ * uniform (ONCV meshes are).  These calls touch no GPU. */
+typedef struct qm_upf qm_upf;
+int qm_upf_load(qm_upf **out, const char *path);
+void qm_upf_free(qm_upf *u);
+/* radial projector count, projectors per atom (sum of 2 l + 1), valence
+ * charge and the largest cutoff radius (bohr); any pointer may be NULL */
+int qm_upf_info(const qm_upf *u, int *nradial, int *nproj, double *z_valence, double *cutoff);
+/* projector k: l, its cutoff_radius and KB energy (Ha); pointers may be NULL */
+int qm_upf_radial(const qm_upf *u, int k, int *l, double *cutoff, double *energy);
+/* f_k(r) and df_k/dr (df may be NULL) at n radii r (bohr) */
+int qm_upf_eval(const qm_upf *u, int k, int n, const double *r, double *f, double *df);
+/* Projector tables in the qm_set_projectors layout for natoms atoms of this
+ * species at positions[3a..3a+2] (bohr) on a grid of spacing hx, hy, hz: every
+ * grid point (unwrapped integer coordinates ix*hx, ...) strictly within the
+ * largest cutoff of an atom, ordered z, y, x; projectors in the reference's
+ * order k, m = -l..l (num_proj_RY, PseudoPotNonlocal_SvF2_kpoint.h:940-966)
+ * valued f_k(|r|) Y_lm(r/|r|) (real Y_lm without the Condon-Shortley sign,
+ * RealSphericalHarmonics.h:148-162), r = grid point - atom, and the gradient
+ * tables dtable (may be NULL).  counts[0..2] receive the voxel, table and
+ * projector totals; with any output array NULL only counts are written, so a
+ * caller sizes its arrays with a first call.  nx, ny, nz are checked but the
+ * coordinates are not wrapped (qm_set_projectors wraps them).  The reference
+ * filters its tables in Fourier space on a refined subgrid
+ * (..._kpoint.h:911-1178, 1912-2208); these are the unfiltered point values. */
+int qm_upf_tables(const qm_upf *u, int natoms, const double *positions, int nx, int ny, int nz, double hx,
+                  double hy, double hz, int64_t *counts, int *nproj, int64_t *vox_off, int *vox_xyz,
+                  double *table, double *dtable, int64_t *tab_off, double *energy, int *proj_off);
+
 #ifdef __cplusplus
 }
 #endif

@@ -25,6 +25,7 @@ EXPORTS = (
     "qm_td_taylor_f64", "qm_td_taylor_f32", "qm_td_expv_f64", "qm_td_expv_f32",
     "qm_set_kinetic", "qm_td_kinetic_fft_f64", "qm_td_kinetic_fft_f32",
     "qm_nonlocal_force", "qm_nonlocal_choice",
+    "qm_upf_load", "qm_upf_free", "qm_upf_info", "qm_upf_radial", "qm_upf_eval", "qm_upf_tables",
 )
 
 QM_OK = 0
@@ -93,8 +94,15 @@ def load():
     L.qm_stencil_choice.argtypes = [P, I, I, I, I, ctypes.POINTER(I)]
     L.qm_sygvd_f64.argtypes = [P, I, P, P, P, P, P, I, P]
     L.qm_release_captures.argtypes = [P]
+    L.qm_upf_load.argtypes = [ctypes.POINTER(P), ctypes.c_char_p]
+    L.qm_upf_free.argtypes = [P]
+    L.qm_upf_free.restype = None
+    L.qm_upf_info.argtypes = [P, ctypes.POINTER(I), ctypes.POINTER(I), ctypes.POINTER(D), ctypes.POINTER(D)]
+    L.qm_upf_radial.argtypes = [P, I, ctypes.POINTER(I), ctypes.POINTER(D), ctypes.POINTER(D)]
+    L.qm_upf_eval.argtypes = [P, I, I, P, P, P]
+    L.qm_upf_tables.argtypes = [P, I, P, I, I, I, D, D, D, P, P, P, P, P, P, P, P, P]
     for n in EXPORTS:
-        if n not in ("qm_last_error", "qm_destroy", "qm_lobpcg_destroy"):
+        if n not in ("qm_last_error", "qm_destroy", "qm_lobpcg_destroy", "qm_upf_free"):
             getattr(L, n).restype = I
     _lib = L
     return L
