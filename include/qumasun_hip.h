@@ -342,6 +342,18 @@ int qm_lobpcg_fallbacks(qm_lobpcg *s);
  * case.  limit = 0 rolls back the first update of a call. */
 int qm_lobpcg_set_rollback_limit(qm_lobpcg *s, double limit);
 
+/* Route of the subspace eigensolve's tridiagonal reduction, process-wide (no
+ * reference counterpart: LAPACK / rocSOLVER's sytrd and hetrd are the
+ * reference's).  By default (-1) the real and complex solves of
+ * qm_lobpcg_solve and qm_sygvd_f64 reduce the subspace matrix with G
+ * cooperating workgroups that hold it in LDS (one grid barrier per column),
+ * where it fits (real n up to ~1700, complex n up to ~1200) and the device
+ * accepts cooperative launches; otherwise the single-workgroup panel
+ * reduction (real n <= 512) or rocSOLVER syevd / heevd.  0 never takes the
+ * cooperative route; 1 takes it wherever it fits.  Results agree to rounding
+ * (different summation orders). */
+int qm_subspace_force(int route);
+
 /* HIP graphs for the iteration segments that hold no host decision (no
  * reference counterpart: the reference launches every kernel from the host).
  * Within one qm_lobpcg_solve call, A.R and the Gram products of an iteration
@@ -360,6 +372,54 @@ int qm_lobpcg_graph_replays(qm_lobpcg *s);
 /* Milliseconds spent in the last qm_lobpcg_solve per phase:
  * t[0] H.psi, t[1] Gram, t[2] subspace eigensolve, t[3] updates, t[4] other, t[5] total. */
 int qm_lobpcg_timings(qm_lobpcg *s, double *t6);
+
+/* ------------------------------------------------------------ SCF glue */
+
+/* The steps of QUMASUN_KPOINT::mSCF's loop (qumasun_kpoint.scfloop.h:96-222)
+ * between one LOBPCG solve and the next: occupations, the band density
+ * (qm_band_density_*), density mixing, and the new potential
+ * V_tot = V_ion + V_H + V_xc (qm_set_vloc).  Fields are FP64 device arrays of
+ * Ng points in the grid order. */
+
+/* V_H from the FFT Poisson solve, V_H(G) = 4 pi rho(G) / |G|^2 with the G = 0
+ * term dropped (SetPotentialByPoissonFFT, poisson_fft.h:16-120; the
+ * reference's refined-grid up/down conversion is not done).  Uses the
+ * context's rocFFT plans and spectrum buffer (ordered with its applies). */
+int qm_hartree_f64(qm_ctx *ctx, const double *d_rho, double *d_vh, void *stream);
+
+/* LDA exchange-correlation (Calc_XC_LDA, Vxc.h:17-76: Perdew-Zunger 1981):
+ * d_vxc = V_x + V_c per point, and, if d_exc is not NULL, the energy density
+ * rho (e_x + e_c).  Points with rho < 1e-8 get zeros, as in the reference. */
+int qm_xc_lda_f64(qm_ctx *ctx, const double *d_rho, double *d_vxc, double *d_exc, void *stream);
+
+/* *result = sum_i a_i b_i over the grid (no volume element), summed in a fixed
+ * order (the same bits on every call).  Synchronises the stream. */
+int qm_grid_dot_f64(qm_ctx *ctx, const double *d_a, const double *d_b, double *result, void *stream);
+
+/* Density mixer on device fields: MixingSimple (mixing_simple.h:45-75) or
+ * MixingDIIS (mixing_DIIS.h:48-229, the reference's default: a ring of
+ * `history` densities and residuals, simple mixing until it is full, then the
+ * coefficients from the bordered residual-overlap matrix inverted by
+ * Gauss-Jordan as InverseM does).  qm_mixer_init takes the first density;
+ * qm_mixer_predict replaces d_rho (the density from the bands) with the
+ * mixed one and returns sum_i |rho_out - rho_in| in *diff (the reference
+ * then multiplies by dV / N_e).  Synchronises the stream. */
+#define QM_MIX_SIMPLE 0
+#define QM_MIX_DIIS 1
+typedef struct qm_mixer qm_mixer;
+int qm_mixer_create(qm_mixer **out, qm_ctx *ctx, int mode, int history, double ratio);
+void qm_mixer_destroy(qm_mixer *m);
+int qm_mixer_init(qm_mixer *m, const double *d_rho, void *stream);
+int qm_mixer_predict(qm_mixer *m, double *d_rho, double *diff, void *stream);
+
+/* Occupations of nb bands at nk k-points (host arrays, eig[k*nb + n]
+ * ascending per k-point; kweight[k] >= 0): below kT = 0.1 K the aufbau
+ * filling of mSetOccupancyZero (qumasun_base1.occupancy.h:35-67; *mu = NaN),
+ * else Fermi-Dirac with mu found by bisection to 1e-9 Ha as in
+ * mSetOccupancyTemperature (occupancy.h:76-175, SetOccupancy.h).  Spin
+ * unpolarised: each band holds 2 kweight[k] / sum(kweight) electrons at most. */
+int qm_occupancy(int nk, int nb, const double *eig, const double *kweight, double nelec, double kbT, double *occ,
+                 double *mu);
 
 /* ------------------------------------------------- UPF pseudopotentials (host) */
 

@@ -25,6 +25,9 @@ EXPORTS = (
     "qm_td_taylor_f64", "qm_td_taylor_f32", "qm_td_expv_f64", "qm_td_expv_f32",
     "qm_set_kinetic", "qm_td_kinetic_fft_f64", "qm_td_kinetic_fft_f32",
     "qm_nonlocal_force", "qm_nonlocal_choice",
+    "qm_subspace_force",
+    "qm_hartree_f64", "qm_xc_lda_f64", "qm_grid_dot_f64", "qm_mixer_create", "qm_mixer_destroy", "qm_mixer_init",
+    "qm_mixer_predict", "qm_occupancy",
     "qm_upf_load", "qm_upf_free", "qm_upf_info", "qm_upf_radial", "qm_upf_eval", "qm_upf_tables",
 )
 
@@ -94,6 +97,16 @@ def load():
     L.qm_stencil_choice.argtypes = [P, I, I, I, I, ctypes.POINTER(I)]
     L.qm_sygvd_f64.argtypes = [P, I, P, P, P, P, P, I, P]
     L.qm_release_captures.argtypes = [P]
+    L.qm_subspace_force.argtypes = [I]
+    L.qm_hartree_f64.argtypes = [P, P, P, P]
+    L.qm_xc_lda_f64.argtypes = [P, P, P, P, P]
+    L.qm_grid_dot_f64.argtypes = [P, P, P, ctypes.POINTER(D), P]
+    L.qm_mixer_create.argtypes = [ctypes.POINTER(P), P, I, I, D]
+    L.qm_mixer_destroy.argtypes = [P]
+    L.qm_mixer_destroy.restype = None
+    L.qm_mixer_init.argtypes = [P, P, P]
+    L.qm_mixer_predict.argtypes = [P, P, ctypes.POINTER(D), P]
+    L.qm_occupancy.argtypes = [I, I, P, P, D, D, P, ctypes.POINTER(D)]
     L.qm_upf_load.argtypes = [ctypes.POINTER(P), ctypes.c_char_p]
     L.qm_upf_free.argtypes = [P]
     L.qm_upf_free.restype = None
@@ -102,7 +115,7 @@ def load():
     L.qm_upf_eval.argtypes = [P, I, I, P, P, P]
     L.qm_upf_tables.argtypes = [P, I, P, I, I, I, D, D, D, P, P, P, P, P, P, P, P, P]
     for n in EXPORTS:
-        if n not in ("qm_last_error", "qm_destroy", "qm_lobpcg_destroy", "qm_upf_free"):
+        if n not in ("qm_last_error", "qm_destroy", "qm_lobpcg_destroy", "qm_upf_free", "qm_mixer_destroy"):
             getattr(L, n).restype = I
     _lib = L
     return L

@@ -439,11 +439,27 @@ int sytrd_lower(void *blas, int n, double *C, int lda, double *trd, double *VW, 
 // Eigenvalues (all, ascending, into w) and the eigenvectors of the nev lowest
 // into Z[:, :nev] (ldz >= n) of the symmetric matrix whose upper triangle C
 // holds (C overwritten).
-// n <= kMaxN: own tridiagonal reduction, rocSOLVER stedc + ormtr (Z must hold
-// n x n); larger n: rocSOLVER syevd.  info receives rocSOLVER's status.
+// Where the cooperative reduction fits (trd_coop.hip): that reduction,
+// rocSOLVER stedc + ormtr; else n <= kMaxN: the single-workgroup panel
+// reduction below, stedc + ormtr (Z must hold n x n in both); larger n:
+// rocSOLVER syevd.  info receives rocSOLVER's status (or the cooperative
+// reduction's barrier error, -77).
 int syev_lower(void *blas, int n, int nev, double *C, int lda, double *w, double *Z, int ldz, double *trd,
                double *VW, int *info, hipStream_t st) {
     rocblas_handle h = (rocblas_handle)blas;
+    if (subspace_route() != 0 && trd_coop_supported(n, false) &&
+        trd_coop_work_bytes(n, false) <= sizeof(double) * (size_t)kTrdPanelCols * n) {
+        const int r = trd_coop_f64(n, C, lda, trd, trd + n, trd + 2 * n, VW,
+                                   sizeof(double) * (size_t)kTrdPanelCols * n, st);
+        if (r == QM_OK) {
+            QM_HIP(hipMemcpyAsync(w, trd, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+            QM_EIG_BLAS(rocsolver_dstedc(h, rocblas_evect_tridiagonal, n, w, trd + n, Z, ldz, info));
+            QM_EIG_BLAS(rocsolver_dormtr(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, n, nev, C,
+                                         lda, trd + 2 * n, Z, ldz));
+            return trd_coop_check(VW, info, st);
+        }
+        if (r != QM_ERR_STATE) return r;
+    }
     if (n > kMaxN) {
         QM_EIG_BLAS(rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, n, C, lda, w, trd, info));
         QM_HIP(hipMemcpy2DAsync(Z, sizeof(double) * ldz, C, sizeof(double) * lda, sizeof(double) * n, nev,
@@ -518,7 +534,20 @@ __global__ void symmetrize_upper_kernel(double *__restrict__ C, const double *__
 
 using namespace qm;
 
+namespace qm {
+namespace {
+int g_subspace_route = -1;
+}
+int subspace_route() { return g_subspace_route; }
+}  // namespace qm
+
 extern "C" {
+
+int qm_subspace_force(int route) {
+    QM_CHECK(route >= -1 && route <= 1, QM_ERR_ARG, "qm_subspace_force: route must be -1, 0 or 1");
+    g_subspace_route = route;
+    return QM_OK;
+}
 
 int qm_sygvd_f64(qm_ctx *ctx, int n, const double *d_A, const double *d_B, double *d_w, double *d_Z, double *d_U,
                  int nev, void *stream) {

@@ -529,6 +529,27 @@ int chol_solve(qm_lobpcg *s, int n3, hipStream_t st) {
                               rocblas_diagonal_non_unit, n3, n3, &one, U, n3, C, n3));
         QM_BLAS(rocblas_ztrsm(s->blas, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
                               rocblas_diagonal_non_unit, n3, n3, &one, U, n3, C, n3));
+        if (subspace_route() != 0 && trd_coop_supported(n3, true) &&
+            trd_coop_work_bytes(n3, true) <= sizeof(double) * (size_t)kTrdPanelCols * n3) {
+            // own cooperative reduction C = Q T Q^H, stedc on T, Z = Q Y for the N
+            // lowest, c = U^-1 Z; eigenvectors in T1
+            double *d = s->trd, *e = s->trd + n3;
+            auto *tau = (rocblas_double_complex *)(s->trd + 2 * n3);
+            auto *Z = (rocblas_double_complex *)s->T1;
+            const int r = trd_coop_z64(n3, C, n3, d, e, tau, s->VW, sizeof(double) * (size_t)kTrdPanelCols * n3, st);
+            if (r == QM_OK) {
+                QM_HIP(hipMemcpyAsync(s->evals, d, sizeof(double) * n3, hipMemcpyDeviceToDevice, st));
+                QM_BLAS(rocsolver_zstedc(s->blas, rocblas_evect_tridiagonal, n3, s->evals, e, Z, n3, s->info));
+                QM_BLAS(rocsolver_zunmtr(s->blas, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, n3,
+                                         N, C, n3, tau, Z, n3));
+                QM_TRY(trd_coop_check(s->VW, s->info, st));
+                QM_BLAS(rocblas_ztrsm(s->blas, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
+                                      rocblas_diagonal_non_unit, n3, N, &one, U, n3, Z, n3));
+                s->evec_t1 = true;
+                return QM_OK;
+            }
+            if (r != QM_ERR_STATE) return r;
+        }
         QM_BLAS(rocsolver_zheevd(s->blas, rocblas_evect_original, rocblas_fill_upper, n3, C, n3, s->evals, s->ework,
                                  s->info));
         QM_BLAS(rocblas_ztrsm(s->blas, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
@@ -1259,7 +1280,7 @@ int qm_lobpcg_create(qm_lobpcg **out, qm_ctx *ctx, int nbands, int ncomp, int pr
         set_error("qm_lobpcg_create: hipHostMalloc failed");
         st = QM_ERR_NOMEM;
     }
-    if (st == QM_OK) st = dalloc((void **)&s->trd, sizeof(double) * 3 * (size_t)n3);
+    if (st == QM_OK) st = dalloc((void **)&s->trd, sizeof(double) * 4 * (size_t)n3);  // d, e, tau (complex)
     if (st == QM_OK) st = dalloc((void **)&s->VW, sizeof(double) * (size_t)kTrdPanelCols * n3);
     if (st == QM_OK && rocblas_create_handle(&s->blas) != rocblas_status_success) {
         set_error("qm_lobpcg_create: rocblas_create_handle failed");

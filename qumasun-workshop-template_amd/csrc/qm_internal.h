@@ -138,6 +138,7 @@ struct qm_ctx {
     void *blas = nullptr;                   // rocblas_handle of qm_sygvd_f64, created on first use
     qm::DevBuf eig_work;                    // qm_sygvd_f64 workspace
     qm::DevBuf td_work;                     // two band blocks of the Taylor propagator
+    qm::DevBuf scf_part;                    // partial sums of qm_grid_dot_f64
     // second stream of a full H.psi apply: the nonlocal projection runs on it
     // beside the stencil, joined by events (created on first use)
     hipStream_t side = nullptr;
@@ -190,6 +191,25 @@ int potrf_upper(void *blas, int n, double *A, int lda, int *info, hipStream_t st
 int chol_transform(void *blas, int n, const double *U, double *C, double *Ui, double *T, hipStream_t st);
 int chol_back(void *blas, int n, int nev, const double *Ui, const double *Z, int ldz, double *Zout, int ldo);
 constexpr int kTrdPanelCols = 96;  // VW workspace columns (n x 96) of sytrd_lower
+
+// trd_coop.hip: tridiagonal reduction by cooperating workgroups (matrix in LDS),
+// LAPACK's lower-storage output; reads A's upper triangle.  Returns
+// QM_ERR_STATE when the order does not fit or the cooperative launch is
+// refused (the caller then takes another route).  work: trd_coop_work_bytes;
+// trd_coop_check merges the kernel's barrier error word into *info (call it
+// after the routine that writes info).  Complex matrices are interleaved
+// (re, im) FP64, tau likewise.
+size_t trd_coop_work_bytes(int n, bool cplx);
+int trd_coop_supported(int n, bool cplx);  // workgroups it would use, 0 if none
+int trd_coop_f64(int n, double *A, int lda, double *d, double *e, double *tau, void *work, size_t work_bytes,
+                 hipStream_t st);
+int trd_coop_z64(int n, void *A, int lda, double *d, double *e, void *tau, void *work, size_t work_bytes,
+                 hipStream_t st);
+int trd_coop_check(void *work, int *info, hipStream_t st);
+// route of the subspace eigensolve's tridiagonal reduction (qm_subspace_force):
+// -1 automatic (cooperative where it fits), 0 never cooperative, 1 cooperative
+// wherever it fits
+int subspace_route();
 
 // kinetic_fft.hip: out = IFFT(|G+k|^2/2 FFT psi)/N + V_loc psi (overwrites); context teardown
 template <typename T>

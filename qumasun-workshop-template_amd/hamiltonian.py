@@ -25,6 +25,13 @@ def _stream(torch, device):
     return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
 
 
+def subspace_force(route):
+    """Route of the subspace eigensolve's tridiagonal reduction (process-wide):
+    -1 automatic, 0 never the cooperative multi-workgroup reduction, 1 that
+    reduction wherever it fits (qm_subspace_force)."""
+    _lib.check(_lib.load().qm_subspace_force(int(route)))
+
+
 class Hamiltonian:
     def __init__(self, system, device=0):
         self.lib = _lib.load()

@@ -38,13 +38,24 @@ def H(qm, torch_cuda):
     h.close()
 
 
-@pytest.mark.parametrize("n", [1, 5, 72, 88, 89, 121, 128, 129, 144, 200, 384, 512, 521])
+@pytest.fixture
+def route(qm, request):
+    qm.hamiltonian.subspace_force(request.param)
+    yield request.param
+    qm.hamiltonian.subspace_force(-1)
+
+
+# route -1: the cooperative multi-workgroup tridiagonal reduction wherever it
+# fits (every n here); route 0: the single-workgroup reduction (n <= 512) or
+# rocSOLVER syevd
+@pytest.mark.parametrize("route", [-1, 0], indirect=True, ids=["coop", "single"])
+@pytest.mark.parametrize("n", [1, 2, 5, 72, 88, 89, 121, 128, 129, 144, 200, 384, 512, 521, 768, 1024])
 @pytest.mark.parametrize("clusters", [False, True])
-def test_sygvd_against_lapack(H, torch_cuda, n, clusters):
+def test_sygvd_against_lapack(H, torch_cuda, route, n, clusters):
     torch = torch_cuda
     if clusters and n < 12:
         pytest.skip("cluster construction needs n >= 12")
-    print(f"sygvd n {n} clusters {clusters}", flush=True)
+    print(f"sygvd n {n} clusters {clusters} route {route}", flush=True)
     A, B = _problem(n, seed=n + (1000 if clusters else 0), clusters=clusters)
     nev = max(1, n // 3)
     w, Z, U = H.sygvd(torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda(), nev)

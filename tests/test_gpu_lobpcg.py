@@ -42,7 +42,15 @@ def _gpu_run(qm, torch, s, X, iters, calls, prec, ik=0):
     return out, ev, rn, t
 
 
-def test_lobpcg_fp64_matches_oracle_c0(qm, orc, torch_cuda):
+@pytest.fixture
+def route(qm, request):
+    qm.hamiltonian.subspace_force(request.param)
+    yield request.param
+    qm.hamiltonian.subspace_force(-1)
+
+
+@pytest.mark.parametrize("route", [-1, 0], indirect=True, ids=["coop", "single"])
+def test_lobpcg_fp64_matches_oracle_c0(qm, orc, torch_cuda, route):
     """8 iterations: <= 1e-12 relative.  20 iterations: within 10x the
     oracle's own spread under a 1e-15 relative perturbation of the start
     vectors (once converged bands make S ill-conditioned the iteration
@@ -85,10 +93,13 @@ def test_lobpcg_converges_c0(qm, torch_cuda, prec, tol):
     assert err[:21].max() <= tol
 
 
-def test_lobpcg_complex_kpoint_matches_oracle(qm, orc, torch_cuda):
+@pytest.mark.parametrize("route", [-1, 0], indirect=True, ids=["coop", "heevd"])
+def test_lobpcg_complex_kpoint_matches_oracle(qm, orc, torch_cuda, route):
+    """Both routes of the complex subspace solve: the cooperative tridiagonal
+    reduction + stedc + unmtr, and rocSOLVER heevd."""
     s = qm.system.config(2, nbands=8, grid=40)
     X0 = s.initial_bands()
-    print("case C2 40^3 8 complex bands k=(1/4,1/4,1/4) FP64 2 x 6 iterations", flush=True)
+    print(f"case C2 40^3 8 complex bands k=(1/4,1/4,1/4) FP64 2 x 6 iterations route {route}", flush=True)
     _, ev_o = _oracle_run(qm, orc, s, X0, 6, 2, ik=3)
     _, ev_g, rn, t = _gpu_run(qm, torch_cuda, s, X0, 6, 2, 64, ik=3)
     err = np.abs(ev_g - ev_o).max() / np.abs(ev_o).max()
