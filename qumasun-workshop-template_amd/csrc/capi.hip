@@ -602,6 +602,7 @@ void qm_destroy(qm_ctx *ctx) {
     ctx->nl_part.release();
     ctx->eig_work.release();
     ctx->td_work.release();
+    ctx->scf_part.release();
     fft_destroy(ctx);
     if (ctx->blas) rocblas_destroy_handle((rocblas_handle)ctx->blas);
     if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);

@@ -70,6 +70,7 @@ struct C2 {
 // x fastest; nxe = nx, or nx/2 + 1 for a real band's half spectrum).
 //   mode 0: f = e2 * scale (kinetic operator)
 //   mode 1: f = e^{-i dt e2} * scale (kinetic propagator)
+//   mode 2: f = 4 pi / |G|^2 * scale, 0 at G = 0 (Poisson solve, k = 0)
 template <typename T, int MODE>
 __global__ __launch_bounds__(256) void kspace_kernel(C2<T> *__restrict__ spec, int64_t per_band, int64_t total,
                                                      int nxe, int nx, int ny, int nz, double gx, double gy, double gz,
@@ -86,6 +87,10 @@ __global__ __launch_bounds__(256) void kspace_kernel(C2<T> *__restrict__ spec, i
         C2<T> v = spec[e];
         if constexpr (MODE == 0) {
             const T f = (T)(e2 * scale);
+            v.x *= f;
+            v.y *= f;
+        } else if constexpr (MODE == 2) {
+            const T f = (ix == 0 && iy == 0 && iz == 0) ? (T)0 : (T)(2.0 * M_PI / e2 * scale);
             v.x *= f;
             v.y *= f;
         } else {
@@ -331,11 +336,32 @@ int fft_kinetic_step(qm_ctx *ctx, T *psi, int nb, const double k[3], double dt, 
     return inverse<T>(ctx, pair, spec, psi, nb, 2, nullptr, st);
 }
 
+// V_H = IFFT(4 pi / |G|^2 FFT rho) / N with the G = 0 term dropped: the FFT
+// Poisson solve of SetPotentialByPoissonFFT (poisson_fft.h:16-120) and
+// mSetPotentialVhart (qumasun_base1.potential.h:32-116), without the
+// refined-grid up/down conversion.  rho and V_H are FP64 grid fields.
+int fft_poisson(qm_ctx *ctx, const double *rho, double *vh, hipStream_t st) {
+    FftPair *pair = nullptr;
+    C2<double> *spec = nullptr;
+    int64_t pb = 0;
+    QM_TRY(forward<double>(ctx, rho, 1, 1, &pair, &spec, &pb, st));
+    const double k0[3] = {0.0, 0.0, 0.0};
+    launch_kspace<double, 2>(ctx, spec, pb, 1, 1, k0, 0.0, st);
+    QM_HIP(hipGetLastError());
+    return inverse<double>(ctx, pair, spec, vh, 1, 1, nullptr, st);
+}
+
 }  // namespace qm
 
 using namespace qm;
 
 extern "C" {
+
+int qm_hartree_f64(qm_ctx *ctx, const double *d_rho, double *d_vh, void *stream) {
+    QM_CHECK(ctx && d_rho && d_vh, QM_ERR_ARG, "qm_hartree_f64: NULL argument");
+    QM_TRY(activate(ctx));
+    return fft_poisson(ctx, d_rho, d_vh, (hipStream_t)stream);
+}
 
 int qm_set_kinetic(qm_ctx *ctx, int mode) {
     QM_CHECK(ctx, QM_ERR_ARG, "qm_set_kinetic: NULL context");
