@@ -81,6 +81,19 @@ def measured_traffic(candidate, workload):
     return None
 
 
+def nonlocal_traffic(workload):
+    """HBM bytes of one FP32 nonlocal apply (projection + coefficient sums +
+    scatter), from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+    (None if that workload was not profiled)."""
+    path = os.path.join(ROOT, "profiles", "nonlocal_c1_fp32_pmc.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic:") and "kernels=nonlocal" in line and f"workload={workload}" in line:
+            return int(line.rsplit("bytes=", 1)[1])
+    return None
+
+
 def cpu_model():
     """The host CPU's model name (as lscpu prints it), from /proc/cpuinfo."""
     try:
@@ -382,6 +395,7 @@ def main():
         "nonlocal_roofline": {"bound": "hbm", "achieved": nl_bytes / (ms_nl * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                               "unit": "GB/s", "frac": nl_bytes / (ms_nl * 1e-3) / 1e9 / HBM_PEAK_GBS,
                               "bytes_per_apply": nl_bytes, "ms_per_apply": ms_nl,
+                              "traffic": nonlocal_traffic(s.name.split()[0]) if args.config == 1 else None,
                               "kernels": "nl_project_kernel + nl_coef_kernel + nl_scatter_kernel"},
         "fp64_hpsi": {"ms_per_apply": ms64, "value": s.ng * nb / (ms64 * 1e-3),
                       "stencil_roofline": {"bound": "hbm", "kernel": H.stencil_candidates()[cand64] if cand64 >= 0

@@ -333,6 +333,13 @@ int qm_lobpcg_set_conditioning(qm_lobpcg *s, double min_pivot_ratio_fp64, double
  * A ratio above 1 (at most 2) fails every pivot test, so every step takes the
  * S-eigenbasis path: a way to exercise the fallback. */
 int qm_lobpcg_fallbacks(qm_lobpcg *s);
+/* Subspace eigensolves of the last qm_lobpcg_solve that had to be redone: a
+ * divide-and-conquer tridiagonal solve that did not converge (redone by
+ * QL/QR), a heevd / syevd that did not converge (redone by Jacobi), or a
+ * tridiagonal reduction with non-finite output (redone by heevd).  note
+ * (may be NULL) receives a description of the last one, at most len bytes.
+ * No counterpart in the reference, which prints LAPACK's info. */
+int qm_lobpcg_eigen_retries(qm_lobpcg *s, char *note, int len);
 
 /* The |1 - |x'|^2| limit above which an update is rolled back: P = R, the
  * previous vectors are returned (renormalised) with their Ritz values and
@@ -341,6 +348,20 @@ int qm_lobpcg_fallbacks(qm_lobpcg *s);
  * returned values are FP64 Rayleigh quotients of the returned vectors in either
  * case.  limit = 0 rolls back the first update of a call. */
 int qm_lobpcg_set_rollback_limit(qm_lobpcg *s, double limit);
+
+/* Generalized Hermitian eigenproblem A z = l B z in complex FP64, in place of
+ * lapack_ZHEGVD_emu (w_zhegvd.h:227-272): B = U^H U (rocSOLVER zpotrf),
+ * C = U^-H A U^-1 (two ztrsm), the cooperative tridiagonal reduction of C,
+ * stedc (steqr where stedc does not converge) and the back-transformations.
+ * Matrices are n x n column-major, complex interleaved (re, im) doubles;
+ * d_A and d_B: upper triangles referenced, not modified.  d_w: all n
+ * eigenvalues, ascending.  d_Z: n x nev, the nev lowest eigenvectors,
+ * B-orthonormal.  d_U: upper triangle receives the Cholesky factor of B.
+ * n <= 4096.  Synchronises the stream.  QM_ERR_NUMERIC when B is not positive
+ * definite.  Where the cooperative reduction does not fit (or
+ * qm_subspace_force(0)), rocSOLVER heevd diagonalises C. */
+int qm_hegvd_z64(qm_ctx *ctx, int n, const void *d_A, const void *d_B, double *d_w, void *d_Z, void *d_U, int nev,
+                 void *stream);
 
 /* Route of the subspace eigensolve's tridiagonal reduction, process-wide (no
  * reference counterpart: LAPACK / rocSOLVER's sytrd and hetrd are the

@@ -386,6 +386,20 @@ __global__ __launch_bounds__(kThreads) void trd_tail_kernel(double *__restrict__
 // C(i, j) = C(j, i) for i > j: the upper triangle defines the matrix, as in
 // LAPACK's uplo = 'U' (a product like U^-T H U^-1 is symmetric only up to
 // rounding, and the reduction below reads both triangles)
+// complex n x n column-major (interleaved re, im): the strictly lower triangle
+// := the conjugate of the upper one, the diagonal's imaginary part := 0
+__global__ void hermitian_from_upper_kernel(double *__restrict__ C, int n) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (int64_t)n * n) return;
+    const int i = (int)(e % n), j = (int)(e / n);
+    if (i > j) {
+        C[2 * e] = C[2 * ((int64_t)j + (int64_t)i * n)];
+        C[2 * e + 1] = -C[2 * ((int64_t)j + (int64_t)i * n) + 1];
+    } else if (i == j) {
+        C[2 * e + 1] = 0.0;
+    }
+}
+
 // strictly lower triangle of the n x n column-major M set to zero
 __global__ void zero_lower_kernel(double *__restrict__ M, int n, int ld) {
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -443,7 +457,8 @@ int sytrd_lower(void *blas, int n, double *C, int lda, double *trd, double *VW, 
 // rocSOLVER stedc + ormtr; else n <= kMaxN: the single-workgroup panel
 // reduction below, stedc + ormtr (Z must hold n x n in both); larger n:
 // rocSOLVER syevd.  info receives rocSOLVER's status (or the cooperative
-// reduction's barrier error, -77).
+// reduction's barrier error, -77).  trd holds 4 n doubles (d, e, tau, and a
+// copy of e for the steqr retry after a stedc failure).
 int syev_lower(void *blas, int n, int nev, double *C, int lda, double *w, double *Z, int ldz, double *trd,
                double *VW, int *info, hipStream_t st) {
     rocblas_handle h = (rocblas_handle)blas;
@@ -452,11 +467,23 @@ int syev_lower(void *blas, int n, int nev, double *C, int lda, double *w, double
         const int r = trd_coop_f64(n, C, lda, trd, trd + n, trd + 2 * n, VW,
                                    sizeof(double) * (size_t)kTrdPanelCols * n, st);
         if (r == QM_OK) {
+            double *e_copy = trd + 3 * n;  // stedc overwrites e
+            QM_HIP(hipMemcpyAsync(e_copy, trd + n, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
             QM_HIP(hipMemcpyAsync(w, trd, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
             QM_EIG_BLAS(rocsolver_dstedc(h, rocblas_evect_tridiagonal, n, w, trd + n, Z, ldz, info));
+            int h_info = 0;
+            QM_HIP(hipMemcpyAsync(&h_info, info, sizeof(int), hipMemcpyDeviceToHost, st));
+            QM_HIP(hipStreamSynchronize(st));
+            if (h_info != 0) {
+                // divide and conquer did not converge (it can on nearly degenerate
+                // spectra): implicit QL/QR on the same tridiagonal instead
+                QM_HIP(hipMemcpyAsync(w, trd, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+                QM_EIG_BLAS(rocsolver_dsteqr(h, rocblas_evect_tridiagonal, n, w, e_copy, Z, ldz, info));
+            }
+            QM_TRY(trd_coop_check(VW, info, st));
             QM_EIG_BLAS(rocsolver_dormtr(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, n, nev, C,
                                          lda, trd + 2 * n, Z, ldz));
-            return trd_coop_check(VW, info, st);
+            return QM_OK;
         }
         if (r != QM_ERR_STATE) return r;
     }
@@ -564,11 +591,11 @@ int qm_sygvd_f64(qm_ctx *ctx, int n, const double *d_A, const double *d_B, doubl
     }
     rocblas_handle h = (rocblas_handle)ctx->blas;
     QM_EIG_BLAS(rocblas_set_stream(h, st));
-    // workspace: C (n^2), Z (n^2), d e tau (3n), VW (96 n), info
+    // workspace: C (n^2), Z (n^2), d e tau + e copy (4n), VW (96 n), U^-1 and T (n^2 each), info
     const size_t nn = (size_t)n * n;
-    const size_t need = sizeof(double) * (4 * nn + 3 * (size_t)n + (size_t)kTrdPanelCols * n) + 256;
+    const size_t need = sizeof(double) * (4 * nn + 4 * (size_t)n + (size_t)kTrdPanelCols * n) + 256;
     QM_TRY(ctx->eig_work.reserve(need));
-    double *C = ctx->eig_work.as<double>(), *Z = C + nn, *trd = Z + nn, *VW = trd + 3 * n;
+    double *C = ctx->eig_work.as<double>(), *Z = C + nn, *trd = Z + nn, *VW = trd + 4 * n;
     double *Ui = VW + (size_t)kTrdPanelCols * n, *T = Ui + nn;
     int *info = (int *)(T + nn);
     hipLaunchKernelGGL(symmetrize_upper_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, C, d_A, n);
@@ -587,6 +614,83 @@ int qm_sygvd_f64(qm_ctx *ctx, int n, const double *d_A, const double *d_B, doubl
     QM_HIP(hipStreamSynchronize(st));
     QM_CHECK(h_info == 0, QM_ERR_NUMERIC,
              "qm_sygvd_f64: tridiagonal eigensolve failed (rocSOLVER stedc info " + std::to_string(h_info) + ")");
+    return QM_OK;
+}
+
+int qm_hegvd_z64(qm_ctx *ctx, int n, const void *d_A, const void *d_B, double *d_w, void *d_Z, void *d_U, int nev,
+                 void *stream) {
+    QM_CHECK(ctx && d_A && d_B && d_w && d_Z && d_U, QM_ERR_ARG, "qm_hegvd_z64: NULL argument");
+    QM_CHECK(n >= 1 && n <= 4096, QM_ERR_ARG, "qm_hegvd_z64: n must be in [1, 4096]");
+    QM_CHECK(nev >= 1 && nev <= n, QM_ERR_ARG, "qm_hegvd_z64: nev must be in [1, n]");
+    QM_TRY(activate(ctx));
+    hipStream_t st = (hipStream_t)stream;
+    if (!ctx->blas) {
+        rocblas_handle h = nullptr;
+        QM_CHECK(rocblas_create_handle(&h) == rocblas_status_success, QM_ERR_HIP,
+                 "qm_hegvd_z64: rocblas_create_handle failed");
+        ctx->blas = h;
+    }
+    rocblas_handle h = (rocblas_handle)ctx->blas;
+    QM_EIG_BLAS(rocblas_set_stream(h, st));
+    using zc = rocblas_double_complex;
+    // workspace: C and Z (n^2 complex each), d, e, tau (complex), e copy (5n doubles),
+    // the cooperative reduction's work, info
+    const size_t nn = (size_t)n * n;
+    const size_t coopb = trd_coop_work_bytes(n, true);
+    const size_t need = sizeof(zc) * 2 * nn + sizeof(double) * 5 * (size_t)n + coopb + 512;
+    QM_TRY(ctx->eig_work.reserve(need));
+    zc *C = ctx->eig_work.as<zc>(), *Z = C + nn;
+    double *d = reinterpret_cast<double *>(Z + nn), *e = d + n, *e_copy = d + 4 * (size_t)n;
+    zc *tau = reinterpret_cast<zc *>(d + 2 * (size_t)n);
+    void *coop = d + 5 * (size_t)n;
+    int *info = reinterpret_cast<int *>(static_cast<char *>(coop) + coopb);
+    zc *U = static_cast<zc *>(d_U);
+    QM_HIP(hipMemcpyAsync(C, d_A, sizeof(zc) * nn, hipMemcpyDeviceToDevice, st));
+    QM_HIP(hipMemcpyAsync(U, d_B, sizeof(zc) * nn, hipMemcpyDeviceToDevice, st));
+    QM_EIG_BLAS(rocsolver_zpotrf(h, rocblas_fill_upper, n, U, n, info));
+    int h_info = 0;
+    QM_HIP(hipMemcpyAsync(&h_info, info, sizeof(int), hipMemcpyDeviceToHost, st));
+    QM_HIP(hipStreamSynchronize(st));
+    QM_CHECK(h_info == 0, QM_ERR_NUMERIC,
+             "qm_hegvd_z64: B is not positive definite (leading minor " + std::to_string(h_info) + ")");
+    const zc one{1.0, 0.0};
+    // C = U^-H A U^-1 (the upper triangle defines A)
+    hipLaunchKernelGGL(hermitian_from_upper_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st,
+                       reinterpret_cast<double *>(C), n);
+    QM_HIP(hipGetLastError());
+    QM_EIG_BLAS(rocblas_ztrsm(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_conjugate_transpose,
+                              rocblas_diagonal_non_unit, n, n, &one, U, n, C, n));
+    QM_EIG_BLAS(rocblas_ztrsm(h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                              rocblas_diagonal_non_unit, n, n, &one, U, n, C, n));
+    int r = QM_ERR_STATE;
+    if (subspace_route() != 0 && trd_coop_supported(n, true))
+        r = trd_coop_z64(n, C, n, d, e, tau, coop, coopb, st);
+    if (r == QM_OK) {
+        QM_HIP(hipMemcpyAsync(e_copy, e, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+        QM_HIP(hipMemcpyAsync(d_w, d, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+        QM_EIG_BLAS(rocsolver_zstedc(h, rocblas_evect_tridiagonal, n, d_w, e, Z, n, info));
+        QM_HIP(hipMemcpyAsync(&h_info, info, sizeof(int), hipMemcpyDeviceToHost, st));
+        QM_HIP(hipStreamSynchronize(st));
+        if (h_info != 0) {  // divide and conquer did not converge: QL/QR instead
+            QM_HIP(hipMemcpyAsync(d_w, d, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+            QM_EIG_BLAS(rocsolver_zsteqr(h, rocblas_evect_tridiagonal, n, d_w, e_copy, Z, n, info));
+        }
+        QM_TRY(trd_coop_check(coop, info, st));
+        QM_EIG_BLAS(rocsolver_zunmtr(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, n, nev, C, n,
+                                     tau, Z, n));
+    } else {
+        if (r != QM_ERR_STATE) return r;
+        QM_EIG_BLAS(rocsolver_zheevd(h, rocblas_evect_original, rocblas_fill_upper, n, C, n, d_w, e, info));
+        QM_HIP(hipMemcpyAsync(Z, C, sizeof(zc) * n * nev, hipMemcpyDeviceToDevice, st));
+    }
+    // z = U^-1 y for the nev lowest
+    QM_EIG_BLAS(rocblas_ztrsm(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
+                              rocblas_diagonal_non_unit, n, nev, &one, U, n, Z, n));
+    QM_HIP(hipMemcpyAsync(d_Z, Z, sizeof(zc) * n * nev, hipMemcpyDeviceToDevice, st));
+    QM_HIP(hipMemcpyAsync(&h_info, info, sizeof(int), hipMemcpyDeviceToHost, st));
+    QM_HIP(hipStreamSynchronize(st));
+    QM_CHECK(h_info == 0, QM_ERR_NUMERIC,
+             "qm_hegvd_z64: tridiagonal eigensolve failed (rocSOLVER info " + std::to_string(h_info) + ")");
     return QM_OK;
 }
 

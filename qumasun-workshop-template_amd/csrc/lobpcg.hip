@@ -34,6 +34,8 @@
 #include <tuple>
 #include <utility>
 
+#include <cstdio>
+
 #include "qm_internal.h"
 
 namespace qm {
@@ -409,6 +411,8 @@ struct qm_lobpcg {
     void *Ev = nullptr; // eigenvectors of the SVQB fallback (n3 x n3)
     int fallbacks = 0;  // subspace-solve fallbacks (P dropped or SVQB) in the last solve
     int rdrops = 0;     // R orthonormalisations that dropped near-dependent directions
+    int stedc_retries = 0;  // tridiagonal eigensolves redone by steqr after stedc failed
+    std::string eig_note;   // what the last failed eigensolve reported (for error messages)
     double cond64 = 1.0e-14, cond32 = 1.0e-4;  // minimum squared Cholesky pivot ratio of S
     double limit_normx = qm::kLimitNormX;       // rollback when max |1 - |x'|^2| exceeds this
     double rdrop64 = 1.0e-13, rdrop32 = 1.0e-6; // R directions kept above this eigenvalue ratio
@@ -429,7 +433,6 @@ struct qm_lobpcg {
     std::vector<int> ev_phase;
     size_t ev_used = 0;
     double *evk = nullptr;   // Ritz values of the last accepted iteration (device, N)
-    int *hinfo = nullptr;    // pinned host word: subspace-solve status, checked at the next synchronisation
     bool graphs = true;      // replay A.R + Gram of an iteration as a HIP graph
     hipStream_t gstream = nullptr;  // capture stream (graphs launch on the caller's stream)
     int graph_replays = 0;   // replays in the last solve
@@ -460,8 +463,6 @@ void lfree(qm_lobpcg *s) {
     s->evk = nullptr;
     if (s->gstream) (void)hipStreamDestroy(s->gstream);
     s->gstream = nullptr;
-    if (s->hinfo) (void)hipHostFree(s->hinfo);
-    s->hinfo = nullptr;
     for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
     s->ev.clear();
     s->ev_phase.clear();
@@ -538,11 +539,51 @@ int chol_solve(qm_lobpcg *s, int n3, hipStream_t st) {
             auto *Z = (rocblas_double_complex *)s->T1;
             const int r = trd_coop_z64(n3, C, n3, d, e, tau, s->VW, sizeof(double) * (size_t)kTrdPanelCols * n3, st);
             if (r == QM_OK) {
+                double *e_copy = s->trd + 4 * n3;  // stedc overwrites e
+                QM_HIP(hipMemcpyAsync(e_copy, e, sizeof(double) * n3, hipMemcpyDeviceToDevice, st));
                 QM_HIP(hipMemcpyAsync(s->evals, d, sizeof(double) * n3, hipMemcpyDeviceToDevice, st));
+                std::vector<double> hde(2 * (size_t)n3);
+                QM_HIP(hipMemcpyAsync(hde.data(), d, sizeof(double) * 2 * n3, hipMemcpyDeviceToHost, st));
                 QM_BLAS(rocsolver_zstedc(s->blas, rocblas_evect_tridiagonal, n3, s->evals, e, Z, n3, s->info));
+                int h_info = 0;
+                QM_HIP(hipMemcpyAsync(&h_info, s->info, sizeof(int), hipMemcpyDeviceToHost, st));
+                QM_HIP(hipStreamSynchronize(st));
+                bool finite = true;
+                for (size_t i = 0; i + 1 < hde.size(); ++i) finite = finite && std::isfinite(hde[i]);
+                if (!finite) {
+                    // the tridiagonal is not finite: say whether its input was, then
+                    // solve by heevd from C's upper triangle (left intact)
+                    std::vector<Cplx> hc((size_t)n3 * n3);
+                    QM_HIP(hipMemcpyAsync(hc.data(), C, sizeof(Cplx) * hc.size(), hipMemcpyDeviceToHost, st));
+                    QM_HIP(hipStreamSynchronize(st));
+                    bool cfin = true;
+                    for (int jj = 0; jj < n3; ++jj)
+                        for (int ii = 0; ii <= jj; ++ii) {
+                            const Cplx &x = hc[ii + (size_t)jj * n3];
+                            cfin = cfin && std::isfinite(x.re) && std::isfinite(x.im);
+                        }
+                    ++s->stedc_retries;
+                    s->eig_note = std::string("the tridiagonal reduction of order ") + std::to_string(n3) +
+                                  " gave non-finite values from a " + (cfin ? "finite" : "non-finite") + " matrix";
+                    QM_BLAS(rocsolver_zheevd(s->blas, rocblas_evect_original, rocblas_fill_upper, n3, C, n3,
+                                             s->evals, s->ework, s->info));
+                    QM_BLAS(rocblas_ztrsm(s->blas, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
+                                          rocblas_diagonal_non_unit, n3, N, &one, U, n3, C, n3));
+                    s->evec_t1 = false;
+                    return QM_OK;
+                }
+                if (h_info != 0) {
+                    // divide and conquer did not converge (it can on nearly degenerate
+                    // spectra): implicit QL/QR on the same tridiagonal instead
+                    ++s->stedc_retries;
+                    s->eig_note = "stedc of order " + std::to_string(n3) + " info " + std::to_string(h_info);
+                    QM_HIP(hipMemcpyAsync(s->evals, d, sizeof(double) * n3, hipMemcpyDeviceToDevice, st));
+                    QM_BLAS(rocsolver_zsteqr(s->blas, rocblas_evect_tridiagonal, n3, s->evals, e_copy, Z, n3,
+                                             s->info));
+                }
+                QM_TRY(trd_coop_check(s->VW, s->info, st));
                 QM_BLAS(rocsolver_zunmtr(s->blas, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, n3,
                                          N, C, n3, tau, Z, n3));
-                QM_TRY(trd_coop_check(s->VW, s->info, st));
                 QM_BLAS(rocblas_ztrsm(s->blas, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
                                       rocblas_diagonal_non_unit, n3, N, &one, U, n3, Z, n3));
                 s->evec_t1 = true;
@@ -580,19 +621,60 @@ int chol_solve(qm_lobpcg *s, int n3, hipStream_t st) {
     return QM_OK;
 }
 
+// All eigenpairs of the Hermitian n x n matrix A (upper triangle, leading
+// dimension n): eigenvalues ascending into w, eigenvectors over A, by rocSOLVER
+// heevd / syevd.  Where that reports non-convergence (its divide and conquer
+// can fail on nearly degenerate spectra), the same problem again from a copy by
+// rocSOLVER's Jacobi solver (heevj / syevj), a different algorithm.  Uses Ev
+// (the copy) and svw (Jacobi residual, sweeps); synchronises the stream;
+// s->info holds the final status.
+template <bool CPLX>
+int heev_all(qm_lobpcg *s, int n, void *A, double *w, hipStream_t st) {
+    using SmT = typename Sm<CPLX>::t;
+    QM_HIP(hipMemcpyAsync(s->Ev, A, sizeof(SmT) * n * n, hipMemcpyDeviceToDevice, st));
+    if constexpr (CPLX)
+        QM_BLAS(rocsolver_zheevd(s->blas, rocblas_evect_original, rocblas_fill_upper, n,
+                                 (rocblas_double_complex *)A, n, w, s->ework, s->info));
+    else
+        QM_BLAS(rocsolver_dsyevd(s->blas, rocblas_evect_original, rocblas_fill_upper, n, (double *)A, n, w, s->ework,
+                                 s->info));
+    int h_info = 0;
+    QM_HIP(hipMemcpyAsync(&h_info, s->info, sizeof(int), hipMemcpyDeviceToHost, st));
+    QM_HIP(hipStreamSynchronize(st));
+    if (h_info == 0) return QM_OK;
+    ++s->stedc_retries;
+    QM_HIP(hipMemcpyAsync(A, s->Ev, sizeof(SmT) * n * n, hipMemcpyDeviceToDevice, st));
+    std::vector<SmT> h((size_t)n * n);
+    QM_HIP(hipMemcpyAsync(h.data(), A, sizeof(SmT) * h.size(), hipMemcpyDeviceToHost, st));
+    QM_HIP(hipStreamSynchronize(st));
+    bool finite = true;
+    for (const SmT &x : h) {
+        if constexpr (CPLX) finite = finite && std::isfinite(x.re) && std::isfinite(x.im);
+        else finite = finite && std::isfinite(x);
+    }
+    s->eig_note = "heevd/syevd of order " + std::to_string(n) + " info " + std::to_string(h_info) +
+                  (finite ? " on a finite matrix" : " on a matrix with NaN/Inf");
+    QM_CHECK(finite, QM_ERR_NUMERIC, "qm_lobpcg_solve: subspace eigensolve failed (" + s->eig_note + ")");
+    int *sweeps = reinterpret_cast<int *>(s->svw + 1);
+    if constexpr (CPLX)
+        QM_BLAS(rocsolver_zheevj(s->blas, rocblas_esort_ascending, rocblas_evect_original, rocblas_fill_upper, n,
+                                 (rocblas_double_complex *)A, n, 0.0, s->svw, 100, sweeps, w, s->info));
+    else
+        QM_BLAS(rocsolver_dsyevj(s->blas, rocblas_esort_ascending, rocblas_evect_original, rocblas_fill_upper, n,
+                                 (double *)A, n, 0.0, s->svw, 100, sweeps, w, s->info));
+    QM_HIP(hipMemcpyAsync(&h_info, s->info, sizeof(int), hipMemcpyDeviceToHost, st));
+    QM_HIP(hipStreamSynchronize(st));
+    s->eig_note += ", then Jacobi info " + std::to_string(h_info);
+    return QM_OK;
+}
+
 // Fallback when S is not numerically positive definite: B = V s^-1/2 over the
 // kept S eigendirections, eigh(B^H H B), eigenvectors C = B y into s->Ev.
 template <bool CPLX>
 int svqb_eig(qm_lobpcg *s, int n3, double tau, hipStream_t st) {
     using SmT = typename Sm<CPLX>::t;
     QM_HIP(hipMemcpyAsync(s->Sw, s->S, sizeof(SmT) * n3 * n3, hipMemcpyDeviceToDevice, st));
-    if constexpr (CPLX) {
-        QM_BLAS(rocsolver_zheevd(s->blas, rocblas_evect_original, rocblas_fill_upper, n3,
-                                 (rocblas_double_complex *)s->Sw, n3, s->svals, s->ework, s->info));
-    } else {
-        QM_BLAS(rocsolver_dsyevd(s->blas, rocblas_evect_original, rocblas_fill_upper, n3, (double *)s->Sw, n3,
-                                 s->svals, s->ework, s->info));
-    }
+    QM_TRY(heev_all<CPLX>(s, n3, s->Sw, s->svals, st));
     hipLaunchKernelGGL((svqb_basis_kernel<CPLX>), dim3((unsigned)(((int64_t)n3 * n3 + 255) / 256)), dim3(256), 0, st,
                        (SmT *)s->Sw, s->svw, s->svals, n3, tau);
     QM_HIP(hipGetLastError());
@@ -607,8 +689,7 @@ int svqb_eig(qm_lobpcg *s, int n3, double tau, hipStream_t st) {
         hipLaunchKernelGGL((svqb_penalty_kernel<CPLX>), dim3((n3 + 63) / 64), dim3(64), 0, st, (SmT *)s->Hw, s->svw,
                            n3);
         QM_HIP(hipGetLastError());
-        QM_BLAS(rocsolver_zheevd(s->blas, rocblas_evect_original, rocblas_fill_upper, n3, Hr, n3, s->evals,
-                                 s->ework, s->info));
+        QM_TRY(heev_all<CPLX>(s, n3, Hr, s->evals, st));
         QM_BLAS(rocblas_zgemm(s->blas, rocblas_operation_none, rocblas_operation_none, n3, n3, n3, &one, B, n3, Hr,
                               n3, &zero, (rocblas_double_complex *)s->Ev, n3));
     } else {
@@ -621,8 +702,7 @@ int svqb_eig(qm_lobpcg *s, int n3, double tau, hipStream_t st) {
         hipLaunchKernelGGL((svqb_penalty_kernel<CPLX>), dim3((n3 + 63) / 64), dim3(64), 0, st, (SmT *)s->Hw, s->svw,
                            n3);
         QM_HIP(hipGetLastError());
-        QM_BLAS(rocsolver_dsyevd(s->blas, rocblas_evect_original, rocblas_fill_upper, n3, Hr, n3, s->evals, s->ework,
-                                 s->info));
+        QM_TRY(heev_all<CPLX>(s, n3, Hr, s->evals, st));
         QM_BLAS(rocblas_dgemm(s->blas, rocblas_operation_none, rocblas_operation_none, n3, n3, n3, &one, B, n3, Hr,
                               n3, &zero, (double *)s->Ev, n3));
     }
@@ -694,12 +774,7 @@ int orthonormalise_r(qm_lobpcg *s, T *Rout, const T *X, const T *R, hipStream_t 
     } else {
         // eigenbasis of M with near-null directions dropped
         QM_HIP(hipMemcpyAsync(s->Sw, s->Mr, sizeof(SmT) * N * N, hipMemcpyDeviceToDevice, st));
-        if constexpr (CPLX)
-            QM_BLAS(rocsolver_zheevd(s->blas, rocblas_evect_original, rocblas_fill_upper, N,
-                                     (rocblas_double_complex *)s->Sw, N, s->svals, s->ework, s->info));
-        else
-            QM_BLAS(rocsolver_dsyevd(s->blas, rocblas_evect_original, rocblas_fill_upper, N, (double *)s->Sw, N,
-                                     s->svals, s->ework, s->info));
+        QM_TRY(heev_all<CPLX>(s, N, s->Sw, s->svals, st));
         hipLaunchKernelGGL((rorth_svqb_kernel<CPLX>), dim3((unsigned)(((int64_t)N * N + 255) / 256)), dim3(256), 0,
                            st, (SmT *)s->Sw, s->rkeep, s->svals, N, drop);
         QM_HIP(hipGetLastError());
@@ -888,6 +963,7 @@ int solve_impl(qm_lobpcg *s, T *x_user, double *eigenvalues, double *residual_no
     for (double &v : s->t) v = 0.0;
     s->fallbacks = 0;
     s->rdrops = 0;
+    s->stedc_retries = 0;
     const double t_start = now_ms();
     // end of a phase: t[ph] receives the stream time since the previous mark
     s->ev_used = 0;
@@ -962,7 +1038,6 @@ int solve_impl(qm_lobpcg *s, T *x_user, double *eigenvalues, double *residual_no
     int rn_off = d;  // s->rnorm index of the first iterated band
     // evk: the Ritz values returned (entry Rayleigh quotients until an iteration is accepted)
     QM_HIP(hipMemcpyAsync(s->evk, s->eps + d, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
-    *s->hinfo = 0;
     T *X = band(Xb, f), *R = band(Rb, f), *P = band(Pb, f), *AX = band(AXb, f), *AR = band((T *)s->AR, f),
       *AP = band((T *)s->AP, f);
     T *W[4] = {band((T *)s->W[0], f), band((T *)s->W[1], f), band((T *)s->W[2], f), band((T *)s->W[3], f)};
@@ -1061,10 +1136,17 @@ int solve_impl(qm_lobpcg *s, T *x_user, double *eigenvalues, double *residual_no
             QM_HIP(hipStreamSynchronize(st));
             if (h_info == 0 && h_ratio >= cond_limit) {
                 QM_TRY(chol_solve<CPLX>(s, n3a, st));
-                // status read into pinned memory, checked after the update's synchronisation
-                QM_HIP(hipMemcpyAsync(s->hinfo, s->info, sizeof(int), hipMemcpyDeviceToHost, st));
-                evec = s->evec_t1 ? s->T1 : s->Hw;
-                break;
+                // a tridiagonal eigensolve that does not converge (a barely
+                // definite S makes U^-H H U^-1 badly scaled) is handled like a
+                // failed Cholesky: without P, then the S eigenbasis
+                QM_HIP(hipMemcpyAsync(&h_info, s->info, sizeof(int), hipMemcpyDeviceToHost, st));
+                QM_HIP(hipStreamSynchronize(st));
+                if (h_info == 0) {
+                    evec = s->evec_t1 ? s->T1 : s->Hw;
+                    break;
+                }
+                s->eig_note += " (Cholesky-route solve of order " + std::to_string(n3a) + " info " +
+                               std::to_string(h_info) + ")";
             }
             if (nbk_used == 3) {  // drop P and retry
                 nbk_used = 2;
@@ -1075,7 +1157,8 @@ int solve_impl(qm_lobpcg *s, T *x_user, double *eigenvalues, double *residual_no
             QM_HIP(hipMemcpyAsync(&h_info, s->info, sizeof(int), hipMemcpyDeviceToHost, st));
             QM_HIP(hipStreamSynchronize(st));
             QM_CHECK(h_info == 0, QM_ERR_NUMERIC,
-                     "qm_lobpcg_solve: subspace eigensolve failed (rocSOLVER info " + std::to_string(h_info) + ")");
+                     "qm_lobpcg_solve: subspace eigensolve failed (rocSOLVER info " + std::to_string(h_info) +
+                         "; " + s->eig_note + ")");
             evec = s->Ev;
             ++s->fallbacks;
             break;
@@ -1118,10 +1201,9 @@ int solve_impl(qm_lobpcg *s, T *x_user, double *eigenvalues, double *residual_no
         QM_TRY(mark(3));
         QM_HIP(hipMemcpyAsync(h_norm.data(), s->xnorm, sizeof(double) * N, hipMemcpyDeviceToHost, st));
         QM_HIP(hipStreamSynchronize(st));
-        QM_CHECK(*s->hinfo == 0, QM_ERR_NUMERIC,
-                 "qm_lobpcg_solve: subspace eigensolve failed (rocSOLVER info " + std::to_string(*s->hinfo) + ")");
         double maxdev = 0.0;
-        for (int n = 0; n < N; ++n) maxdev = std::max(maxdev, std::fabs(1.0 - h_norm[n]));
+        for (int n = 0; n < N; ++n)  // a non-finite norm (a failed update) rolls back too
+            maxdev = std::isfinite(h_norm[n]) ? std::max(maxdev, std::fabs(1.0 - h_norm[n])) : HUGE_VAL;
         if (maxdev > s->limit_normx) {
             // rollback and escape (lobpcg_z_multi_mpi.h:2611-2666): P = R and leave the
             // loop.  The reference rescales the old x by the norms of the rejected
@@ -1275,12 +1357,7 @@ int qm_lobpcg_create(qm_lobpcg **out, qm_ctx *ctx, int nbands, int ncomp, int pr
         set_error("qm_lobpcg_create: hipStreamCreateWithFlags failed");
         st = QM_ERR_HIP;
     }
-    if (st == QM_OK && hipHostMalloc((void **)&s->hinfo, 64, hipHostMallocDefault) != hipSuccess) {
-        s->hinfo = nullptr;
-        set_error("qm_lobpcg_create: hipHostMalloc failed");
-        st = QM_ERR_NOMEM;
-    }
-    if (st == QM_OK) st = dalloc((void **)&s->trd, sizeof(double) * 4 * (size_t)n3);  // d, e, tau (complex)
+    if (st == QM_OK) st = dalloc((void **)&s->trd, sizeof(double) * 5 * (size_t)n3);  // d, e, tau (complex), e copy
     if (st == QM_OK) st = dalloc((void **)&s->VW, sizeof(double) * (size_t)kTrdPanelCols * n3);
     if (st == QM_OK && rocblas_create_handle(&s->blas) != rocblas_status_success) {
         set_error("qm_lobpcg_create: rocblas_create_handle failed");
@@ -1361,6 +1438,14 @@ int qm_lobpcg_set_conditioning(qm_lobpcg *s, double min_pivot_ratio_fp64, double
 }
 
 int qm_lobpcg_fallbacks(qm_lobpcg *s) { return s ? s->fallbacks : QM_ERR_ARG; }
+
+int qm_lobpcg_eigen_retries(qm_lobpcg *s, char *note, int len) {
+    QM_CHECK(s, QM_ERR_ARG, "qm_lobpcg_eigen_retries: NULL solver");
+    if (note && len > 0) {
+        std::snprintf(note, (size_t)len, "%s", s->eig_note.c_str());
+    }
+    return s->stedc_retries;
+}
 
 int qm_lobpcg_set_rollback_limit(qm_lobpcg *s, double limit) {
     QM_CHECK(s, QM_ERR_ARG, "qm_lobpcg_set_rollback_limit: NULL solver");

@@ -43,8 +43,9 @@ constexpr int kCoopThreads = 256;
 constexpr int kCoopWaves = kCoopThreads / 64;
 constexpr int kCoopLdsBytes = 150 * 1024;
 constexpr long kCoopSpinLimit = 1L << 24;  // ~0.5 s of polling
-constexpr int kCoopPer = 8;                 // vector elements per thread: n <= 2048
+constexpr int kCoopMaxPer = 8;              // vector elements per thread: n <= 2048
 constexpr int kCoopMaxGroups = 256;
+constexpr int kCoopRowsPerWave = 4;         // own rows per wave: nr <= 16
 
 struct zdbl {
     double re, im;
@@ -83,17 +84,6 @@ __device__ inline T rank2(T a, T vr, T wc, T wr, T vc) {
     return sub(sub(a, mul(vr, conj_(wc))), mul(wr, conj_(vc)));
 }
 
-// total of one double per thread, the same bits in every thread (fixed order)
-__device__ inline double block_sum(double s, double *red) {
-    for (int m = 32; m >= 1; m >>= 1) s += __shfl_xor(s, m);
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-    __syncthreads();
-    double t = 0.0;
-    for (int i = 0; i < kCoopWaves; ++i) t += red[i];
-    return t;
-}
-
 // Every wave waits for its own stores to complete (they are then in this
 // XCD's L2); thread 0's agent-scope release writes the L2 back and its
 // acquire invalidates this CU's caches, so after the closing workgroup
@@ -125,17 +115,15 @@ __device__ void coop_barrier(unsigned *count, unsigned *gen, unsigned nblk, unsi
 
 // sync[0]: arrival count, sync[32]: generation (separate 128-byte lines),
 // sync[64]: error word.  All zero at launch.
-template <typename T>
+template <typename T, int PER>
 __global__ __launch_bounds__(kCoopThreads) void trd_coop_kernel(T *__restrict__ A, int lda, int n, int nr,
                                                                 double *__restrict__ dout,
                                                                 double *__restrict__ eout, T *__restrict__ tauout,
                                                                 T *__restrict__ rowbuf, T *__restrict__ pbuf,
                                                                 T *__restrict__ dots, unsigned *sync) {
     __shared__ __align__(16) unsigned char lds_raw[kCoopLdsBytes];
-    __shared__ double red[kCoopWaves];
     __shared__ T wsum[kCoopWaves];
     __shared__ T dsh[kCoopMaxGroups];
-    __shared__ T bcast;
     T *L = reinterpret_cast<T *>(lds_raw);  // own rows: row g + q G at L + q n
     T *xr = L + (size_t)nr * n;             // the current column's row (conjugate of x)
     T *v = xr + n;                          // reflector, global row index
@@ -144,6 +132,7 @@ __global__ __launch_bounds__(kCoopThreads) void trd_coop_kernel(T *__restrict__ 
     unsigned *count = sync, *gen = sync + 32;
     int *err = reinterpret_cast<int *>(sync + 64);
     unsigned lg = 0;
+    const int nr_own = g < n ? min(nr, (n - g + G - 1) / G) : 0;  // own rows r = g + q G < n
 
     for (int q = 0; q < nr; ++q) {
         const int r = g + q * G;
@@ -167,10 +156,13 @@ __global__ __launch_bounds__(kCoopThreads) void trd_coop_kernel(T *__restrict__ 
 
     for (int j = 0; j < n - 1; ++j) {
         const int par = j & 1;
-        // ---- reflector from x = conj(xr[j+1 .. n-1]) (LAPACK dlarfg / zlarfg)
+        // ---- reflector from x = conj(xr[j+1 .. n-1]) (LAPACK dlarfg / zlarfg).
+        // Every wave forms |x_1..|^2 itself, in the same order, and takes lane
+        // 0's total, so all waves of all workgroups hold the same scalars.
         double s = 0.0;
-        for (int c = j + 2 + tid; c < n; c += kCoopThreads) s += abs2(xr[c]);
-        s = block_sum(s, red);
+        for (int c = j + 2 + lane; c < n; c += 64) s += abs2(xr[c]);
+        for (int m = 32; m >= 1; m >>= 1) s += __shfl_xor(s, m);
+        s = __shfl(s, 0);
         const T alpha = conj_(xr[j + 1]);
         T tau = make<T>(0.0, 0.0), scal = make<T>(0.0, 0.0);
         double beta = re_(alpha);
@@ -181,34 +173,51 @@ __global__ __launch_bounds__(kCoopThreads) void trd_coop_kernel(T *__restrict__ 
             const double dd = abs2(den);
             scal = make<T>(re_(den) / dd, -im_(den) / dd);  // 1 / (alpha - beta)
         }
-        for (int c = j + 1 + tid; c < n; c += kCoopThreads)
-            v[c] = c == j + 1 ? make<T>(1.0, 0.0) : mul(conj_(xr[c]), scal);
-        __syncthreads();
+        auto vof = [&](int c) { return c == j + 1 ? make<T>(1.0, 0.0) : mul(conj_(xr[c]), scal); };
+        // ---- p = tau A22 v on own rows (a wave takes rows wave, wave + 4, ...,
+        // all at once), partial p^H v
+        T acc[kCoopRowsPerWave];
+#pragma unroll
+        for (int i = 0; i < kCoopRowsPerWave; ++i) acc[i] = make<T>(0.0, 0.0);
+        // every slot loads (finished or missing rows from a clamped valid row,
+        // their sums unused), so the loads of a column issue together
+        const int qmax = nr_own > 0 ? nr_own - 1 : 0;
+        for (int c = j + 1 + lane; c < n; c += 64) {
+            const T vc = vof(c);
+            T lv[kCoopRowsPerWave];
+#pragma unroll
+            for (int i = 0; i < kCoopRowsPerWave; ++i)
+                lv[i] = L[(size_t)min(wave + i * kCoopWaves, qmax) * n + c];
+#pragma unroll
+            for (int i = 0; i < kCoopRowsPerWave; ++i) acc[i] = add(acc[i], mul(lv[i], vc));
+        }
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1)
+#pragma unroll
+            for (int i = 0; i < kCoopRowsPerWave; ++i) acc[i] = add(acc[i], shfl_xor(acc[i], m));
+        if (lane == 0) {
+            T dot = make<T>(0.0, 0.0);
+#pragma unroll
+            for (int i = 0; i < kCoopRowsPerWave; ++i) {
+                const int q = wave + i * kCoopWaves;
+                const int r = g + q * G;
+                if (q < nr_own && r > j) {
+                    const T p = mul(tau, acc[i]);
+                    pbuf[(size_t)par * n + r] = p;
+                    dot = add(dot, mul(conj_(p), vof(r)));
+                }
+            }
+            wsum[wave] = dot;
+        }
+        for (int c = j + 1 + tid; c < n; c += kCoopThreads) v[c] = vof(c);
         if (g == 0) {
             if (tid == 0) {
                 dout[j] = re_(xr[j]);
                 eout[j] = beta;
                 tauout[j] = tau;
             }
-            for (int c = j + 2 + tid; c < n; c += kCoopThreads) A[c + (size_t)j * lda] = v[c];
+            for (int c = j + 2 + tid; c < n; c += kCoopThreads) A[c + (size_t)j * lda] = vof(c);
         }
-        // ---- p = tau A22 v on own rows, partial p^H v
-        T dot = make<T>(0.0, 0.0);
-        for (int q = wave; q < nr; q += kCoopWaves) {
-            const int r = g + q * G;
-            if (r >= n) break;
-            if (r <= j) continue;
-            const T *Lq = L + (size_t)q * n;
-            T acc = make<T>(0.0, 0.0);
-            for (int c = j + 1 + lane; c < n; c += 64) acc = add(acc, mul(Lq[c], v[c]));
-            for (int m = 32; m >= 1; m >>= 1) acc = add(acc, shfl_xor(acc, m));
-            const T p = mul(tau, acc);
-            if (lane == 0) {
-                pbuf[(size_t)par * n + r] = p;
-                dot = add(dot, mul(conj_(p), v[r]));
-            }
-        }
-        if (lane == 0) wsum[wave] = dot;
         __syncthreads();
         if (tid == 0) {
             T t = make<T>(0.0, 0.0);
@@ -218,9 +227,9 @@ __global__ __launch_bounds__(kCoopThreads) void trd_coop_kernel(T *__restrict__ 
         coop_barrier(count, gen, G, lg, err);
         // ---- one round of loads: the partial dots, p and the old row j+1
         const T *old = rowbuf + (size_t)((j + 1) & 1) * n;
-        T pr[kCoopPer], orow[kCoopPer];
+        T pr[PER], orow[PER];
 #pragma unroll
-        for (int k = 0; k < kCoopPer; ++k) {
+        for (int k = 0; k < PER; ++k) {
             const int c = j + 1 + tid + k * kCoopThreads;
             if (c < n) {
                 pr[k] = pbuf[(size_t)par * n + c];
@@ -229,35 +238,64 @@ __global__ __launch_bounds__(kCoopThreads) void trd_coop_kernel(T *__restrict__ 
         }
         if (tid < G) dsh[tid] = dots[(size_t)par * G + tid];
         __syncthreads();
-        // ---- w = p - (tau / 2) (p^H v) v
-        if (tid == 0) {
-            T t = make<T>(0.0, 0.0);
-            for (int i = 0; i < G; ++i) t = add(t, dsh[i]);
-            bcast = mul(make<T>(-0.5, 0.0), mul(tau, t));
-        }
-        __syncthreads();
-        const T a2 = bcast;
+        // ---- w = p - (tau / 2) (p^H v) v; every wave sums the G partials in
+        // the same order
+        T t = make<T>(0.0, 0.0);
+        for (int i = lane; i < G; i += 64) t = add(t, dsh[i]);
 #pragma unroll
-        for (int k = 0; k < kCoopPer; ++k) {
+        for (int m = 32; m >= 1; m >>= 1) t = add(t, shfl_xor(t, m));
+        if constexpr (sizeof(T) == 8) {
+            t = make<T>(__shfl(re_(t), 0), 0.0);
+        } else {
+            t = make<T>(__shfl(re_(t), 0), __shfl(im_(t), 0));
+        }
+        const T a2 = mul(make<T>(-0.5, 0.0), mul(tau, t));
+        // this thread's columns of v and w stay in registers for the updates
+        T vc[PER], wc[PER];
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
             const int c = j + 1 + tid + k * kCoopThreads;
-            if (c < n) w[c] = add(pr[k], mul(a2, v[c]));
+            if (c < n) {
+                vc[k] = v[c];
+                wc[k] = add(pr[k], mul(a2, vc[k]));
+                w[c] = wc[k];
+            }
         }
         __syncthreads();
         // ---- look-ahead: row j+1 after this update, from its owner's copy
         const T vj = v[j + 1], wj = w[j + 1];
 #pragma unroll
-        for (int k = 0; k < kCoopPer; ++k) {
+        for (int k = 0; k < PER; ++k) {
             const int c = j + 1 + tid + k * kCoopThreads;
-            if (c < n) xr[c] = rank2(orow[k], vj, w[c], wj, v[c]);
+            if (c < n) xr[c] = rank2(orow[k], vj, wc[k], wj, vc[k]);
         }
-        // ---- A22 -= v w^H + w v^H on own rows
-        for (int q = 0; q < nr; ++q) {
-            const int r = g + q * G;
-            if (r >= n) break;
-            if (r <= j) continue;
-            T *Lq = L + (size_t)q * n;
-            const T vr = v[r], wr = w[r];
-            for (int c = j + 1 + tid; c < n; c += kCoopThreads) Lq[c] = rank2(Lq[c], vr, w[c], wr, v[c]);
+        // ---- A22 -= v w^H + w v^H on own rows r > j, four rows per pass (each
+        // pass loads all four rows' columns, then stores them)
+        const int q_lo = j >= g ? (j - g) / G + 1 : 0;  // first own row below j
+        for (int q0 = q_lo; q0 < nr_own; q0 += 4) {
+            T a[4][PER], vr[4], wr[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int q = min(q0 + u, nr_own - 1);
+                const int r = g + q * G;
+                vr[u] = v[r];
+                wr[u] = w[r];
+#pragma unroll
+                for (int k = 0; k < PER; ++k) {
+                    const int c = j + 1 + tid + k * kCoopThreads;
+                    if (c < n) a[u][k] = L[(size_t)q * n + c];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (q0 + u >= nr_own) break;
+                T *Lq = L + (size_t)(q0 + u) * n;
+#pragma unroll
+                for (int k = 0; k < PER; ++k) {
+                    const int c = j + 1 + tid + k * kCoopThreads;
+                    if (c < n) Lq[c] = rank2(a[u][k], vr[u], wc[k], wr[u], vc[k]);
+                }
+            }
         }
         __syncthreads();
         // ---- the owner of row j+2 publishes it for the next column's look-ahead
@@ -298,10 +336,11 @@ int coop_cus() {
 template <typename T>
 int coop_groups(int n) {
     const int cus = coop_cus();
-    if (cus <= 0 || n < 2 || n > kCoopPer * kCoopThreads) return 0;
+    if (cus <= 0 || n < 2 || n > kCoopMaxPer * kCoopThreads) return 0;
     for (int G = 8; G <= cus && G <= kCoopMaxGroups; G += 8) {
         const int nr = (n + G - 1) / G;
-        if ((size_t)(nr + 3) * n * sizeof(T) <= (size_t)kCoopLdsBytes) return G;
+        if (nr <= kCoopRowsPerWave * kCoopWaves && (size_t)(nr + 3) * n * sizeof(T) <= (size_t)kCoopLdsBytes)
+            return G;
     }
     return 0;
 }
@@ -322,8 +361,11 @@ int trd_coop_impl(int n, T *A, int lda, double *d, double *e, T *tau, void *work
     hipLaunchKernelGGL(coop_clear_kernel, dim3(1), dim3(128), 0, st, sync);
     QM_HIP(hipGetLastError());
     void *args[] = {&A, &lda, &n, (void *)&nr, &d, &e, &tau, &rowbuf, &pbuf, &dots, &sync};
-    const hipError_t le = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&trd_coop_kernel<T>), dim3(G),
-                                                     dim3(kCoopThreads), args, 0, st);
+    const int per = (n + kCoopThreads - 1) / kCoopThreads;  // columns per thread
+    const void *fn = per <= 2   ? reinterpret_cast<const void *>(&trd_coop_kernel<T, 2>)
+                     : per <= 4 ? reinterpret_cast<const void *>(&trd_coop_kernel<T, 4>)
+                                : reinterpret_cast<const void *>(&trd_coop_kernel<T, 8>);
+    const hipError_t le = hipLaunchCooperativeKernel(fn, dim3(G), dim3(kCoopThreads), args, 0, st);
     if (le != hipSuccess) {
         (void)hipGetLastError();
         return QM_ERR_STATE;  // the caller takes the single-workgroup route

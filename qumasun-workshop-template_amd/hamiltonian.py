@@ -184,6 +184,24 @@ class Hamiltonian:
                                          ctypes.c_void_p(Ut.data_ptr()), nev, _stream(torch, A.device)))
         return w, Zt.T, Ut.T
 
+    def hegvd(self, A, B, nev=None):
+        """A z = l B z in complex FP64 (Hermitian A, HPD B; upper triangles
+        referenced), qm_hegvd_z64.  Returns (all eigenvalues ascending, Z of
+        shape (n, nev), U the Cholesky factor of B in its upper triangle)."""
+        import torch
+        n = A.shape[0]
+        nev = n if nev is None else nev
+        assert A.dtype == torch.complex128 and B.dtype == torch.complex128 and A.shape == B.shape == (n, n)
+        At = A.T.contiguous()
+        Bt = B.T.contiguous()
+        w = torch.empty(n, dtype=torch.float64, device=A.device)
+        Zt = torch.empty((nev, n), dtype=torch.complex128, device=A.device)
+        Ut = torch.empty((n, n), dtype=torch.complex128, device=A.device)
+        _lib.check(self.lib.qm_hegvd_z64(self.ctx, n, ctypes.c_void_p(At.data_ptr()), ctypes.c_void_p(Bt.data_ptr()),
+                                         ctypes.c_void_p(w.data_ptr()), ctypes.c_void_p(Zt.data_ptr()),
+                                         ctypes.c_void_p(Ut.data_ptr()), nev, _stream(torch, A.device)))
+        return w, Zt.T, Ut.T
+
     # -- adjoint -------------------------------------------------------------
     def band_density(self, psi, occ, ncomp=None, rho=None):
         """rho += sum_n occ_n |psi_n|^2 (FP64, device); dE_band/dV_loc = rho * dV."""

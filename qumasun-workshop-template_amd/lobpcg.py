@@ -107,6 +107,12 @@ class LOBPCG:
     def fallbacks(self):
         return self.lib.qm_lobpcg_fallbacks(self.h)
 
+    def eigen_retries(self):
+        """(subspace eigensolves redone in the last solve, note on the last one)"""
+        buf = ctypes.create_string_buffer(512)
+        n = self.lib.qm_lobpcg_eigen_retries(self.h, buf, 512)
+        return n, buf.value.decode()
+
     def set_graphs(self, enable=True):
         """HIP-graph replay of the A.R and Gram segments of an iteration (default on)."""
         _lib.check(self.lib.qm_lobpcg_set_graphs(self.h, int(bool(enable))))

@@ -94,3 +94,42 @@ def test_sygvd_reports_lapack_failing_minor(qm, H, torch_cuda, n, bad):
     with pytest.raises(qm._lib.QMError) as e:
         H.sygvd(torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda(), 4)
     assert f"leading minor {info})" in str(e.value)
+
+
+def _zproblem(n, seed, clusters=False):
+    rng = np.random.default_rng(seed)
+    M = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    if clusters:
+        lam = np.sort(np.concatenate([np.repeat(rng.uniform(-5, 5, n // 6), 3),
+                                      rng.uniform(-5, 5, n - 3 * (n // 6))]))
+        Q, _ = np.linalg.qr(M)
+        A = (Q * lam) @ Q.conj().T
+    else:
+        A = M + M.conj().T
+    G = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) * (0.3 / np.sqrt(n))
+    B = np.eye(n) + G @ G.conj().T
+    return 0.5 * (A + A.conj().T), 0.5 * (B + B.conj().T)
+
+
+# route -1: the cooperative reduction (complex, every n here); 0: rocSOLVER heevd
+@pytest.mark.parametrize("route", [-1, 0], indirect=True, ids=["coop", "heevd"])
+@pytest.mark.parametrize("n", [1, 2, 5, 24, 72, 129, 192, 256, 384, 768])
+@pytest.mark.parametrize("clusters", [False, True])
+def test_hegvd_against_lapack(H, torch_cuda, route, n, clusters):
+    torch = torch_cuda
+    if clusters and n < 12:
+        pytest.skip("cluster construction needs n >= 12")
+    print(f"hegvd n {n} clusters {clusters} route {route}", flush=True)
+    A, B = _zproblem(n, seed=7 * n + (1000 if clusters else 0), clusters=clusters)
+    nev = max(1, n // 3)
+    w, Z, U = H.hegvd(torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda(), nev)
+    torch.cuda.synchronize()
+    w, Z, U = w.cpu().numpy(), Z.cpu().numpy(), U.cpu().numpy()
+    w_ref = scipy.linalg.eigh(A, B, eigvals_only=True)
+    scale = max(1.0, np.abs(A).max() * n)
+    assert np.abs(w - w_ref).max() <= 1e-11 * scale
+    res = A @ Z - (B @ Z) * w[:nev]
+    assert np.abs(res).max() <= 1e-11 * scale
+    assert np.abs(Z.conj().T @ B @ Z - np.eye(nev)).max() <= 1e-11 * n
+    Uu = np.triu(U)
+    assert np.abs(Uu.conj().T @ Uu - B).max() <= 1e-12 * n
