@@ -152,6 +152,27 @@ def cpu_baseline(s, psi_sample):
                       f"(median of {len(t1)}, {sum(t1):.1f} s)"}
 
 
+def gemm_leg(torch, H, reps):
+    """The LOBPCG iteration's two big products at the 96^3 x 256 complex-band
+    shape: the Gram [X R P]^T [R AR] (m = 3 x 512, n = 2 x 512 real-packed
+    columns, K = 884,736 points) and the update [X R P] C (m = 1536 rows of C,
+    n = 1024 output columns), in FP32 and FP64 on random device data."""
+    m, n, k = 1536, 1024, 884736
+    for dt in (torch.float32, torch.float64):
+        a = torch.rand((m, k), device="cuda", dtype=dt)
+        b = torch.rand((n, k), device="cuda", dtype=dt)
+        c = torch.rand((m, n), device="cuda", dtype=dt)
+        H.gram(a, b)
+        H.update(a, c)
+        ms_g = _time_applies(torch, lambda: H.gram(a, b), reps)
+        ms_u = _time_applies(torch, lambda: H.update(a, c), reps)
+        fl = 2.0 * m * n * k
+        print(f"[bench] {str(dt)[6:]} Gram {ms_g:.3f} ms ({fl / ms_g / 1e9:.1f} TFLOP/s), "
+              f"update {ms_u:.3f} ms ({fl / ms_u / 1e9:.1f} TFLOP/s)", file=sys.stderr, flush=True)
+        del a, b, c
+        torch.cuda.empty_cache()
+
+
 def hpsi_config_leg(qm, torch, cfg, steps, warmup):
     """FP32 H.psi throughput of a larger workload (C2: 96^3, 256 complex bands,
     4 k-points; C3: 160^3, 512 real bands, uneven spheres), summed over its
@@ -278,6 +299,10 @@ def main():
     ap.add_argument("--c2-lobpcg-iters", type=int, default=5,
                     help="iterations per precision of the 96^3 x 256 complex-band LOBPCG leg at one k-point "
                          "(0 skips it)")
+    ap.add_argument("--gemm-only", type=int, default=0, metavar="N",
+                    help="profiling mode: N Gram products (3 x 2 blocks) and N updates at the 96^3 x 256 complex-band "
+                         "LOBPCG shape (real-packed: m = 1536, n = 1024, K = 884,736) in FP32 and FP64, timed with "
+                         "HIP events, then exit (no JSON line; for rocprofv3 runs)")
     ap.add_argument("--stencil-candidates", type=int, default=0, metavar="N",
                     help="profiling mode: N FP32 local applies with each stencil tile shape in turn, then exit "
                          "(no JSON line; for rocprofv3 counter runs)")
@@ -329,6 +354,9 @@ def main():
             nonlocal_()
         torch.cuda.synchronize()
         print(f"[bench] {args.nonlocal_only} nonlocal applies done", file=sys.stderr, flush=True)
+        return
+    if args.gemm_only > 0:
+        gemm_leg(torch, H, args.gemm_only)
         return
     if args.stencil_candidates > 0:
         names = H.stencil_candidates()
