@@ -335,8 +335,11 @@ int qm_lobpcg_set_conditioning(qm_lobpcg *s, double min_pivot_ratio_fp64, double
 int qm_lobpcg_fallbacks(qm_lobpcg *s);
 /* Subspace eigensolves of the last qm_lobpcg_solve that had to be redone: a
  * divide-and-conquer tridiagonal solve that did not converge (redone by
- * QL/QR), a heevd / syevd that did not converge (redone by Jacobi), or a
- * tridiagonal reduction with non-finite output (redone by heevd).  note
+ * QL/QR), a heevd / syevd that did not converge (redone by Jacobi), a
+ * tridiagonal reduction with non-finite output (redone by heevd), or Cholesky-
+ * route Ritz vectors rejected because max_i |1 - c_i^H S c_i| exceeded 1e-4 (S
+ * too ill-conditioned for its Cholesky factor; redone without P, then on the
+ * S eigenbasis, and also counted by qm_lobpcg_fallbacks).  note
  * (may be NULL) receives a description of the last one, at most len bytes.
  * No counterpart in the reference, which prints LAPACK's info. */
 int qm_lobpcg_eigen_retries(qm_lobpcg *s, char *note, int len);

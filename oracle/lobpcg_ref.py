@@ -18,7 +18,9 @@ as the product).  Follows lobpcg_z_multi_mpi.h:1732-2852:
   lapack_ZHEGVD_emu route, w_zhegvd.h:227-272).  When S is not numerically
   positive definite, or its squared Cholesky pivots span less than `cond`, the
   step is redone on [X R] (P dropped) and then on the S eigenbasis with
-  near-null directions dropped (rr_solve, subspace_eig).
+  near-null directions dropped (rr_solve, subspace_eig).  The same happens when
+  the N lowest Ritz vectors of the Cholesky route are not S-orthonormal to
+  RITZ_ORTH_TOL (this build's guard; the reference's zhegvd has none).
 * Gram blocks: every block computed directly on iteration 0 and every 6th
   iteration (S_MAT_REST_LIMIT = 6, 1808, 2093); in between the X/P blocks come
   from the subspace recurrence (NextMatrix_z_blasmp, 1104-1436:
@@ -69,15 +71,17 @@ def cplx_to_soa(z, ncomp):
 
 
 SVQB_DROP = 1.0e-12     # relative eigenvalue of S below which a direction is dropped
+RITZ_ORTH_TOL = 1.0e-4  # max |1 - c_i^H S c_i| of accepted Cholesky-route Ritz vectors
 
 
-def subspace_eig(Hm, S):
+def subspace_eig(Hm, S, pen=1.0e30):
     """H c = l S c.  Cholesky-based generalized solve (LAPACK *sygvd/*hegvd, the
     reference's lapack_ZHEGVD path); when S is not numerically positive
     definite (R and P nearly dependent near convergence, where the reference's
     LAPACK call would report info > 0), fall back to the S eigenbasis with the
     near-null directions dropped (SVQB): B = V s^-1/2, eigh(B^H H B), C = B y.
-    Dropped directions get a huge diagonal so they sort above every kept one."""
+    Dropped directions get the diagonal `pen` (above the N lowest Ritz values,
+    see dropped_penalty) so they sort above them."""
     try:
         return scipy.linalg.eigh(Hm, S, driver="gvd")
     except np.linalg.LinAlgError:
@@ -87,7 +91,7 @@ def subspace_eig(Hm, S):
         B = V * w[None, :]
         Hr = np.conj(B).T @ Hm @ B
         Hr = 0.5 * (Hr + np.conj(Hr).T)
-        Hr[np.diag_indices_from(Hr)] += np.where(keep, 0.0, 1.0e30)
+        Hr[np.diag_indices_from(Hr)] += np.where(keep, 0.0, pen)
         lam, Y = scipy.linalg.eigh(Hr, driver="evd")
         return lam, B @ Y
 
@@ -121,6 +125,14 @@ def orthonormalise_r(x, r, drop):
     return W[:N].T @ x + W[N:].T @ r, keep
 
 
+def dropped_penalty(Hm, N):
+    """Diagonal given to dropped directions: twice a Gershgorin bound of the X
+    block's H plus one, above its largest Ritz value and so above the N-th Ritz
+    value of any subspace containing X (a fixed 1e30 would put eps * 1e30 of
+    rounding on every eigenvalue of a dense eigensolver)."""
+    return 2.0 * np.abs(Hm[:N, :N]).sum(axis=1).max() + 1.0
+
+
 def rr_solve(Hm, S, N, have_p, cond):
     """Rayleigh-Ritz step as the GPU driver takes it: Cholesky of S with a
     squared-pivot ratio >= cond, else retry on [X R] (P dropped), else the
@@ -136,9 +148,15 @@ def rr_solve(Hm, S, N, have_p, cond):
         if d.min() < cond * d.max():
             continue
         lam, C = scipy.linalg.eigh(Hs, Ss, driver="gvd")
+        # the N lowest Ritz vectors must be S-orthonormal to 1e-4 (an S too
+        # ill-conditioned for its Cholesky factor breaks this first)
+        CN = C[:, :N]
+        dev = np.abs(1.0 - np.real(np.einsum("ji,jk,ki->i", np.conj(CN), Ss, CN)))
+        if not np.all(dev <= RITZ_ORTH_TOL):
+            continue
         return lam, C, n3 == 3 * N
     n3 = 2 * N
-    lam, C = subspace_eig(Hm[:n3, :n3], S[:n3, :n3])
+    lam, C = subspace_eig(Hm[:n3, :n3], S[:n3, :n3], dropped_penalty(Hm, N))
     return lam, C, False
 
 
@@ -259,9 +277,10 @@ class LOBPCGRef:
             # Hermitian from the upper triangle of blocks (Conj_U_to_L, conjugate.h)
             S = np.triu(S) + np.triu(S, 1).conj().T
             Hm = np.triu(Hm) + np.triu(Hm, 1).conj().T
-            for j in np.nonzero(~rkeep)[0]:  # dropped R directions: decoupled, sorted last
+            pen = dropped_penalty(Hm, N)
+            for j in np.nonzero(~rkeep)[0]:  # dropped R directions: decoupled, sorted above the N lowest
                 S[N + j, N + j] = 1.0
-                Hm[N + j, N + j] = 1.0e30
+                Hm[N + j, N + j] = pen
             lam, C, used_p = rr_solve(Hm, S, N, have_p, self.cond)
             if not used_p:  # solved on [X R]: continue as if P were empty
                 S, Hm = S[:2 * N, :2 * N], Hm[:2 * N, :2 * N]

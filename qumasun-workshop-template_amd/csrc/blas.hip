@@ -15,6 +15,7 @@
 // band reductions: per-band sums over Ng (both SoA components) in FP64 for
 //         eps = Re <x|Ax>, |r|^2 with r = Ax - eps x, |x|^2 (innerprod_hermite.h:12-133).
 #include <algorithm>
+#include <cstdint>
 
 #include "qm_internal.h"
 #include "qm_mfma.h"
@@ -174,33 +175,47 @@ __global__ __launch_bounds__(256) void gram_partial_kernel(ColBlocks A, ColBlock
 
 // Large Gram products: workgroup tile 128 x 128 of C = A^T B over a K range,
 // panels of KT rows of the 256 columns staged through LDS (double-buffered),
-// 8 waves in a 2 x 4 grid, each 64 x 32 outputs (4 x 2 MFMA 16x16 tiles).
-// FP32: KT = 32 (128 B per column per panel), FP32 accumulators flushed into
-// FP64 registers every GT_FLUSH panels; FP64: KT = 16.
-constexpr int GT_M = 128, GT_N = 128, GT_THREADS = 512;
-constexpr int GT_FLUSH = 8;  // FP32 panels (of 32 K values) between FP64 flushes: 256 K values
+// WGM x WGN waves, each (128 / WGM) x (128 / WGN) outputs in 16 x 16 MFMA
+// tiles.  FP32 accumulators are flushed into FP64 registers every 256 K
+// values (FP32) ; FP64 accumulates in FP64.
+constexpr int GT_M = 128, GT_N = 128;
+constexpr int GT_FLUSH_K = 256;  // FP32 K values between FP32 -> FP64 flushes
 
-template <typename T>
+template <typename T, int WGM, int WGN, int KTV>
 struct GramTile {
-    static constexpr int KT = sizeof(T) == 4 ? 32 : 16;   // K rows per panel (128 B per column)
+    static constexpr int WM = WGM, WN = WGN;             // waves along m and n
+    static constexpr int THREADS = 64 * WGM * WGN;
+    static constexpr int KT = KTV;                       // K rows per panel
     static constexpr int VEC = 16 / sizeof(T);           // values per 16-byte load
     static constexpr int PAD = VEC;                      // LDS row padding (keeps 16-byte alignment)
     static constexpr int LD = KT + PAD;                  // LDS stride of one column
-    static constexpr int LOADS = (GT_M + GT_N) * KT / VEC / GT_THREADS;  // 16-byte loads per thread per panel
+    static constexpr int LOADS = (GT_M + GT_N) * KT / VEC / THREADS;  // 16-byte loads per thread per panel
+    static constexpr int TA = GT_M / WGM / 16;           // MFMA tiles per wave along m
+    static constexpr int TB = GT_N / WGN / 16;           // MFMA tiles per wave along n
+    static constexpr int FLUSH = GT_FLUSH_K / KT;        // panels between flushes (FP32)
+    static_assert(LOADS * VEC * THREADS == (GT_M + GT_N) * KT, "whole 16-byte loads per panel");
+    static_assert(GT_FLUSH_K % KT == 0, "flushes on panel boundaries");
 };
+// the shapes in use: FP32 16 waves (4 x 4, 32 x 32 per wave: 88 VGPRs, 4
+// waves per SIMD), 32 K rows per panel; FP64 8 waves (2 x 4, 64 x 32 per wave:
+// 126 VGPRs, two workgroups per CU), 16 K rows per panel (128 B per column).
+// On the 1536 x 1024 x 884,736 Gram the FP32 4 x 4 shape ran 23.8 ms against
+// 24.3 for 2 x 4 (same chunks), FP64 4 x 4 47.8 against 43.7.  Both give the
+// same bits: every output is the same MFMA chain over K.
+template <typename T>
+using GramShape = GramTile<T, sizeof(T) == 4 ? 4 : 2, 4, sizeof(T) == 4 ? 32 : 16>;
 
 // partial[chunk][j * m + i] = sum_{k in chunk} A[k][i] B[k][j] for the tile's
 // 128 x 128 outputs.  Requires ng % KT == 0 and kchunk % KT == 0 (16-byte
 // aligned columns and whole panels).
-template <typename T>
-__global__ __launch_bounds__(GT_THREADS) void gram_tile_kernel(ColBlocks A, ColBlocks B, int m, int n, int64_t ng,
+template <typename T, typename P>
+__global__ __launch_bounds__(P::THREADS) void gram_tile_kernel(ColBlocks A, ColBlocks B, int m, int n, int64_t ng,
                                                                int64_t kchunk, int tiles_n, int tiles, int nblk,
                                                                int per_xcd, double *__restrict__ partial) {
-    using P = GramTile<T>;
     using M = Mfma<T>;
     using acc_t = typename M::acc_t;
     using vec_t = typename M::vec_t;
-    constexpr int KT = P::KT, VEC = P::VEC, LD = P::LD;
+    constexpr int KT = P::KT, VEC = P::VEC, LD = P::LD, TA = P::TA, TB = P::TB, THREADS = P::THREADS;
     __shared__ __align__(16) T lds[2][GT_M + GT_N][LD];
     const int lb = xcd_block(nblk, per_xcd);
     if (lb < 0) return;
@@ -213,14 +228,14 @@ __global__ __launch_bounds__(GT_THREADS) void gram_tile_kernel(ColBlocks A, ColB
     const int npanel = (int)((k1 - k0) / KT);
 
     // loader: thread t loads 16-byte pieces (VEC values) of columns; KT / VEC
-    // consecutive threads cover one column's panel rows (one 128-byte line)
+    // consecutive threads cover one column's panel rows
     constexpr int PER_COL = KT / VEC;
     const T *src[P::LOADS];
     int dcol[P::LOADS];
     const int piece = threadIdx.x % PER_COL;
 #pragma unroll
     for (int q = 0; q < P::LOADS; ++q) {
-        const int c = threadIdx.x / PER_COL + q * (GT_THREADS / PER_COL);  // 0 .. 255
+        const int c = threadIdx.x / PER_COL + q * (THREADS / PER_COL);  // 0 .. 255
         dcol[q] = c;
         const T *p = nullptr;
         if (c < GT_M) {
@@ -245,14 +260,14 @@ __global__ __launch_bounds__(GT_THREADS) void gram_tile_kernel(ColBlocks A, ColB
 
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int li = lane & 15, g = lane >> 4;
-    const int wm = (wave >> 2) * 64;  // wave's first output row in the tile (2 x 4 waves)
-    const int wn = (wave & 3) * 32;   // wave's first output column in the tile
-    acc_t acc[4][2];
-    double acc64[4][2][4];
+    const int wm = (wave / P::WN) * 16 * TA;  // wave's first output row in the tile
+    const int wn = (wave % P::WN) * 16 * TB;  // wave's first output column in the tile
+    acc_t acc[TA][TB];
+    double acc64[TA][TB][4];
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < TA; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
+        for (int b = 0; b < TB; ++b) {
             acc[a][b] = acc_t{0, 0, 0, 0};
 #pragma unroll
             for (int r = 0; r < 4; ++r) acc64[a][b][r] = 0.0;
@@ -269,26 +284,26 @@ __global__ __launch_bounds__(GT_THREADS) void gram_tile_kernel(ColBlocks A, ColB
         // K steps of 4 VEC values: lane group g holds k = 4g .. (VEC values per lane)
 #pragma unroll
         for (int ks = 0; ks < KT; ks += 4 * VEC) {
-            vec_t av[4], bv[2];
+            vec_t av[TA], bv[TB];
 #pragma unroll
-            for (int a = 0; a < 4; ++a)
+            for (int a = 0; a < TA; ++a)
                 av[a] = *reinterpret_cast<const vec_t *>(&lds[cur][wm + 16 * a + li][ks + VEC * g]);
 #pragma unroll
-            for (int b = 0; b < 2; ++b)
+            for (int b = 0; b < TB; ++b)
                 bv[b] = *reinterpret_cast<const vec_t *>(&lds[cur][GT_M + wn + 16 * b + li][ks + VEC * g]);
 #pragma unroll
             for (int c = 0; c < VEC; ++c)
 #pragma unroll
-                for (int a = 0; a < 4; ++a)
+                for (int a = 0; a < TA; ++a)
 #pragma unroll
-                    for (int b = 0; b < 2; ++b) acc[a][b] = M::mma(av[a][c], bv[b][c], acc[a][b]);
+                    for (int b = 0; b < TB; ++b) acc[a][b] = M::mma(av[a][c], bv[b][c], acc[a][b]);
         }
         if constexpr (sizeof(T) == 4) {
-            if ((p + 1) % GT_FLUSH == 0) {
+            if ((p + 1) % P::FLUSH == 0) {
 #pragma unroll
-                for (int a = 0; a < 4; ++a)
+                for (int a = 0; a < TA; ++a)
 #pragma unroll
-                    for (int b = 0; b < 2; ++b) {
+                    for (int b = 0; b < TB; ++b) {
 #pragma unroll
                         for (int r = 0; r < 4; ++r) acc64[a][b][r] += (double)acc[a][b][r];
                         acc[a][b] = acc_t{0, 0, 0, 0};
@@ -300,9 +315,9 @@ __global__ __launch_bounds__(GT_THREADS) void gram_tile_kernel(ColBlocks A, ColB
     }
     double *out = partial + (int64_t)chunk * m * n;
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < TA; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
+        for (int b = 0; b < TB; ++b) {
             const int j = tj * GT_N + wn + 16 * b + li;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
@@ -396,22 +411,26 @@ __global__ __launch_bounds__(256) void update_kernel(ColBlocks Z, ColBlocks Y, c
 // group.
 constexpr int UT_N = 128, UT_P = 128, UT_THREADS = 512;
 
-template <typename T>
+template <typename T, int KTV, int PADV>
 struct UpdTile {
-    static constexpr int KT = sizeof(T) == 4 ? 32 : 16;  // K rows (Z columns) per panel
+    static constexpr int KT = KTV;                                   // K rows (Z columns) per panel
     static constexpr int VEC = 16 / sizeof(T);
     static constexpr int ZLOADS = KT * UT_P / VEC / UT_THREADS;  // 16-byte Z loads per thread per panel
     static constexpr int CLOADS = KT * UT_N / VEC / UT_THREADS;  // 16-byte C loads per thread per panel
-    static constexpr int PAD = 4;                                // LDS row padding (values)
+    static constexpr int PAD = PADV;                             // LDS row padding (values)
+    static_assert(ZLOADS * VEC * UT_THREADS == KT * UT_P && CLOADS * VEC * UT_THREADS == KT * UT_N,
+                  "whole 16-byte loads per panel");
 };
+// the shape in use: 32 (FP32) / 16 (FP64) K rows per panel, rows padded by 4
+template <typename T>
+using UpdShape = UpdTile<T, sizeof(T) == 4 ? 32 : 16, 4>;
 
 // Requires ng % UT_P == 0 and m % KT == 0 (whole tiles and panels; the caller
 // checks) and 16-byte aligned column blocks.
-template <typename T>
+template <typename T, typename P>
 __global__ __launch_bounds__(UT_THREADS) void update_tile_kernel(ColBlocks Z, ColBlocks Y, const T *__restrict__ Cm,
                                                                  int m, int n, int64_t ng, int tiles_n, int nblk,
                                                                  int per_xcd) {
-    using P = UpdTile<T>;
     using M = Mfma<T>;
     using acc_t = typename M::acc_t;
     using vec_t = typename M::vec_t;
@@ -591,21 +610,66 @@ int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
 // LDS-tiled 128 x 128 kernel: each panel row is read once per workgroup tile
 // and shared by its 8 waves, and the tiles of one tile row (the same A panel)
 // run consecutively on one XCD.  K is split so that about 1024 workgroups run.
+// K rows per chunk are a multiple of this (whole flush intervals for FP32)
 template <typename T>
+constexpr int gram_kround() { return sizeof(T) == 4 ? GT_FLUSH_K : 128; }
+
+// K chunks of the tiled Gram, from a cost model fitted to measured launches:
+//   time = per-CU rounds x chunk length x (per-workgroup time per K row)
+//        + the partials' write and reduction (16 B per element and chunk, ~2.6 TB/s).
+// Each CU takes b = ceil(blocks / CUs) workgroups, per_cu of them at a time; a
+// last, partly filled round costs its share plus 14 % (a lone FP64 workgroup
+// hides less latency than a pair: measured 0.57 of a pair's time).  A
+// workgroup-round advances one K row of a 128 x 128 tile (32,768 flops) at ~0.8
+// of the CU's MFMA peak.  The cheapest chunk count is taken, the smallest one
+// on ties, with at most 256 chunks and 512 MB of partials.  Measured on the
+// LOBPCG shapes: 384 x 256 over 64^3 points, FP32 128 chunks (0.52 ms; the
+// previous 147: 0.66 ms), FP64 128 (0.87 ms; 158: 1.01 ms); 1536 x 1024 over
+// 96^3, FP32 8 (23.8 ms; 10: 25.3), FP64 16 (40.8 ms; 10: 43.7).
+inline int64_t gram_chunks(int tiles, int64_t ng, int64_t kround, int cus, int per_cu, double cu_flops_us,
+                           int64_t mn, int64_t *kchunk_out) {
+    const double t_k = per_cu * 32768.0 / (0.8 * cu_flops_us);  // us per K row of a full round
+    int64_t best_nc = 1, best_k = round_up(ng, kround);
+    double best_cost = 1e300;
+    for (int64_t nc = 1; nc <= 256; ++nc) {
+        const int64_t kchunk = round_up((ng + nc - 1) / nc, kround);
+        const int64_t act = (ng + kchunk - 1) / kchunk;
+        if (act < nc) continue;  // the same partition as a smaller count
+        if (act > 1 && act * mn * (int64_t)sizeof(double) > ((int64_t)512 << 20)) break;
+        const int64_t b = (tiles * act + cus - 1) / cus;
+        const double rounds = (double)(b / per_cu) + std::min(1.0, 1.14 * (double)(b % per_cu) / per_cu);
+        const double cost = rounds * (double)kchunk * t_k + (double)act * mn * 16.0 / 2.6e6;
+        if (cost < best_cost) {
+            best_cost = cost;
+            best_nc = act;
+            best_k = kchunk;
+        }
+    }
+    *kchunk_out = best_k;
+    return best_nc;
+}
+
+template <typename T, typename P = GramShape<T>>
 int gram_tiled(DevBuf &partial, double *C, const ColBlocks &A, const ColBlocks &B, int m, int n, int64_t ng,
                hipStream_t st) {
-    constexpr int KT = GramTile<T>::KT;
+    static_assert(gram_kround<T>() % P::KT == 0, "chunks of whole panels");
     const int tiles_m = (m + GT_M - 1) / GT_M, tiles_n = (n + GT_N - 1) / GT_N;
     const int tiles = tiles_m * tiles_n;
-    int64_t nchunk = std::max<int64_t>(1, std::min<int64_t>(256, 1024 / tiles));
-    int64_t kchunk = round_up((ng + nchunk - 1) / nchunk, (int64_t)KT * GT_FLUSH);
-    nchunk = (ng + kchunk - 1) / kchunk;
+    int dev = 0, cus = 0, per_cu = 0;
+    QM_HIP(hipGetDevice(&dev));
+    QM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    QM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, gram_tile_kernel<T, P>, P::THREADS, 0));
+    // dense MFMA peak per CU and microsecond: FP32 157.3, FP64 78.6 TFLOP/s over 256 CUs
+    const double cu_flops_us = (sizeof(T) == 4 ? 157.3e6 : 78.6e6) / 256.0;
+    int64_t kchunk = 0;
+    const int64_t nchunk = gram_chunks(tiles, ng, gram_kround<T>(), std::max(cus, 1), std::max(per_cu, 1),
+                                       cu_flops_us, (int64_t)m * n, &kchunk);
     QM_TRY(partial.reserve(sizeof(double) * (size_t)nchunk * m * n));
     const int64_t nblk = (int64_t)tiles * nchunk;
     QM_CHECK(nblk < ((int64_t)1 << 30), QM_ERR_ARG, "gram: grid too large");
     const int per_xcd = (int)((nblk + 7) / 8);
-    hipLaunchKernelGGL((gram_tile_kernel<T>), dim3((unsigned)(8 * per_xcd)), dim3(GT_THREADS), 0, st, A, B, m, n, ng,
-                       kchunk, tiles_n, tiles, (int)nblk, per_xcd, partial.as<double>());
+    hipLaunchKernelGGL((gram_tile_kernel<T, P>), dim3((unsigned)(8 * per_xcd)), dim3(P::THREADS), 0, st, A, B, m, n,
+                       ng, kchunk, tiles_n, tiles, (int)nblk, per_xcd, partial.as<double>());
     QM_HIP(hipGetLastError());
     const int64_t mn = (int64_t)m * n;
     hipLaunchKernelGGL(gram_reduce_kernel, dim3((unsigned)((mn + 255) / 256)), dim3(256), 0, st, C,
@@ -625,7 +689,7 @@ template <typename T>
 int gram(DevBuf &partial, double *C, const ColBlocks &A, const ColBlocks &B, int m, int n, int64_t ng,
          hipStream_t st) {
     const bool al = aligned16(A) && aligned16(B);
-    if (al && m >= GT_M && n >= GT_N && ng % GramTile<T>::KT == 0)
+    if (al && m >= GT_M && n >= GT_N && ng % GramShape<T>::KT == 0)
         return gram_tiled<T>(partial, C, A, B, m, n, ng, st);
     constexpr int KSTEP = 4 * Mfma<T>::VEC;
     constexpr int TN = kGramTN;  // 16-column MFMA tiles per wave along n
@@ -658,19 +722,23 @@ int gram(DevBuf &partial, double *C, const ColBlocks &A, const ColBlocks &B, int
     return QM_OK;
 }
 
+template <typename T, typename P = UpdShape<T>>
+int update_tiled(const ColBlocks &Z, const ColBlocks &Y, const T *Cm, int m, int n, int64_t ng, hipStream_t st) {
+    const int tiles_n = (n + UT_N - 1) / UT_N;
+    const int64_t nblk = (ng / UT_P) * tiles_n;
+    QM_CHECK(nblk < ((int64_t)1 << 30), QM_ERR_ARG, "update: grid too large");
+    const int per_xcd = (int)((nblk + 7) / 8);
+    hipLaunchKernelGGL((update_tile_kernel<T, P>), dim3((unsigned)(8 * per_xcd)), dim3(UT_THREADS), 0, st, Z, Y, Cm, m,
+                       n, ng, tiles_n, (int)nblk, per_xcd);
+    QM_HIP(hipGetLastError());
+    return QM_OK;
+}
+
 template <typename T>
 int update(const ColBlocks &Z, const ColBlocks &Y, const T *Cm, int m, int n, int64_t ng, hipStream_t st) {
-    if (n >= UT_N && ng % UT_P == 0 && m % UpdTile<T>::KT == 0 && aligned16(Z) && aligned16(Y) &&
-        reinterpret_cast<uintptr_t>(Cm) % 16 == 0) {
-        const int tiles_n = (n + UT_N - 1) / UT_N;
-        const int64_t nblk = (ng / UT_P) * tiles_n;
-        QM_CHECK(nblk < ((int64_t)1 << 30), QM_ERR_ARG, "update: grid too large");
-        const int per_xcd = (int)((nblk + 7) / 8);
-        hipLaunchKernelGGL((update_tile_kernel<T>), dim3((unsigned)(8 * per_xcd)), dim3(UT_THREADS), 0, st, Z, Y, Cm,
-                           m, n, ng, tiles_n, (int)nblk, per_xcd);
-        QM_HIP(hipGetLastError());
-        return QM_OK;
-    }
+    if (n >= UT_N && ng % UT_P == 0 && m % UpdShape<T>::KT == 0 && aligned16(Z) && aligned16(Y) &&
+        reinterpret_cast<uintptr_t>(Cm) % 16 == 0)
+        return update_tiled<T>(Z, Y, Cm, m, n, ng, st);
     const int tiles_n = (n + 63) / 64;
     const int64_t gtiles = (ng + 127) / 128;
     const int64_t nblk = gtiles * tiles_n;

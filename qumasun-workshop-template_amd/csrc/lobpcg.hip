@@ -43,6 +43,10 @@ namespace {
 
 constexpr int kRestLimit = 6;          // S_MAT_REST_LIMIT
 constexpr double kLimitNormX = 1.0e-3; // LIMIT_Z_NORM_X_FOR_RESET_MATRIX
+// Ritz vectors of the Cholesky route are accepted when max_i |1 - c_i^H S c_i|
+// stays below this (S-orthonormality; an S too ill-conditioned for its
+// Cholesky factor breaks it long before the pivot ratio shows it)
+constexpr double kRitzOrthTol = 1.0e-4;
 
 // ------------------------------------------------------------------ small FP64 matrices
 // Complex elements are interleaved (re, im) pairs, as rocblas_double_complex.
@@ -108,10 +112,38 @@ __global__ void extract_blocks_kernel(typename Sm<CPLX>::t *blk, const double *G
 // S and H (n3 = nbk*N, full Hermitian, column-major) from the kept blocks and
 // G1 = [X R P]^T [R AR] (real, ld1 rows, columns 2*c*N).  R rows/columns are
 // scaled by dr = 1/|r|.
+// Diagonal given to dropped directions (decoupled R columns, SVQB null
+// directions) so that they sort above the N lowest Ritz values: twice a
+// Gershgorin bound of the X block's H (above its largest Ritz value, which
+// bounds the N-th Ritz value of any subspace containing X) plus one.  A fixed
+// huge value (1e30) would be exact in exact arithmetic, but a dense eigensolver's
+// backward error (eps |M|, 1e14 for 1e30) then lands on every eigenvalue.
+template <bool CPLX>
+__global__ __launch_bounds__(256) void penalty_kernel(double *pen, const typename Sm<CPLX>::t *hxx, int N) {
+    __shared__ double red[256];
+    double m = 0.0;
+    for (int i = threadIdx.x; i < N; i += 256) {
+        double r = 0.0;
+        for (int j = 0; j < N; ++j) {
+            const auto v = hxx[i + (int64_t)j * N];
+            if constexpr (CPLX) r += sqrt(v.re * v.re + v.im * v.im);
+            else r += fabs(v);
+        }
+        m = fmax(m, r);
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) pen[0] = 2.0 * red[0] + 1.0;
+}
+
 template <bool CPLX>
 __global__ void assemble_kernel(typename Sm<CPLX>::t *Sm_, typename Sm<CPLX>::t *Hm_,
                                 const typename Sm<CPLX>::t *blk, const double *G1, const double *dr, int N, int c,
-                                int nbk, int ld1, const double *rkeep) {
+                                int nbk, int ld1, const double *rkeep, const double *pen) {
     using S = Sm<CPLX>;
     using t = typename S::t;
     const int n3 = nbk * N;
@@ -149,10 +181,10 @@ __global__ void assemble_kernel(typename Sm<CPLX>::t *Sm_, typename Sm<CPLX>::t 
     if (rkeep && I == J && bi == 1 && rkeep[I - N] == 0.0) {  // R column dropped by the orthonormalisation
         if constexpr (CPLX) {
             s = {1.0, 0.0};
-            h = {1.0e30, 0.0};
+            h = {*pen, 0.0};
         } else {
             s = 1.0;
-            h = 1.0e30;
+            h = *pen;
         }
     }
     Sm_[e] = s;
@@ -365,6 +397,55 @@ __global__ void pivot_ratio_kernel(double *out, const typename Sm<CPLX>::t *U, i
     if (threadIdx.x == 0) out[0] = mx[0] > 0.0 ? mn[0] / mx[0] : 0.0;
 }
 
+// dev[i] = |1 - c_i^H S c_i| for the N lowest Ritz vectors (block i; columns
+// of C, leading dimension n3; S full, column-major); non-finite -> 1e300
+template <bool CPLX>
+__global__ __launch_bounds__(256) void ritz_orth_kernel(double *dev, const typename Sm<CPLX>::t *S,
+                                                        const typename Sm<CPLX>::t *C, int n3) {
+    __shared__ double red[256];
+    const auto *c = C + (int64_t)blockIdx.x * n3;
+    double acc = 0.0;
+    for (int j = threadIdx.x; j < n3; j += 256) {
+        if constexpr (CPLX) {
+            double sr = 0.0, si = 0.0;
+            for (int k = 0; k < n3; ++k) {
+                const Cplx a = S[j + (int64_t)k * n3], b = c[k];
+                sr += a.re * b.re - a.im * b.im;
+                si += a.re * b.im + a.im * b.re;
+            }
+            acc += c[j].re * sr + c[j].im * si;  // Re(conj(c_j) (S c)_j)
+        } else {
+            double sc = 0.0;
+            for (int k = 0; k < n3; ++k) sc += S[j + (int64_t)k * n3] * c[k];
+            acc += c[j] * sc;
+        }
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double d = fabs(1.0 - red[0]);
+        dev[blockIdx.x] = d == d && d < 1.0e300 ? d : 1.0e300;
+    }
+}
+
+// dev[0] = max(dev[0 .. n-1]) (one block)
+__global__ __launch_bounds__(256) void max_reduce_kernel(double *dev, int n) {
+    __shared__ double red[256];
+    double m = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) m = fmax(m, dev[i]);
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) dev[0] = red[0];
+}
+
 // SVQB fallback: columns of the S eigenvectors scaled by s^-1/2, near-null
 // directions (s <= tau * s_max) zeroed; w[j] records the scale (0 = dropped).
 template <bool CPLX>
@@ -379,14 +460,15 @@ __global__ void svqb_basis_kernel(typename Sm<CPLX>::t *V, double *w, const doub
     if (e % n3 == 0) w[j] = wj;
 }
 
-// dropped directions of the reduced H get a huge diagonal so they sort last
+// dropped directions of the reduced H (zero rows and columns) get the
+// penalty diagonal so they sort above the N lowest
 template <bool CPLX>
-__global__ void svqb_penalty_kernel(typename Sm<CPLX>::t *Hr, const double *w, int n3) {
+__global__ void svqb_penalty_kernel(typename Sm<CPLX>::t *Hr, const double *w, int n3, const double *pen) {
     const int i = blockIdx.x * 64 + threadIdx.x;
     if (i >= n3) return;
     if (w[i] == 0.0) {
-        if constexpr (CPLX) Hr[(int64_t)i * n3 + i].re += 1.0e30;
-        else Hr[(int64_t)i * n3 + i] += 1.0e30;
+        if constexpr (CPLX) Hr[(int64_t)i * n3 + i].re += *pen;
+        else Hr[(int64_t)i * n3 + i] += *pen;
     }
 }
 
@@ -424,6 +506,7 @@ struct qm_lobpcg {
     int *info = nullptr;
     double *trd = nullptr, *VW = nullptr;  // tridiagonal reduction workspace (real subspace problems)
     bool evec_t1 = false;                  // last chol_solve left the eigenvectors in T1 (else Hw)
+    int ortho_rejects = 0;                 // Cholesky-route solves rejected for S-orthonormality (last call)
     qm::DevBuf gram_part, red_part;
     rocblas_handle blas = nullptr;
     double t[6] = {0, 0, 0, 0, 0, 0};
@@ -433,6 +516,7 @@ struct qm_lobpcg {
     std::vector<int> ev_phase;
     size_t ev_used = 0;
     double *evk = nullptr;   // Ritz values of the last accepted iteration (device, N)
+    double *pen = nullptr;   // diagonal of dropped directions in the subspace problem (device, 1)
     bool graphs = true;      // replay A.R + Gram of an iteration as a HIP graph
     hipStream_t gstream = nullptr;  // capture stream (graphs launch on the caller's stream)
     int graph_replays = 0;   // replays in the last solve
@@ -461,6 +545,8 @@ void lfree(qm_lobpcg *s) {
     s->info = nullptr;
     if (s->evk) (void)hipFree(s->evk);
     s->evk = nullptr;
+    if (s->pen) (void)hipFree(s->pen);
+    s->pen = nullptr;
     if (s->gstream) (void)hipStreamDestroy(s->gstream);
     s->gstream = nullptr;
     for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
@@ -687,7 +773,7 @@ int svqb_eig(qm_lobpcg *s, int n3, double tau, hipStream_t st) {
         QM_BLAS(rocblas_zgemm(s->blas, rocblas_operation_conjugate_transpose, rocblas_operation_none, n3, n3, n3,
                               &one, B, n3, T1, n3, &zero, Hr, n3));
         hipLaunchKernelGGL((svqb_penalty_kernel<CPLX>), dim3((n3 + 63) / 64), dim3(64), 0, st, (SmT *)s->Hw, s->svw,
-                           n3);
+                           n3, s->pen);
         QM_HIP(hipGetLastError());
         QM_TRY(heev_all<CPLX>(s, n3, Hr, s->evals, st));
         QM_BLAS(rocblas_zgemm(s->blas, rocblas_operation_none, rocblas_operation_none, n3, n3, n3, &one, B, n3, Hr,
@@ -700,7 +786,7 @@ int svqb_eig(qm_lobpcg *s, int n3, double tau, hipStream_t st) {
         QM_BLAS(rocblas_dgemm(s->blas, rocblas_operation_transpose, rocblas_operation_none, n3, n3, n3, &one, B, n3,
                               T1, n3, &zero, Hr, n3));
         hipLaunchKernelGGL((svqb_penalty_kernel<CPLX>), dim3((n3 + 63) / 64), dim3(64), 0, st, (SmT *)s->Hw, s->svw,
-                           n3);
+                           n3, s->pen);
         QM_HIP(hipGetLastError());
         QM_TRY(heev_all<CPLX>(s, n3, Hr, s->evals, st));
         QM_BLAS(rocblas_dgemm(s->blas, rocblas_operation_none, rocblas_operation_none, n3, n3, n3, &one, B, n3, Hr,
@@ -964,6 +1050,8 @@ int solve_impl(qm_lobpcg *s, T *x_user, double *eigenvalues, double *residual_no
     s->fallbacks = 0;
     s->rdrops = 0;
     s->stedc_retries = 0;
+    s->ortho_rejects = 0;
+    s->eig_note.clear();
     const double t_start = now_ms();
     // end of a phase: t[ph] receives the stream time since the previous mark
     s->ev_used = 0;
@@ -1122,11 +1210,14 @@ int solve_impl(qm_lobpcg *s, T *x_user, double *eigenvalues, double *residual_no
         int nbk_used = nbk;
         const void *evec = s->Hw;
         const double cond_limit = sizeof(T) == 8 ? s->cond64 : s->cond32;
+        hipLaunchKernelGGL((penalty_kernel<CPLX>), dim3(1), dim3(256), 0, st, s->pen,
+                           (const SmT *)s->blk + (int64_t)B_XAX * N * N, N);
+        QM_HIP(hipGetLastError());
         for (int attempt = 0;; ++attempt) {
             const int n3a = nbk_used * N;
             hipLaunchKernelGGL((assemble_kernel<CPLX>), dim3(nblk((int64_t)n3a * n3a)), dim3(256), 0, st,
                                (SmT *)s->S, (SmT *)s->H, (const SmT *)s->blk, s->G1, s->dr, N, c, nbk_used,
-                               nbk * nc, s->rkeep);
+                               nbk * nc, s->rkeep, s->pen);
             QM_HIP(hipGetLastError());
             QM_TRY(chol_factor<CPLX>(s, n3a, st));
             int h_info = 0;
@@ -1137,16 +1228,33 @@ int solve_impl(qm_lobpcg *s, T *x_user, double *eigenvalues, double *residual_no
             if (h_info == 0 && h_ratio >= cond_limit) {
                 QM_TRY(chol_solve<CPLX>(s, n3a, st));
                 // a tridiagonal eigensolve that does not converge (a barely
-                // definite S makes U^-H H U^-1 badly scaled) is handled like a
+                // definite S makes U^-H H U^-1 badly scaled), or Ritz vectors
+                // that have lost their S-orthonormality (S too ill-conditioned
+                // for its Cholesky factor: the update would fail its norm check
+                // and its Ritz values can be spurious), are handled like a
                 // failed Cholesky: without P, then the S eigenbasis
+                const SmT *cv = (const SmT *)(s->evec_t1 ? s->T1 : s->Hw);
+                hipLaunchKernelGGL((ritz_orth_kernel<CPLX>), dim3(N), dim3(256), 0, st, s->svw, (const SmT *)s->S, cv,
+                                   n3a);
+                QM_HIP(hipGetLastError());
+                hipLaunchKernelGGL(max_reduce_kernel, dim3(1), dim3(256), 0, st, s->svw, N);
+                QM_HIP(hipGetLastError());
+                double h_dev = 0.0;
                 QM_HIP(hipMemcpyAsync(&h_info, s->info, sizeof(int), hipMemcpyDeviceToHost, st));
+                QM_HIP(hipMemcpyAsync(&h_dev, s->svw, sizeof(double), hipMemcpyDeviceToHost, st));
                 QM_HIP(hipStreamSynchronize(st));
-                if (h_info == 0) {
-                    evec = s->evec_t1 ? s->T1 : s->Hw;
+                if (h_info == 0 && h_dev <= kRitzOrthTol) {
+                    evec = cv;
                     break;
                 }
-                s->eig_note += " (Cholesky-route solve of order " + std::to_string(n3a) + " info " +
-                               std::to_string(h_info) + ")";
+                if (h_info == 0) {
+                    ++s->ortho_rejects;
+                    s->eig_note += " (Cholesky-route Ritz vectors of order " + std::to_string(n3a) +
+                                   " off S-orthonormal by " + std::to_string(h_dev) + ")";
+                } else {
+                    s->eig_note += " (Cholesky-route solve of order " + std::to_string(n3a) + " info " +
+                                   std::to_string(h_info) + ")";
+                }
             }
             if (nbk_used == 3) {  // drop P and retry
                 nbk_used = 2;
@@ -1352,6 +1460,7 @@ int qm_lobpcg_create(qm_lobpcg **out, qm_ctx *ctx, int nbands, int ncomp, int pr
         if (st == QM_OK) st = dalloc((void **)p, sizeof(double) * n3);
     if (st == QM_OK) st = dalloc((void **)&s->info, sizeof(int));
     if (st == QM_OK) st = dalloc((void **)&s->evk, sizeof(double) * N);
+    if (st == QM_OK) st = dalloc((void **)&s->pen, sizeof(double));
     if (st == QM_OK && hipStreamCreateWithFlags(&s->gstream, hipStreamNonBlocking) != hipSuccess) {
         s->gstream = nullptr;
         set_error("qm_lobpcg_create: hipStreamCreateWithFlags failed");
@@ -1444,7 +1553,7 @@ int qm_lobpcg_eigen_retries(qm_lobpcg *s, char *note, int len) {
     if (note && len > 0) {
         std::snprintf(note, (size_t)len, "%s", s->eig_note.c_str());
     }
-    return s->stedc_retries;
+    return s->stedc_retries + s->ortho_rejects;
 }
 
 int qm_lobpcg_set_rollback_limit(qm_lobpcg *s, double limit) {
