@@ -421,9 +421,14 @@ struct UpdTile {
     static_assert(ZLOADS * VEC * UT_THREADS == KT * UT_P && CLOADS * VEC * UT_THREADS == KT * UT_N,
                   "whole 16-byte loads per panel");
 };
-// the shape in use: 32 (FP32) / 16 (FP64) K rows per panel, rows padded by 4
+// the shape in use: 16 (FP32) / 8 (FP64) K rows per panel, rows padded by 4.
+// Shallow panels leave room for more resident workgroups (FP32: 64 VGPRs,
+// 34 KB of LDS): the 1536 -> 1024 column update over 884,736 points ran
+// 21.4 ms against 23.2 with 32 K rows (FP64 44.6 against 45.1 with 16), the
+// 384 -> 256 one over 262,144 points 0.49 against 0.57 ms (FP64 0.93 against
+// 1.02); 64 K rows and wider padding were slower.  Same bits for every depth.
 template <typename T>
-using UpdShape = UpdTile<T, sizeof(T) == 4 ? 32 : 16, 4>;
+using UpdShape = UpdTile<T, sizeof(T) == 4 ? 16 : 8, 4>;
 
 // Requires ng % UT_P == 0 and m % KT == 0 (whole tiles and panels; the caller
 // checks) and 16-byte aligned column blocks.
