@@ -104,6 +104,7 @@ static void free_projectors(Projectors &p) {
     dfree(p.d_tabp_32);
     dfree(p.d_chunk_atom);
     dfree(p.d_chunk_v0);
+    dfree(p.d_item_order);
     dfree(p.d_atom_chunk_off);
     dfree(p.d_proj_atom);
     dfree(p.d_block_atom_off);
@@ -259,6 +260,21 @@ static int build_projectors(const qm_ctx *ctx, Projectors &p, int natoms, const 
             acoff[a + 1] = (int)catom.size();
         }
         p.nchunk = (int)catom.size();
+        // launch order of the work items: by the (wrapped) z, then y, of each
+        // chunk's first voxel, so that the chunks of neighbouring atoms that
+        // read the same grid lines run together on one XCD and share its L2
+        std::vector<int> iord(p.nchunk);
+        {
+            std::vector<int64_t> key(p.nchunk);
+            for (int it = 0; it < p.nchunk; ++it) {
+                const int64_t v = vox_off[catom[it]] + cv0[it];
+                key[it] = (int64_t)wrap_host(vox_xyz[3 * v + 2], ctx->nz) * ctx->ny +
+                          wrap_host(vox_xyz[3 * v + 1], ctx->ny);
+            }
+            std::iota(iord.begin(), iord.end(), 0);
+            std::stable_sort(iord.begin(), iord.end(), [&](int a, int b) { return key[a] < key[b]; });
+        }
+        QM_TRY(upload(&p.d_item_order, iord.data(), iord.size()));
         QM_TRY(upload(&p.d_tabp_64, tp.data(), tp.size()));
         QM_TRY(upload(&p.d_tabp_32, tpf.data(), tpf.size()));
         QM_TRY(upload(&p.d_chunk_atom, catom.data(), catom.size()));

@@ -68,7 +68,8 @@ __global__ __launch_bounds__(256) void nl_project_kernel(double *__restrict__ pa
                                                          const int *__restrict__ chunk_v0,
                                                          const int64_t *__restrict__ vox_off,
                                                          const int *__restrict__ vox_grid, const T *__restrict__ tabp,
-                                                         const T *__restrict__ phase, int groups, int nblk,
+                                                         const T *__restrict__ phase,
+                                                         const int *__restrict__ item_order, int groups, int nblk,
                                                          int per_xcd) {
     constexpr int MPAD = 16 * MT;
     using M = Mfma<T>;
@@ -77,8 +78,9 @@ __global__ __launch_bounds__(256) void nl_project_kernel(double *__restrict__ pa
     __shared__ double red[4][16 * 16];
     const int lb = xcd_block(nblk, per_xcd);
     if (lb < 0) return;  // the whole workgroup
-    const int item = lb / groups;
-    const int c0 = (lb - item * groups) * 16 * GT;
+    const int slot = lb / groups;
+    const int item = item_order[slot];
+    const int c0 = (lb - slot * groups) * 16 * GT;
     const int a = chunk_atom[item];
     const int64_t vg0 = vox_off[a];
     const int V = (int)(vox_off[a + 1] - vg0);
@@ -444,8 +446,8 @@ void launch_project(qm_ctx *ctx, double *part, const T *psi, int ncol, const T *
     const int64_t nblk = (int64_t)p.nchunk * groups;
     const int per_xcd = (int)((nblk + 7) / 8);
     hipLaunchKernelGGL((nl_project_kernel<T, NC, PH, GT, MT>), dim3(8 * per_xcd), dim3(256), 0, st, part, psi, ctx->ng,
-                       ncol, p.d_chunk_atom, p.d_chunk_v0, p.d_vox_off, p.d_vox_grid, tabp_of<T>(p), phase, groups,
-                       (int)nblk, per_xcd);
+                       ncol, p.d_chunk_atom, p.d_chunk_v0, p.d_vox_off, p.d_vox_grid, tabp_of<T>(p), phase,
+                       p.d_item_order, groups, (int)nblk, per_xcd);
 }
 
 // Projection coefficients of nb bands into coef (ptot x ncol, T) through the

@@ -92,6 +92,7 @@ struct Projectors {
     // projection work items: chunks of kProjChunk voxels of one atom
     int nchunk = 0;
     int *d_chunk_atom = nullptr, *d_chunk_v0 = nullptr;
+    int *d_item_order = nullptr;  // launch order of the projection work items (a permutation)
     int *d_atom_chunk_off = nullptr;  // [natoms + 1]
     int *d_proj_atom = nullptr;       // [ptot] owning atom of each projector
     // scatter blocks of kScatterBlock covered points: the atoms they touch
