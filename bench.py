@@ -303,6 +303,8 @@ def main():
                     help="profiling mode: N Gram products (3 x 2 blocks) and N updates at the 96^3 x 256 complex-band "
                          "LOBPCG shape (real-packed: m = 1536, n = 1024, K = 884,736) in FP32 and FP64, timed with "
                          "HIP events, then exit (no JSON line; for rocprofv3 runs)")
+    ap.add_argument("--k-index", type=int, default=0,
+                    help="k-point of the workload for --stencil-candidates / --nonlocal-only runs (default 0)")
     ap.add_argument("--stencil-candidates", type=int, default=0, metavar="N",
                     help="profiling mode: N FP32 local applies with each stencil tile shape in turn, then exit "
                          "(no JSON line; for rocprofv3 counter runs)")
@@ -329,14 +331,18 @@ def main():
     x = torch.from_numpy(psi64).to(device="cuda:0", dtype=torch.float32)
     y = torch.empty_like(x)
 
+    kidx = args.k_index  # 0 (the first k-point) for the headline and the timed legs
+    if not 0 <= kidx < s.kpoints.shape[0]:
+        sys.exit(f"bench.py: --k-index {kidx} outside the workload's {s.kpoints.shape[0]} k-point(s)")
+
     def full():
-        H.apply(y, x)
+        H.apply(y, x, k_index=kidx)
 
     def local():
-        H.apply(y, x, part="local")
+        H.apply(y, x, part="local", k_index=kidx)
 
     def nonlocal_():
-        H.apply(y, x, part="nonlocal")
+        H.apply(y, x, part="nonlocal", k_index=kidx)
 
     if args.copy_calibration:
         # 128 Mi floats: 512 MiB read + 512 MiB written by one copy kernel
