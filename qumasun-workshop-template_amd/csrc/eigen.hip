@@ -531,7 +531,13 @@ int chol_back(void *blas, int n, int nev, const double *Ui, const double *Z, int
     return QM_OK;
 }
 
-int potrf_upper(void *blas, int n, double *A, int lda, int *info, hipStream_t st) {
+int potrf_upper(void *blas, int n, double *A, int lda, int *info, hipStream_t st, void *sync) {
+    // one workgroup per 32-row panel where the caller gives the barrier words
+    // (and the subspace route is not forced onto the single-workgroup kernels)
+    if (sync && subspace_route() < 0 && potrf_coop_supported(n)) {
+        const int rc = potrf_coop_f64(n, A, lda, info, sync, st);
+        if (rc != QM_ERR_STATE) return rc;
+    }
     if (n > kMaxN) {
         QM_EIG_BLAS(rocsolver_dpotrf((rocblas_handle)blas, rocblas_fill_upper, n, A, lda, info));
         return QM_OK;
@@ -593,15 +599,16 @@ int qm_sygvd_f64(qm_ctx *ctx, int n, const double *d_A, const double *d_B, doubl
     QM_EIG_BLAS(rocblas_set_stream(h, st));
     // workspace: C (n^2), Z (n^2), d e tau + e copy (4n), VW (96 n), U^-1 and T (n^2 each), info
     const size_t nn = (size_t)n * n;
-    const size_t need = sizeof(double) * (4 * nn + 4 * (size_t)n + (size_t)kTrdPanelCols * n) + 256;
+    const size_t need = sizeof(double) * (4 * nn + 4 * (size_t)n + (size_t)kTrdPanelCols * n) + 256 + 512;
     QM_TRY(ctx->eig_work.reserve(need));
     double *C = ctx->eig_work.as<double>(), *Z = C + nn, *trd = Z + nn, *VW = trd + 4 * n;
     double *Ui = VW + (size_t)kTrdPanelCols * n, *T = Ui + nn;
     int *info = (int *)(T + nn);
+    void *psync = reinterpret_cast<char *>(T + nn) + 256;  // the cooperative Cholesky's barrier words
     hipLaunchKernelGGL(symmetrize_upper_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, C, d_A, n);
     QM_HIP(hipGetLastError());
     QM_HIP(hipMemcpyAsync(d_U, d_B, sizeof(double) * nn, hipMemcpyDeviceToDevice, st));
-    QM_TRY(potrf_upper(h, n, d_U, n, info, st));
+    QM_TRY(potrf_upper(h, n, d_U, n, info, st, psync));
     int h_info = 0;
     QM_HIP(hipMemcpyAsync(&h_info, info, sizeof(int), hipMemcpyDeviceToHost, st));
     QM_HIP(hipStreamSynchronize(st));

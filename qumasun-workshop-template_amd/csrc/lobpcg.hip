@@ -517,6 +517,7 @@ struct qm_lobpcg {
     size_t ev_used = 0;
     double *evk = nullptr;   // Ritz values of the last accepted iteration (device, N)
     double *pen = nullptr;   // diagonal of dropped directions in the subspace problem (device, 1)
+    void *psync = nullptr;   // barrier words of the cooperative Cholesky (device, 512 bytes)
     bool graphs = true;      // replay A.R + Gram of an iteration as a HIP graph
     hipStream_t gstream = nullptr;  // capture stream (graphs launch on the caller's stream)
     int graph_replays = 0;   // replays in the last solve
@@ -547,6 +548,8 @@ void lfree(qm_lobpcg *s) {
     s->evk = nullptr;
     if (s->pen) (void)hipFree(s->pen);
     s->pen = nullptr;
+    if (s->psync) (void)hipFree(s->psync);
+    s->psync = nullptr;
     if (s->gstream) (void)hipStreamDestroy(s->gstream);
     s->gstream = nullptr;
     for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
@@ -600,7 +603,7 @@ int chol_factor(qm_lobpcg *s, int n3, hipStream_t st) {
     if constexpr (CPLX)
         QM_BLAS(rocsolver_zpotrf(s->blas, rocblas_fill_upper, n3, (rocblas_double_complex *)s->Sw, n3, s->info));
     else
-        QM_TRY(potrf_upper(s->blas, n3, (double *)s->Sw, n3, s->info, st));
+        QM_TRY(potrf_upper(s->blas, n3, (double *)s->Sw, n3, s->info, st, s->psync));
     hipLaunchKernelGGL((pivot_ratio_kernel<CPLX>), dim3(1), dim3(256), 0, st, s->svals, (const SmT *)s->Sw, n3);
     QM_HIP(hipGetLastError());
     return QM_OK;
@@ -831,7 +834,7 @@ int orthonormalise_r(qm_lobpcg *s, T *Rout, const T *X, const T *R, hipStream_t 
     if constexpr (CPLX)
         QM_BLAS(rocsolver_zpotrf(s->blas, rocblas_fill_upper, N, (rocblas_double_complex *)s->Sw, N, s->info));
     else
-        QM_TRY(potrf_upper(s->blas, N, (double *)s->Sw, N, s->info, st));
+        QM_TRY(potrf_upper(s->blas, N, (double *)s->Sw, N, s->info, st, s->psync));
     hipLaunchKernelGGL((pivot_ratio_kernel<CPLX>), dim3(1), dim3(256), 0, st, s->svals, (const SmT *)s->Sw, N);
     QM_HIP(hipGetLastError());
     int h_info = 0;
@@ -1461,6 +1464,7 @@ int qm_lobpcg_create(qm_lobpcg **out, qm_ctx *ctx, int nbands, int ncomp, int pr
     if (st == QM_OK) st = dalloc((void **)&s->info, sizeof(int));
     if (st == QM_OK) st = dalloc((void **)&s->evk, sizeof(double) * N);
     if (st == QM_OK) st = dalloc((void **)&s->pen, sizeof(double));
+    if (st == QM_OK) st = dalloc(&s->psync, 512);
     if (st == QM_OK && hipStreamCreateWithFlags(&s->gstream, hipStreamNonBlocking) != hipSuccess) {
         s->gstream = nullptr;
         set_error("qm_lobpcg_create: hipStreamCreateWithFlags failed");

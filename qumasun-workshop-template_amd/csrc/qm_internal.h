@@ -184,9 +184,15 @@ int sytrd_lower(void *blas, int n, double *C, int lda, double *trd, double *VW, 
 int syev_lower(void *blas, int n, int nev, double *C, int lda, double *w, double *Z, int ldz, double *trd,
                double *VW, int *info, hipStream_t st);
 int own_eig_max_n();
-// A = U^T U, upper triangle of the column-major n x n A (FP64): one workgroup
-// for n <= own_eig_max_n(), rocSOLVER above; *info (device) as LAPACK's dpotrf.
-int potrf_upper(void *blas, int n, double *A, int lda, int *info, hipStream_t st);
+// A = U^T U, upper triangle of the column-major n x n A (FP64): with sync (512
+// bytes of device memory) and n <= 512, one workgroup per 32-row panel
+// (potrf_coop_f64); else one workgroup for n <= own_eig_max_n(), rocSOLVER
+// above; *info (device) as LAPACK's dpotrf.
+int potrf_upper(void *blas, int n, double *A, int lda, int *info, hipStream_t st, void *sync = nullptr);
+// trd_coop.hip: the cooperative Cholesky (QM_ERR_STATE when it cannot run: the
+// caller takes another kernel); *info as dpotrf, -77 if a barrier wait timed out
+int potrf_coop_supported(int n);
+int potrf_coop_f64(int n, double *A, int lda, int *info, void *sync, hipStream_t st);
 // C := U^-T C U^-1 with Ui = U^-1 kept (n x n each, leading dimension n; T: n x n work),
 // and Zout = Ui Z: the two halves of the Cholesky route around the eigensolve.
 int chol_transform(void *blas, int n, const double *U, double *C, double *Ui, double *T, hipStream_t st);

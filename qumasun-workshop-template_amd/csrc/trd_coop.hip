@@ -377,6 +377,113 @@ int trd_coop_impl(int n, T *A, int lda, double *d, double *e, T *tau, void *work
     return QM_OK;
 }
 
+// *info := the Cholesky's failing minor (sync[96]), or -77 where a barrier
+// wait ran past its limit
+__global__ void potrf_coop_info_kernel(const unsigned *sync, int *info) {
+    if (threadIdx.x == 0) *info = sync[96] != 0u ? (int)sync[96] : (sync[64] != 0u ? -77 : 0);
+}
+
+// ---------------------------------------------------------------- Cholesky
+// Upper Cholesky factor A = U^T U (LAPACK dpotrf, uplo 'U') by cooperating
+// workgroups, one per 32-row panel.  Thread c of a workgroup holds column
+// r0 + c of its panel's rows in registers (32 values).  Step k: the owner of
+// panel k factors it, one column per workgroup barrier (the row being
+// eliminated is shared through LDS, double-buffered; rows stay unscaled until
+// the panel is done, so a thread needs no other thread's updated values), and
+// writes it to A; one grid barrier; every later panel subtracts the rank-32
+// product of panel k's rows from its own (thread c: panel k's column c from
+// memory into registers, the 32 x 32 block over the panel's own columns from
+// LDS), and the owner of panel k + 1 goes on to factor it.  One grid barrier
+// per panel.  *info = 0, or the order of the first leading minor whose pivot
+// is not positive (or NaN), as dpotrf reports it; the strictly lower triangle
+// is never written.  sync (zeroed by coop_clear_kernel): as the reduction's,
+// plus sync[96] = info.
+constexpr int kPcNB = 32;
+constexpr int kPcMaxN = 512;
+constexpr int kPcThreads = kPcMaxN;  // one thread per column
+
+__global__ __launch_bounds__(kPcThreads) void potrf_coop_kernel(double *__restrict__ A, int lda, int n,
+                                                                unsigned *sync) {
+    __shared__ double rowj[2][kPcMaxN];          // the row being eliminated (by parity of j)
+    __shared__ double piv[kPcNB];                // the panel's pivots
+    __shared__ double Ux[kPcNB][kPcNB + 1];      // panel k's rows over this panel's columns
+    const int g = blockIdx.x, tid = threadIdx.x;
+    unsigned *count = sync, *gen = sync + 32;
+    int *err = reinterpret_cast<int *>(sync + 64);
+    int *info = reinterpret_cast<int *>(sync + 96);
+    unsigned lg = 0;
+    const int P = gridDim.x;
+    const int r0 = g * kPcNB, nrow = min(kPcNB, n - r0), w = n - r0;
+    const bool mine = tid < w;  // column c = r0 + tid of the panel's rows
+    const int c = r0 + tid;
+    rowj[0][tid] = 0.0;  // columns past n stay zero
+    rowj[1][tid] = 0.0;
+    double a[kPcNB];
+#pragma unroll
+    for (int i = 0; i < kPcNB; ++i) a[i] = (mine && i < nrow && tid >= i) ? A[(r0 + i) + (size_t)c * lda] : 0.0;
+    for (int k = 0; k < P; ++k) {
+        if (k == g) {
+            int bad = 0;
+            // nrow and bad are uniform over the workgroup, so every thread
+            // takes the same branches around the barriers
+#pragma unroll
+            for (int j = 0; j < kPcNB; ++j) {
+                if (j < nrow && bad == 0) {
+                    if (mine) rowj[j & 1][tid] = a[j];
+                    __syncthreads();
+                    const double d = rowj[j & 1][j];  // the same pivot for every thread
+                    if (!(d > 0.0)) {
+                        bad = r0 + j + 1;
+                    } else {
+                        if (tid == 0) piv[j] = d;
+                        const double inv = 1.0 / d;
+                        // rows below the diagonal (i > tid) or past the panel
+                        // (i >= nrow) take values that are never read or written out
+                        if (mine && tid > j) {
+                            const double f = a[j] * inv;
+#pragma unroll
+                            for (int i = j + 1; i < kPcNB; ++i) a[i] = fma(-rowj[j & 1][i], f, a[i]);
+                        }
+                    }
+                }
+            }
+            __syncthreads();  // piv complete
+            if (bad) {
+                if (tid == 0) __hip_atomic_store(info, bad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            } else if (mine) {
+#pragma unroll
+                for (int i = 0; i < kPcNB; ++i) {
+                    if (i >= nrow || i > tid) continue;
+                    const double u = sqrt(piv[i]);
+                    A[(r0 + i) + (size_t)c * lda] = i == tid ? u : a[i] / u;
+                }
+            }
+        }
+        coop_barrier(count, gen, P, lg, err);
+        if (__hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;  // all workgroups
+        if (g <= k) continue;
+        // own rows -= U(panel k rows, own columns)^T U(panel k rows, column c)
+        const int k0 = k * kPcNB;
+        for (int e = tid; e < kPcNB * kPcNB; e += kPcThreads) {
+            const int t = e % kPcNB, i = e / kPcNB;
+            Ux[t][i] = i < nrow ? A[(k0 + t) + (size_t)(r0 + i) * lda] : 0.0;
+        }
+        double uk[kPcNB];
+#pragma unroll
+        for (int t = 0; t < kPcNB; ++t) uk[t] = mine ? A[(k0 + t) + (size_t)c * lda] : 0.0;
+        __syncthreads();
+        if (mine) {  // (rows below the diagonal and past the panel: as above)
+#pragma unroll
+            for (int t = 0; t < kPcNB; ++t) {
+                const double ut = uk[t];
+#pragma unroll
+                for (int i = 0; i < kPcNB; ++i) a[i] = fma(-Ux[t][i], ut, a[i]);
+            }
+        }
+        __syncthreads();  // Ux is rewritten at the next step
+    }
+}
+
 }  // namespace
 
 size_t trd_coop_work_bytes(int n, bool cplx) { return 512 + (4 * (size_t)n + 512) * (cplx ? 16 : 8); }
@@ -396,6 +503,34 @@ int trd_coop_z64(int n, void *A, int lda, double *d, double *e, void *tau, void 
 
 int trd_coop_check(void *work, int *info, hipStream_t st) {
     hipLaunchKernelGGL(coop_err_kernel, dim3(1), dim3(64), 0, st, static_cast<const unsigned *>(work), info);
+    QM_HIP(hipGetLastError());
+    return QM_OK;
+}
+
+}  // namespace qm
+
+namespace qm {
+
+int potrf_coop_supported(int n) {
+    const int cus = coop_cus();
+    const int P = (n + kPcNB - 1) / kPcNB;
+    return n >= 1 && n <= kPcMaxN && cus > 0 && P <= cus;
+}
+
+int potrf_coop_f64(int n, double *A, int lda, int *info, void *sync, hipStream_t st) {
+    if (!potrf_coop_supported(n)) return QM_ERR_STATE;
+    const int P = (n + kPcNB - 1) / kPcNB;
+    unsigned *sy = static_cast<unsigned *>(sync);
+    hipLaunchKernelGGL(coop_clear_kernel, dim3(1), dim3(128), 0, st, sy);
+    QM_HIP(hipGetLastError());
+    void *args[] = {&A, &lda, &n, &sy};
+    const hipError_t le = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&potrf_coop_kernel), dim3(P),
+                                                     dim3(kPcThreads), args, 0, st);
+    if (le != hipSuccess) {
+        (void)hipGetLastError();
+        return QM_ERR_STATE;  // the caller takes the single-workgroup kernel
+    }
+    hipLaunchKernelGGL(potrf_coop_info_kernel, dim3(1), dim3(64), 0, st, sy, info);
     QM_HIP(hipGetLastError());
     return QM_OK;
 }
