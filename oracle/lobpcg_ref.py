@@ -332,3 +332,65 @@ class LOBPCGRef:
                 x = x / np.sqrt(norms * self.dV)[:, None]
         keep_p(p)
         return finish(x, eps, rnorm2, it_done, history)
+
+def lobpcg_reference_variant(apply_soa, X_soa, ncomp, dV, iters):
+    """The reference's own iteration, for pinning this build's variant against
+    it: Ritz values after each iteration of one first-time call.
+
+    Follows Eigen_z_multi_mpi (lobpcg_z_multi_mpi.h:1732-2852) with its
+    default build options: R is used as computed (no orthonormalisation
+    against X), P is not normalised (EVERY_NORMALIZE_P off, 71), and X^H R and
+    R^H P come from the previous step's coefficients (DIRECT_S_MATRIX_XR_RP,
+    58): with R = A X - X L and the recurrence blocks X^H X, X^H A X, X^H P,
+    X^H A P, P^H P, P^H A P, X^H R = X^H A X - X^H X L and
+    R^H P = X^H A P - L X^H P (A Hermitian).  R^H R, X^H A R, R^H A R and R^H A P
+    are formed from the vectors (MakeMatrix_z_blas, 300-558).  Rayleigh-Ritz
+    on [X R P] (or [X R] on the first iteration) by LAPACK's generalized
+    eigensolver; X' = X Cx + R Cr + P Cp, P' = R Cr + P Cp (NextVector_z2,
+    1515-1598).  The subspaces equal the build variant's (its R
+    orthonormalisation and P scaling are changes of basis), so in exact
+    arithmetic the Ritz values agree iteration by iteration."""
+    nc = ncomp
+    dt = np.complex128 if nc == 2 else np.float64
+    A = lambda z: soa_to_cplx(apply_soa(cplx_to_soa(z, nc)), nc)  # noqa: E731
+    G = lambda a, b: np.conj(a) @ b.T  # noqa: E731
+    x = soa_to_cplx(X_soa, nc).astype(dt) * np.sqrt(dV)
+    N = x.shape[0]
+    Ax = A(x)
+    lam = np.real(np.sum(np.conj(x) * Ax, axis=1))
+    p = Ap = None
+    rec = {}
+    history = []
+    for it in range(iters):
+        r = Ax - lam[:, None] * x
+        Ar = A(r)
+        if p is None:
+            XX, XAX = G(x, x), G(x, Ax)
+            XR, RR, XAR, RAR = G(x, r), G(r, r), G(x, Ar), G(r, Ar)
+            S = np.block([[XX, XR], [XR.conj().T, RR]])
+            Hm = np.block([[XAX, XAR], [XAR.conj().T, RAR]])
+        else:
+            XX, XAX, XP, XAP, PP, PAP = (rec[k] for k in ("XX", "XAX", "XP", "XAP", "PP", "PAP"))
+            XR = XAX - XX * lam[None, :]                     # X^H (A X - X L)
+            RP = XAP - lam[:, None] * XP  # (A X - X L)^H P, A Hermitian
+            RR, XAR, RAR, RAP = G(r, r), G(x, Ar), G(r, Ar), G(r, Ap)
+            S = np.block([[XX, XR, XP], [XR.conj().T, RR, RP], [XP.conj().T, RP.conj().T, PP]])
+            Hm = np.block([[XAX, XAR, XAP], [XAR.conj().T, RAR, RAP], [XAP.conj().T, RAP.conj().T, PAP]])
+        S = 0.5 * (S + S.conj().T)
+        Hm = 0.5 * (Hm + Hm.conj().T)
+        w, C = scipy.linalg.eigh(Hm, S, driver="gvd")
+        lam, C = w[:N], C[:, :N]
+        Cx, Cr = C[:N], C[N:2 * N]
+        Cp = C[2 * N:] if p is not None else None
+        pn = Cr.T @ r + (Cp.T @ p if p is not None else 0)
+        Apn = Cr.T @ Ar + (Cp.T @ Ap if p is not None else 0)
+        xn, Axn = Cx.T @ x + pn, Cx.T @ Ax + Apn
+        # recurrence blocks of the next step (NextMatrix_z_blasmp, 1104-1436)
+        Cp0 = np.concatenate([np.zeros((N, N), dtype=dt), C[N:]], axis=0)
+        Cb = np.concatenate([C, Cp0], axis=1)
+        SS, HH = np.conj(Cb).T @ (S @ Cb), np.conj(Cb).T @ (Hm @ Cb)
+        rec = {"XX": SS[:N, :N], "XP": SS[:N, N:], "PP": SS[N:, N:],
+               "XAX": HH[:N, :N], "XAP": HH[:N, N:], "PAP": HH[N:, N:]}
+        x, Ax, p, Ap = xn, Axn, pn, Apn
+        history.append(lam.copy())
+    return history
