@@ -119,18 +119,23 @@ __global__ void extract_blocks_kernel(typename Sm<CPLX>::t *blk, const double *G
 // huge value (1e30) would be exact in exact arithmetic, but a dense eigensolver's
 // backward error (eps |M|, 1e14 for 1e30) then lands on every eigenvalue.
 template <bool CPLX>
-__global__ __launch_bounds__(256) void penalty_kernel(double *pen, const typename Sm<CPLX>::t *hxx, int N) {
+__global__ __launch_bounds__(64) void row_abs_sum_kernel(double *rs, const typename Sm<CPLX>::t *hxx, int N) {
+    const int i = blockIdx.x;
+    double r = 0.0;
+    for (int j = threadIdx.x; j < N; j += 64) {
+        const auto v = hxx[i + (int64_t)j * N];
+        if constexpr (CPLX) r += sqrt(v.re * v.re + v.im * v.im);
+        else r += fabs(v);
+    }
+    for (int off = 32; off > 0; off >>= 1) r += __shfl_xor(r, off, 64);
+    if (threadIdx.x == 0) rs[i] = r;
+}
+
+// pen[0] = 2 max(rs[0 .. n-1]) + 1 (one block)
+__global__ __launch_bounds__(256) void penalty_kernel(double *pen, const double *rs, int n) {
     __shared__ double red[256];
     double m = 0.0;
-    for (int i = threadIdx.x; i < N; i += 256) {
-        double r = 0.0;
-        for (int j = 0; j < N; ++j) {
-            const auto v = hxx[i + (int64_t)j * N];
-            if constexpr (CPLX) r += sqrt(v.re * v.re + v.im * v.im);
-            else r += fabs(v);
-        }
-        m = fmax(m, r);
-    }
+    for (int i = threadIdx.x; i < n; i += 256) m = fmax(m, rs[i]);
     red[threadIdx.x] = m;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
@@ -397,28 +402,18 @@ __global__ void pivot_ratio_kernel(double *out, const typename Sm<CPLX>::t *U, i
     if (threadIdx.x == 0) out[0] = mx[0] > 0.0 ? mn[0] / mx[0] : 0.0;
 }
 
-// dev[i] = |1 - c_i^H S c_i| for the N lowest Ritz vectors (block i; columns
-// of C, leading dimension n3; S full, column-major); non-finite -> 1e300
+// dev[i] = |1 - c_i^H t_i| for the N lowest Ritz vectors c_i (block i) and
+// t_i = S c_i (columns of C and T, leading dimension n3); non-finite -> 1e300
 template <bool CPLX>
-__global__ __launch_bounds__(256) void ritz_orth_kernel(double *dev, const typename Sm<CPLX>::t *S,
-                                                        const typename Sm<CPLX>::t *C, int n3) {
+__global__ __launch_bounds__(256) void ritz_orth_kernel(double *dev, const typename Sm<CPLX>::t *C,
+                                                        const typename Sm<CPLX>::t *T, int n3) {
     __shared__ double red[256];
     const auto *c = C + (int64_t)blockIdx.x * n3;
+    const auto *t = T + (int64_t)blockIdx.x * n3;
     double acc = 0.0;
     for (int j = threadIdx.x; j < n3; j += 256) {
-        if constexpr (CPLX) {
-            double sr = 0.0, si = 0.0;
-            for (int k = 0; k < n3; ++k) {
-                const Cplx a = S[j + (int64_t)k * n3], b = c[k];
-                sr += a.re * b.re - a.im * b.im;
-                si += a.re * b.im + a.im * b.re;
-            }
-            acc += c[j].re * sr + c[j].im * si;  // Re(conj(c_j) (S c)_j)
-        } else {
-            double sc = 0.0;
-            for (int k = 0; k < n3; ++k) sc += S[j + (int64_t)k * n3] * c[k];
-            acc += c[j] * sc;
-        }
+        if constexpr (CPLX) acc += c[j].re * t[j].re + c[j].im * t[j].im;  // Re(conj(c_j) t_j)
+        else acc += c[j] * t[j];
     }
     red[threadIdx.x] = acc;
     __syncthreads();
@@ -517,6 +512,7 @@ struct qm_lobpcg {
     size_t ev_used = 0;
     double *evk = nullptr;   // Ritz values of the last accepted iteration (device, N)
     double *pen = nullptr;   // diagonal of dropped directions in the subspace problem (device, 1)
+    void *rt = nullptr;      // S times the N lowest Ritz vectors (device, 3N x N subspace elements)
     void *psync = nullptr;   // barrier words of the cooperative Cholesky (device, 512 bytes)
     bool graphs = true;      // replay A.R + Gram of an iteration as a HIP graph
     hipStream_t gstream = nullptr;  // capture stream (graphs launch on the caller's stream)
@@ -548,6 +544,8 @@ void lfree(qm_lobpcg *s) {
     s->evk = nullptr;
     if (s->pen) (void)hipFree(s->pen);
     s->pen = nullptr;
+    if (s->rt) (void)hipFree(s->rt);
+    s->rt = nullptr;
     if (s->psync) (void)hipFree(s->psync);
     s->psync = nullptr;
     if (s->gstream) (void)hipStreamDestroy(s->gstream);
@@ -1213,8 +1211,10 @@ int solve_impl(qm_lobpcg *s, T *x_user, double *eigenvalues, double *residual_no
         int nbk_used = nbk;
         const void *evec = s->Hw;
         const double cond_limit = sizeof(T) == 8 ? s->cond64 : s->cond32;
-        hipLaunchKernelGGL((penalty_kernel<CPLX>), dim3(1), dim3(256), 0, st, s->pen,
+        hipLaunchKernelGGL((row_abs_sum_kernel<CPLX>), dim3(N), dim3(64), 0, st, s->svw,
                            (const SmT *)s->blk + (int64_t)B_XAX * N * N, N);
+        QM_HIP(hipGetLastError());
+        hipLaunchKernelGGL(penalty_kernel, dim3(1), dim3(256), 0, st, s->pen, (const double *)s->svw, N);
         QM_HIP(hipGetLastError());
         for (int attempt = 0;; ++attempt) {
             const int n3a = nbk_used * N;
@@ -1237,8 +1237,20 @@ int solve_impl(qm_lobpcg *s, T *x_user, double *eigenvalues, double *residual_no
                 // and its Ritz values can be spurious), are handled like a
                 // failed Cholesky: without P, then the S eigenbasis
                 const SmT *cv = (const SmT *)(s->evec_t1 ? s->T1 : s->Hw);
-                hipLaunchKernelGGL((ritz_orth_kernel<CPLX>), dim3(N), dim3(256), 0, st, s->svw, (const SmT *)s->S, cv,
-                                   n3a);
+                if constexpr (CPLX) {
+                    const rocblas_double_complex one{1.0, 0.0}, zero{0.0, 0.0};
+                    QM_BLAS(rocblas_zgemm(s->blas, rocblas_operation_none, rocblas_operation_none, n3a, N, n3a, &one,
+                                          (const rocblas_double_complex *)s->S, n3a,
+                                          (const rocblas_double_complex *)cv, n3a, &zero,
+                                          (rocblas_double_complex *)s->rt, n3a));
+                } else {
+                    const double one = 1.0, zero = 0.0;
+                    QM_BLAS(rocblas_dgemm(s->blas, rocblas_operation_none, rocblas_operation_none, n3a, N, n3a, &one,
+                                          (const double *)s->S, n3a, (const double *)cv, n3a, &zero,
+                                          (double *)s->rt, n3a));
+                }
+                hipLaunchKernelGGL((ritz_orth_kernel<CPLX>), dim3(N), dim3(256), 0, st, s->svw, cv,
+                                   (const SmT *)s->rt, n3a);
                 QM_HIP(hipGetLastError());
                 hipLaunchKernelGGL(max_reduce_kernel, dim3(1), dim3(256), 0, st, s->svw, N);
                 QM_HIP(hipGetLastError());
@@ -1464,6 +1476,7 @@ int qm_lobpcg_create(qm_lobpcg **out, qm_ctx *ctx, int nbands, int ncomp, int pr
     if (st == QM_OK) st = dalloc((void **)&s->info, sizeof(int));
     if (st == QM_OK) st = dalloc((void **)&s->evk, sizeof(double) * N);
     if (st == QM_OK) st = dalloc((void **)&s->pen, sizeof(double));
+    if (st == QM_OK) st = dalloc(&s->rt, esz * (size_t)n3 * N);
     if (st == QM_OK) st = dalloc(&s->psync, 512);
     if (st == QM_OK && hipStreamCreateWithFlags(&s->gstream, hipStreamNonBlocking) != hipSuccess) {
         s->gstream = nullptr;
