@@ -481,8 +481,7 @@ int syev_lower(void *blas, int n, int nev, double *C, int lda, double *w, double
                 QM_EIG_BLAS(rocsolver_dsteqr(h, rocblas_evect_tridiagonal, n, w, e_copy, Z, ldz, info));
             }
             QM_TRY(trd_coop_check(VW, info, st));
-            QM_EIG_BLAS(rocsolver_dormtr(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, n, nev, C,
-                                         lda, trd + 2 * n, Z, ldz));
+            QM_TRY(ormtr_lower_f64(n, nev, C, lda, trd + 2 * n, Z, ldz, VW, st));
             return QM_OK;
         }
         if (r != QM_ERR_STATE) return r;
@@ -683,8 +682,7 @@ int qm_hegvd_z64(qm_ctx *ctx, int n, const void *d_A, const void *d_B, double *d
             QM_EIG_BLAS(rocsolver_zsteqr(h, rocblas_evect_tridiagonal, n, d_w, e_copy, Z, n, info));
         }
         QM_TRY(trd_coop_check(coop, info, st));
-        QM_EIG_BLAS(rocsolver_zunmtr(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, n, nev, C, n,
-                                     tau, Z, n));
+        QM_TRY(unmtr_lower_z64(n, nev, C, n, tau, Z, n, coop, st));
     } else {
         if (r != QM_ERR_STATE) return r;
         QM_EIG_BLAS(rocsolver_zheevd(h, rocblas_evect_original, rocblas_fill_upper, n, C, n, d_w, e, info));

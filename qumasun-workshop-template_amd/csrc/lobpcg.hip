@@ -669,8 +669,7 @@ int chol_solve(qm_lobpcg *s, int n3, hipStream_t st) {
                                              s->info));
                 }
                 QM_TRY(trd_coop_check(s->VW, s->info, st));
-                QM_BLAS(rocsolver_zunmtr(s->blas, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, n3,
-                                         N, C, n3, tau, Z, n3));
+                QM_TRY(unmtr_lower_z64(n3, N, C, n3, tau, Z, n3, s->VW, st));
                 QM_BLAS(rocblas_ztrsm(s->blas, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
                                       rocblas_diagonal_non_unit, n3, N, &one, U, n3, Z, n3));
                 s->evec_t1 = true;

@@ -213,6 +213,14 @@ int trd_coop_f64(int n, double *A, int lda, double *d, double *e, double *tau, v
 int trd_coop_z64(int n, void *A, int lda, double *d, double *e, void *tau, void *work, size_t work_bytes,
                  hipStream_t st);
 int trd_coop_check(void *work, int *info, hipStream_t st);
+// Z := Q Z (first nev columns) for the Q of a LAPACK lower tridiagonal
+// reduction (reflectors below A's subdiagonal, tau): dormtr / zunmtr with
+// side L, uplo L, no transpose; n <= 2048.  work: the cooperative reduction's
+// workspace (trd_coop_work_bytes), after trd_coop_check has read it
+int ormtr_lower_f64(int n, int nev, const double *A, int lda, const double *tau, double *Z, int ldz, void *work,
+                    hipStream_t st);
+int unmtr_lower_z64(int n, int nev, const void *A, int lda, const void *tau, void *Z, int ldz, void *work,
+                    hipStream_t st);
 // route of the subspace eigensolve's tridiagonal reduction (qm_subspace_force):
 // -1 automatic (cooperative where it fits), 0 never cooperative, 1 cooperative
 // wherever it fits

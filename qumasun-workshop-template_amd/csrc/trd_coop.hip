@@ -484,9 +484,255 @@ __global__ __launch_bounds__(kPcThreads) void potrf_coop_kernel(double *__restri
     }
 }
 
+// ------------------------------------------------- back-transformation (Q Z)
+// Z := Q Z for the first nev columns of Z, Q = H(0) H(1) ... H(n-2) from the
+// reflectors a LAPACK-convention lower tridiagonal reduction leaves (H(j) =
+// I - tau_j v_j v_j^H, v_j = (0 .. 0, 1 at j + 1, A(j+2 .. n-1, j)); dormtr /
+// zunmtr with side L, uplo L, no transpose).  The reflectors are applied last
+// to first in groups of QB: group g holds r_k = n - 2 - (g QB + k), applied
+// in the order k = 0 .. QB-1.  With d_k = v_{r_k}^H z (z on entry to the
+// group) and M_km = v_{r_k}^H v_{r_m}, the k-th reflector sees
+// s_k = d_k - sum_{m<k} M_km w_m, w_k = tau_{r_k} s_k, and the group leaves
+// z - sum_k w_k v_{r_k}.  The QB dot products of a group share one round of
+// reductions, which cuts the serial chain of loads and reductions QB-fold;
+// M depends on the reflectors only and is formed once for all columns by
+// reflector_gram_kernel.  A wave per column left each SIMD a single
+// dependent instruction stream (nev waves on the chip): a workgroup per
+// column spreads the column over four SIMDs and nev CUs.
+
+template <typename T>
+__device__ inline T refl_elem(const T *aj, int i, int j, int n) {
+    return i == j + 1 ? make<T>(1.0, 0.0) : (i >= j + 2 && i < n ? aj[i] : make<T>(0.0, 0.0));
+}
+
+// M[g][k][m] = v_{r_k}^H v_{r_m}, m < k (r_k < r_m: the product's support
+// starts at r_m + 1, where v_{r_m} is 1 and v_{r_k} an element of A); zero
+// when r_k < 0 (the last group is short).  One workgroup per group, one wave
+// per pair.
+template <typename T, int QB>
+__global__ __launch_bounds__(256) void reflector_gram_kernel(const T *__restrict__ A, int lda, int n,
+                                                             T *__restrict__ M) {
+    const int g = blockIdx.x, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int p = wave; p < QB * (QB - 1) / 2; p += 4) {
+        int k = 1;
+        while (k * (k + 1) / 2 <= p) ++k;
+        const int m = p - k * (k - 1) / 2;
+        const int rk = n - 2 - (g * QB + k), rm = n - 2 - (g * QB + m);
+        T sum = make<T>(0.0, 0.0);
+        if (rk >= 0) {
+            const T *ak = A + (size_t)rk * lda, *am = A + (size_t)rm * lda;
+            for (int i = rm + 1 + lane; i < n; i += 64) sum = add(sum, mul(conj_(ak[i]), refl_elem(am, i, rm, n)));
+        }
+#pragma unroll
+        for (int s = 32; s >= 1; s >>= 1) sum = add(sum, shfl_xor(sum, s));
+        if (lane == 0) M[((size_t)g * QB + k) * QB + m] = sum;
+    }
+}
+
+// Raw element for row i of reflector r: an unconditional load from a row
+// clamped into r + 1 .. n - 1 of column max(r, 0), always inside the matrix.
+// The value is masked (refl_mask) only where it is used, a group later, so
+// that the load stays in flight meanwhile: a select next to the load would
+// make the compiler wait for it there.
+template <typename T>
+__device__ inline T refl_raw(const T *__restrict__ A, int lda, int i, int r, int n) {
+    const int rc = r < 0 ? 0 : r;
+    const int ic = i < rc + 1 ? rc + 1 : (i > n - 1 ? n - 1 : i);
+    return A[(size_t)rc * lda + ic];
+}
+
+template <typename T>
+__device__ inline T refl_mask(T raw, int i, int r, int n) {
+    return r < 0 || i < r + 1 || i > n - 1 ? make<T>(0.0, 0.0) : (i == r + 1 ? make<T>(1.0, 0.0) : raw);
+}
+
+template <typename T, int PER, int QB>
+__device__ inline void load_group(const T *__restrict__ A, int lda, int n, int g, int tid, T (&v)[QB][PER]) {
+#pragma unroll
+    for (int k = 0; k < QB; ++k) {
+#pragma unroll
+        for (int q = 0; q < PER; ++q) v[k][q] = refl_raw(A, lda, tid + 256 * q, n - 2 - (g * QB + k), n);
+    }
+}
+
+// The group's coefficient for thread tid, raw: an M entry (tid < QB^2), else
+// the tau of reflector k = tid - QB^2 (address clamped into tau); coef_mask
+// zeroes the tau past the last reflector.
+template <typename T, int QB>
+__device__ inline T coef_raw(const T *__restrict__ M, const T *__restrict__ tau, int n, int g, int tid) {
+    const int r = n - 2 - (g * QB + tid - QB * QB);
+    const T *p = tid < QB * QB ? M + (size_t)g * QB * QB + tid : tau + (r < 0 || r > n - 2 ? 0 : r);
+    return *p;
+}
+
+template <typename T, int QB>
+__device__ inline T coef_mask(T raw, int n, int g, int tid) {
+    return tid >= QB * QB && n - 2 - (g * QB + tid - QB * QB) < 0 ? make<T>(0.0, 0.0) : raw;
+}
+
+// Wave sums of QB values at once by recursive halving: at the exchange with
+// lane ^ s (s = 32, 16, ..) each lane keeps the half of its list selected by
+// its lane bit s and adds the partner's copy of that half, so the list halves
+// with every exchange; the last value is then summed over the lanes that
+// still share it: QB - 1 + 6 - log2(QB) exchanges in all (10 for QB = 8)
+// where separate reductions take 6 QB.  Lane l ends with the sum of value
+// multi_index<QB>(l).
+template <int QB>
+__device__ inline int multi_index(int lane) {
+    int k = 0;
+#pragma unroll
+    for (int m = QB, s = 32; m > 1; m >>= 1, s >>= 1) k += (lane & s) ? m / 2 : 0;
+    return k;
+}
+
+// M values left, next exchange distance S (compile-time, so that d stays in
+// registers: a run-time index into d would turn into select chains)
+template <typename T, int QB, int M, int S>
+__device__ inline void multi_halve(T (&d)[QB], int lane) {
+    if constexpr (M > 1) {
+        const bool up = (lane & S) != 0;
+#pragma unroll
+        for (int j = 0; j < M / 2; ++j) {
+            const T send = up ? d[j] : d[j + M / 2];
+            const T keep = up ? d[j + M / 2] : d[j];
+            d[j] = add(keep, shfl_xor(send, S));
+        }
+        multi_halve<T, QB, M / 2, S / 2>(d, lane);
+    } else {
+#pragma unroll
+        for (int s = S; s >= 1; s >>= 1) d[0] = add(d[0], shfl_xor(d[0], s));
+    }
+}
+
+template <typename T, int QB>
+__device__ inline T wave_multi_sum(T (&d)[QB], int lane) {
+    multi_halve<T, QB, QB, 32>(d, lane);
+    return d[0];
+}
+
+// Workgroup barrier that orders LDS only: global loads still in flight (the
+// next group's prefetch) are not waited for, as the full __syncthreads fence
+// would.
+__device__ inline void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// One workgroup per column of Z, row i of the column in thread i mod 256
+// (PER rows a thread).  Per group of QB reflectors: QB partial dot products a
+// thread, a wave reduction of each, the four waves' partials through LDS (one
+// barrier), the triangular recurrence for w (every thread, uniform), the
+// update.  The next group's reflector elements, M and tau are loaded while
+// the current group is applied; M and tau reach every thread through LDS.
+template <typename T, int PER, int QB>
+__global__ __launch_bounds__(256) void apply_q_kernel(const T *__restrict__ A, int lda, const T *__restrict__ tau,
+                                                      const T *__restrict__ M, T *__restrict__ Z, int ldz, int n) {
+    static_assert(QB * QB + QB <= 256, "group coefficients: one a thread");
+    // by group parity, so that one barrier a group suffices
+    __shared__ T part[2][QB][4];
+    __shared__ T coef[2][QB * QB + QB];  // M (QB x QB), then tau of the group
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    T *z = Z + (size_t)blockIdx.x * ldz;
+    T zr[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int i = tid + 256 * q;
+        zr[q] = i < n ? z[i] : make<T>(0.0, 0.0);
+    }
+    const int groups = (n - 1 + QB - 1) / QB;
+    T vn[QB][PER];
+    load_group<T, PER, QB>(A, lda, n, 0, tid, vn);
+    T cn = coef_raw<T, QB>(M, tau, n, 0, tid);
+    for (int g = 0; g < groups; ++g) {
+        T vr[QB][PER], d[QB];
+#pragma unroll
+        for (int k = 0; k < QB; ++k) {
+#pragma unroll
+            for (int q = 0; q < PER; ++q) vr[k][q] = refl_mask(vn[k][q], tid + 256 * q, n - 2 - (g * QB + k), n);
+        }
+        const T cc = coef_mask<T, QB>(cn, n, g, tid);
+        // next group (the last group reloads itself: no branch around the loads)
+        const int gn = g + 1 < groups ? g + 1 : g;
+        load_group<T, PER, QB>(A, lda, n, gn, tid, vn);
+        cn = coef_raw<T, QB>(M, tau, n, gn, tid);
+#pragma unroll
+        for (int k = 0; k < QB; ++k) {
+            d[k] = make<T>(0.0, 0.0);
+#pragma unroll
+            for (int q = 0; q < PER; ++q) d[k] = add(d[k], mul(conj_(vr[k][q]), zr[q]));
+        }
+        const T dk = wave_multi_sum<T, QB>(d, lane);
+        const int par = g & 1;
+        if ((lane & (64 / QB - 1)) == 0) part[par][multi_index<QB>(lane)][wave] = dk;
+        if (tid < QB * QB + QB) coef[par][tid] = cc;
+        lds_barrier();
+        T w[QB];
+#pragma unroll
+        for (int k = 0; k < QB; ++k) {
+            T t = add(add(part[par][k][0], part[par][k][1]), add(part[par][k][2], part[par][k][3]));
+#pragma unroll
+            for (int m = 0; m < k; ++m) t = sub(t, mul(coef[par][k * QB + m], w[m]));
+            w[k] = mul(coef[par][QB * QB + k], t);
+        }
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+#pragma unroll
+            for (int k = 0; k < QB; ++k) zr[q] = sub(zr[q], mul(vr[k][q], w[k]));
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int i = tid + 256 * q;
+        if (i < n) z[i] = zr[q];
+    }
+}
+
+// reflectors per group: the two register copies of a group's reflector
+// elements (PER x QB each) stay within 32 values a thread
+template <typename T>
+constexpr int apply_q_group(int per) {
+    return 16 / per < 8 ? 16 / per : 8;
+}
+
+size_t trd_core_bytes(int n, size_t esz) { return 512 + (4 * (size_t)n + 512) * esz; }
+
+size_t apply_q_m_offset(int n, size_t esz) { return (trd_core_bytes(n, esz) + 255) / 256 * 256; }
+
+template <typename T, int PER>
+int apply_q_launch(int n, int nev, const T *A, int lda, const T *tau, T *Z, int ldz, T *M, hipStream_t st) {
+    constexpr int QB = apply_q_group<T>(PER);
+    const int groups = (n - 1 + QB - 1) / QB;
+    if (QB > 1) {
+        hipLaunchKernelGGL((reflector_gram_kernel<T, QB>), dim3((unsigned)groups), dim3(256), 0, st, A, lda, n, M);
+        QM_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL((apply_q_kernel<T, PER, QB>), dim3((unsigned)nev), dim3(256), 0, st, A, lda, tau,
+                       static_cast<const T *>(M), Z, ldz, n);
+    QM_HIP(hipGetLastError());
+    return QM_OK;
+}
+
+template <typename T>
+int apply_q_impl(int n, int nev, const T *A, int lda, const T *tau, T *Z, int ldz, void *work, hipStream_t st) {
+    if (n < 2 || nev < 1) return QM_OK;  // Q = I
+    QM_CHECK(n <= 64 * 32, QM_ERR_ARG, "back-transformation: order above 2048");
+    T *M = reinterpret_cast<T *>(static_cast<char *>(work) + apply_q_m_offset(n, sizeof(T)));
+    const int per = (n + 255) / 256;
+    if (per <= 1) return apply_q_launch<T, 1>(n, nev, A, lda, tau, Z, ldz, M, st);
+    if (per <= 2) return apply_q_launch<T, 2>(n, nev, A, lda, tau, Z, ldz, M, st);
+    if (per <= 4) return apply_q_launch<T, 4>(n, nev, A, lda, tau, Z, ldz, M, st);
+    return apply_q_launch<T, 8>(n, nev, A, lda, tau, Z, ldz, M, st);
+}
+
 }  // namespace
 
-size_t trd_coop_work_bytes(int n, bool cplx) { return 512 + (4 * (size_t)n + 512) * (cplx ? 16 : 8); }
+size_t trd_coop_work_bytes(int n, bool cplx) {
+    // the reduction's own words, then the reflector group products of the
+    // back-transformation (at most 8 reflectors a group)
+    const size_t esz = cplx ? 16 : 8;
+    return apply_q_m_offset(n, esz) + ((size_t)n / 8 + 1) * 64 * esz;
+}
 
 int trd_coop_supported(int n, bool cplx) { return cplx ? coop_groups<zdbl>(n) : coop_groups<double>(n); }
 
@@ -499,6 +745,17 @@ int trd_coop_z64(int n, void *A, int lda, double *d, double *e, void *tau, void 
                  hipStream_t st) {
     return trd_coop_impl<zdbl>(n, static_cast<zdbl *>(A), lda, d, e, static_cast<zdbl *>(tau), work, work_bytes,
                                nullptr, st);
+}
+
+int ormtr_lower_f64(int n, int nev, const double *A, int lda, const double *tau, double *Z, int ldz, void *work,
+                    hipStream_t st) {
+    return apply_q_impl<double>(n, nev, A, lda, tau, Z, ldz, work, st);
+}
+
+int unmtr_lower_z64(int n, int nev, const void *A, int lda, const void *tau, void *Z, int ldz, void *work,
+                    hipStream_t st) {
+    return apply_q_impl<zdbl>(n, nev, static_cast<const zdbl *>(A), lda, static_cast<const zdbl *>(tau),
+                              static_cast<zdbl *>(Z), ldz, work, st);
 }
 
 int trd_coop_check(void *work, int *info, hipStream_t st) {

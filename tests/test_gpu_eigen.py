@@ -49,7 +49,7 @@ def route(qm, request):
 # fits (every n here); route 0: the single-workgroup reduction (n <= 512) or
 # rocSOLVER syevd
 @pytest.mark.parametrize("route", [-1, 0], indirect=True, ids=["coop", "single"])
-@pytest.mark.parametrize("n", [1, 2, 5, 72, 88, 89, 121, 128, 129, 144, 200, 384, 512, 521, 768, 1024])
+@pytest.mark.parametrize("n", [1, 2, 5, 72, 88, 89, 121, 128, 129, 144, 200, 384, 512, 521, 768, 1024, 1280])
 @pytest.mark.parametrize("clusters", [False, True])
 def test_sygvd_against_lapack(H, torch_cuda, route, n, clusters):
     torch = torch_cuda
@@ -113,7 +113,7 @@ def _zproblem(n, seed, clusters=False):
 
 # route -1: the cooperative reduction (complex, every n here); 0: rocSOLVER heevd
 @pytest.mark.parametrize("route", [-1, 0], indirect=True, ids=["coop", "heevd"])
-@pytest.mark.parametrize("n", [1, 2, 5, 24, 72, 129, 192, 256, 384, 768])
+@pytest.mark.parametrize("n", [1, 2, 5, 24, 72, 129, 192, 256, 384, 768, 1040])
 @pytest.mark.parametrize("clusters", [False, True])
 def test_hegvd_against_lapack(H, torch_cuda, route, n, clusters):
     torch = torch_cuda
