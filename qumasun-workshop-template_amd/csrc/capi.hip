@@ -121,10 +121,6 @@ static void free_kpoints(qm_ctx *ctx) {
     ctx->kpts.clear();
 }
 
-static inline int wrap_host(int i, int n) {
-    int r = i % n;
-    return r < 0 ? r + n : r;
-}
 
 // e^{ik.r} on every voxel for one k-point (mCreateBlochExp, ..._kpoint.h:3537-3618):
 // phase = k_x h_x ix + k_y h_y iy + k_z h_z iz on the unwrapped coordinates.
@@ -188,43 +184,20 @@ static int build_projectors(const qm_ctx *ctx, Projectors &p, int natoms, const 
     p.energy.assign(energy, energy + p.ptot);
     p.has_grad = dtable != nullptr;
 
-    // wrapped grid index and owner atom of every voxel
-    std::vector<int> grid(p.vtot), atom(p.vtot);
-    for (int a = 0; a < natoms; ++a)
-        for (int64_t v = vox_off[a]; v < vox_off[a + 1]; ++v) {
-            grid[v] = wrap_host(vox_xyz[3 * v], ctx->nx) +
-                      ctx->nx * (wrap_host(vox_xyz[3 * v + 1], ctx->ny) +
-                                 ctx->ny * wrap_host(vox_xyz[3 * v + 2], ctx->nz));
-            atom[v] = a;
-        }
-    // covered-point map for the owner-computes scatter: voxels sorted by grid point
-    std::vector<int> order(p.vtot);
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return grid[x] < grid[y]; });
-    std::vector<int> cov_grid, cov_off;
-    cov_off.reserve(p.vtot + 1);
-    for (int64_t i = 0; i < p.vtot; ++i) {
-        if (i == 0 || grid[order[i]] != grid[order[i - 1]]) {
-            cov_grid.push_back(grid[order[i]]);
-            cov_off.push_back((int)i);
-        }
-    }
-    cov_off.push_back((int)p.vtot);
-    p.ncov = (int)cov_grid.size();
-    // heaviest atoms first in the projection launch
-    std::vector<int> aord(natoms);
-    std::iota(aord.begin(), aord.end(), 0);
-    std::stable_sort(aord.begin(), aord.end(), [&](int x, int y) {
-        return (vox_off[x + 1] - vox_off[x]) * nproj[x] > (vox_off[y + 1] - vox_off[y]) * nproj[y];
-    });
+    ProjectorMaps m;
+    build_projector_maps(ctx->nx, ctx->ny, ctx->nz, natoms, nproj, vox_off, vox_xyz, table, tab_off, proj_off,
+                         p.mpad, m);
+    p.ncov = (int)m.cov_grid.size();
+    p.nchunk = (int)m.chunk_atom.size();
+    p.nsblock = (int)m.block_atom_off.size() - 1;
 
     std::vector<float> t32(table, table + p.ttot);
     QM_TRY(upload(&p.d_nproj, nproj, natoms));
     QM_TRY(upload(&p.d_proj_off, proj_off, natoms + 1));
     QM_TRY(upload(&p.d_vox_off, vox_off, natoms + 1));
     QM_TRY(upload(&p.d_tab_off, tab_off, natoms + 1));
-    QM_TRY(upload(&p.d_vox_grid, grid.data(), p.vtot));
-    QM_TRY(upload(&p.d_vox_atom, atom.data(), p.vtot));
+    QM_TRY(upload(&p.d_vox_grid, m.grid.data(), p.vtot));
+    QM_TRY(upload(&p.d_vox_atom, m.atom.data(), p.vtot));
     QM_TRY(upload(&p.d_energy, energy, p.ptot));
     QM_TRY(upload(&p.d_table64, table, p.ttot));
     QM_TRY(upload(&p.d_table32, t32.data(), p.ttot));
@@ -233,83 +206,20 @@ static int build_projectors(const qm_ctx *ctx, Projectors &p, int natoms, const 
         QM_TRY(upload(&p.d_dtable64, dtable, 3 * p.ttot));
         QM_TRY(upload(&p.d_dtable32, d32.data(), 3 * p.ttot));
     }
-    QM_TRY(upload(&p.d_cov_grid, cov_grid.data(), cov_grid.size()));
-    QM_TRY(upload(&p.d_cov_off, cov_off.data(), cov_off.size()));
-    QM_TRY(upload(&p.d_cov_vox, order.data(), order.size()));
-    QM_TRY(upload(&p.d_atom_order, aord.data(), aord.size()));
-    // voxel-major tables padded to mpad projectors, and the projection work list
-    {
-        const int mp = p.mpad;
-        std::vector<double> tp((size_t)p.vtot * mp, 0.0);
-        std::vector<float> tpf((size_t)p.vtot * mp, 0.0f);
-        std::vector<int> catom, cv0, acoff(natoms + 1, 0), patom(p.ptot);
-        for (int a = 0; a < natoms; ++a) {
-            const int64_t V = vox_off[a + 1] - vox_off[a];
-            for (int w = 0; w < nproj[a]; ++w) {
-                patom[proj_off[a] + w] = a;
-                for (int64_t v = 0; v < V; ++v) {
-                    const double val = table[tab_off[a] + w * V + v];
-                    tp[(size_t)(vox_off[a] + v) * mp + w] = val;
-                    tpf[(size_t)(vox_off[a] + v) * mp + w] = (float)val;
-                }
-            }
-            for (int64_t v0 = 0; v0 < V; v0 += kProjChunk) {
-                catom.push_back(a);
-                cv0.push_back((int)v0);
-            }
-            acoff[a + 1] = (int)catom.size();
-        }
-        p.nchunk = (int)catom.size();
-        // launch order of the work items: by the (wrapped) z, then y, of each
-        // chunk's first voxel, so that the chunks of neighbouring atoms that
-        // read the same grid lines run together on one XCD and share its L2
-        std::vector<int> iord(p.nchunk);
-        {
-            std::vector<int64_t> key(p.nchunk);
-            for (int it = 0; it < p.nchunk; ++it) {
-                const int64_t v = vox_off[catom[it]] + cv0[it];
-                key[it] = (int64_t)wrap_host(vox_xyz[3 * v + 2], ctx->nz) * ctx->ny +
-                          wrap_host(vox_xyz[3 * v + 1], ctx->ny);
-            }
-            std::iota(iord.begin(), iord.end(), 0);
-            std::stable_sort(iord.begin(), iord.end(), [&](int a, int b) { return key[a] < key[b]; });
-        }
-        QM_TRY(upload(&p.d_item_order, iord.data(), iord.size()));
-        QM_TRY(upload(&p.d_tabp_64, tp.data(), tp.size()));
-        QM_TRY(upload(&p.d_tabp_32, tpf.data(), tpf.size()));
-        QM_TRY(upload(&p.d_chunk_atom, catom.data(), catom.size()));
-        QM_TRY(upload(&p.d_chunk_v0, cv0.data(), cv0.size()));
-        QM_TRY(upload(&p.d_atom_chunk_off, acoff.data(), acoff.size()));
-        QM_TRY(upload(&p.d_proj_atom, patom.data(), patom.size()));
-    }
-    // per scatter workgroup: distinct atoms of its entries, and every entry's slot
-    {
-        const int nb = (p.ncov + kScatterBlock - 1) / kScatterBlock;
-        std::vector<int> boff(nb + 1, 0), batoms, slot(p.vtot, -1);
-        for (int b = 0; b < nb; ++b) {
-            std::vector<int> list;
-            const int i0 = b * kScatterBlock, i1 = std::min(p.ncov, i0 + kScatterBlock);
-            for (int i = i0; i < i1; ++i)
-                for (int e = cov_off[i]; e < cov_off[i + 1]; ++e) {
-                    const int a = atom[order[e]];
-                    int s = -1;
-                    for (int q = 0; q < (int)list.size(); ++q)
-                        if (list[q] == a) s = q;
-                    if (s < 0 && (int)list.size() < kScatterSlots) {
-                        list.push_back(a);
-                        s = (int)list.size() - 1;
-                    }
-                    slot[e] = s;
-                }
-            batoms.insert(batoms.end(), list.begin(), list.end());
-            boff[b + 1] = (int)batoms.size();
-        }
-        if (batoms.empty()) batoms.push_back(0);
-        p.nsblock = nb;
-        QM_TRY(upload(&p.d_block_atom_off, boff.data(), boff.size()));
-        QM_TRY(upload(&p.d_block_atoms, batoms.data(), batoms.size()));
-        QM_TRY(upload(&p.d_cov_slot, slot.data(), slot.size()));
-    }
+    QM_TRY(upload(&p.d_cov_grid, m.cov_grid.data(), m.cov_grid.size()));
+    QM_TRY(upload(&p.d_cov_off, m.cov_off.data(), m.cov_off.size()));
+    QM_TRY(upload(&p.d_cov_vox, m.cov_vox.data(), m.cov_vox.size()));
+    QM_TRY(upload(&p.d_atom_order, m.atom_order.data(), m.atom_order.size()));
+    QM_TRY(upload(&p.d_item_order, m.item_order.data(), m.item_order.size()));
+    QM_TRY(upload(&p.d_tabp_64, m.tabp64.data(), m.tabp64.size()));
+    QM_TRY(upload(&p.d_tabp_32, m.tabp32.data(), m.tabp32.size()));
+    QM_TRY(upload(&p.d_chunk_atom, m.chunk_atom.data(), m.chunk_atom.size()));
+    QM_TRY(upload(&p.d_chunk_v0, m.chunk_v0.data(), m.chunk_v0.size()));
+    QM_TRY(upload(&p.d_atom_chunk_off, m.atom_chunk_off.data(), m.atom_chunk_off.size()));
+    QM_TRY(upload(&p.d_proj_atom, m.proj_atom.data(), m.proj_atom.size()));
+    QM_TRY(upload(&p.d_block_atom_off, m.block_atom_off.data(), m.block_atom_off.size()));
+    QM_TRY(upload(&p.d_block_atoms, m.block_atoms.data(), m.block_atoms.size()));
+    QM_TRY(upload(&p.d_cov_slot, m.cov_slot.data(), m.cov_slot.size()));
     return QM_OK;
 }
 

@@ -11,9 +11,12 @@
 //                     matrix_2x2real_to_complex.h:8-56)
 //   H c = l S c       the Cholesky route of lapack_ZHEGVD_emu (w_zhegvd.h:227-272;
 //                     lobpcg_z_multi_mpi.h:2254-2263, 2492-2511): real n <= 512 by
-//                     the own Cholesky, trtri + GEMMs and tridiagonal reduction
-//                     (eigen.hip) with rocSOLVER stedc / ormtr; complex and larger
-//                     real problems by rocSOLVER potrf / trsm / syevd or heevd
+//                     the own cooperative Cholesky, trtri + GEMMs; real and complex
+//                     by the own cooperative tridiagonal reduction (trd_coop.hip),
+//                     rocSOLVER stedc and the own back-transformation of the
+//                     wanted eigenvectors; complex Cholesky and the problems the
+//                     cooperative reduction does not fit by rocSOLVER potrf /
+//                     trsm / syevd or heevd
 //   recurrence        S' = Cb^H S Cb, H' = Cb^H H Cb with Cb = [C | C_P] (rocBLAS;
 //                     NextMatrix_z_blasmp, 1104-1436); P' normalised by its norm
 //   update            [X' P'] = [X R P] U, [AX' AP'] = [AX AR AP] U (MFMA, T)

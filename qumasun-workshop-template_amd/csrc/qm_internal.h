@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/qumasun_hip.h"
+#include "projector_maps.h"
 
 namespace qm {
 
@@ -102,9 +103,6 @@ struct Projectors {
     int *d_block_atom_off = nullptr, *d_block_atoms = nullptr, *d_cov_slot = nullptr;
 };
 
-constexpr int kProjChunk = 512;   // voxels per projection work item (4 waves x 128)
-constexpr int kScatterBlock = 256;  // covered points per scatter workgroup
-constexpr int kScatterSlots = 8;    // atoms whose coefficients a scatter workgroup stages in LDS
 
 struct FftState;  // rocFFT plans and buffers of the spectral kinetic term (kinetic_fft.hip)
 
