@@ -52,6 +52,19 @@ def _time_applies(torch, fn, n):
     return e0.elapsed_time(e1) / n
 
 
+def copy_rate(torch, nbytes, n):
+    """GB/s of a plain device copy (torch copy_) moving nbytes in all: nbytes / 2
+    read and nbytes / 2 written, timed like the applies.  The bandwidth this
+    chip delivers to a streaming kernel, beside the published peak."""
+    a = torch.empty(nbytes // 8, dtype=torch.float32, device="cuda:0").fill_(1.0)
+    b = torch.empty_like(a)
+    for _ in range(2):
+        b.copy_(a)
+    ms = _time_applies(torch, lambda: b.copy_(a), n)
+    del a, b
+    return 8 * (nbytes // 8) / (ms * 1e-3) / 1e9
+
+
 def local_bytes(s, nb, itemsize):
     """Algorithmic HBM bytes of one fused stencil launch: read psi, write Hpsi, read V."""
     return nb * s.ng * s.ncomp * 2 * itemsize + s.ng * itemsize
@@ -399,6 +412,9 @@ def main():
     kernel_name = H.stencil_candidates()[cand] if cand >= 0 else "untuned"
     bytes_local = local_bytes(s, nb, 4)
     achieved = bytes_local / (ms_local * 1e-3) / 1e9
+    # plain copies: the stencil's own byte count (same cache residency) and 4 GiB (HBM only)
+    copy_same = copy_rate(torch, bytes_local, max(10, args.steps))
+    copy_big = copy_rate(torch, 4 << 30, 10)
     bytes64 = local_bytes(s, nb, 8)
     nl_bytes = nonlocal_bytes(s, nb, 4)
     out = {
@@ -421,7 +437,9 @@ def main():
         "roofline": {"bound": "hbm", "kernel": kernel_name,
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(kernel_name, s.name.split()[0]) if args.config == 1 else None,
-                     "bytes_per_launch": bytes_local, "ms_per_launch": ms_local},
+                     "bytes_per_launch": bytes_local, "ms_per_launch": ms_local,
+                     "copy_gbs_same_bytes": copy_same, "copy_gbs_4gib": copy_big,
+                     "frac_of_copy_same_bytes": achieved / copy_same},
         "parts_ms": {"local_fd8_v": ms_local, "nonlocal": ms_nl, "full": ms},
         "nonlocal_projection_tiles": H.nonlocal_choice(32, nb),
         "stencil_shape_in_full_apply": (H.stencil_candidates()[H.stencil_choice(32, nb, full=True)]
