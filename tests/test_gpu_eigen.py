@@ -1,5 +1,6 @@
-"""GPU parity of the FP64 generalized symmetric eigensolver qm_sygvd_f64 (own
-tridiagonal reduction + rocSOLVER stedc/ormtr, Cholesky route of
+"""GPU parity of the FP64 generalized symmetric eigensolvers qm_sygvd_f64 and
+qm_hegvd_z64 (own Cholesky and tridiagonal reduction, rocSOLVER stedc, own
+back-transformation of the wanted eigenvectors; the Cholesky route of
 lapack_DSYGVD / lapack_ZHEGVD_emu) against LAPACK through scipy.
 
 Tolerances (FP64): eigenvalues within 1e-11 * ||A||; residual
@@ -70,6 +71,35 @@ def test_sygvd_against_lapack(H, torch_cuda, route, n, clusters):
     assert np.abs(Z.T @ B @ Z - np.eye(nev)).max() <= 1e-11 * n
     Uu = np.triu(U)
     assert np.abs(Uu.T @ Uu - B).max() <= 1e-12 * n
+
+
+# the back-transformation runs one workgroup per wanted eigenvector: one
+# column (a single workgroup) and all n columns, on the cooperative route
+@pytest.mark.parametrize("route", [-1], indirect=True, ids=["coop"])
+@pytest.mark.parametrize("n", [3, 129, 384, 1040])
+@pytest.mark.parametrize("which", ["one", "all"])
+def test_sygvd_hegvd_nev_extremes(H, torch_cuda, route, n, which):
+    torch = torch_cuda
+    nev = 1 if which == "one" else n
+    print(f"sygvd/hegvd n {n} nev {nev}", flush=True)
+    A, B = _problem(n, seed=3 * n + nev)
+    w, Z, _ = H.sygvd(torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda(), nev)
+    torch.cuda.synchronize()
+    w, Z = w.cpu().numpy(), Z.cpu().numpy()
+    w_ref = scipy.linalg.eigh(A, B, eigvals_only=True)
+    scale = max(1.0, np.abs(A).max() * n)
+    assert np.abs(w - w_ref).max() <= 1e-11 * scale
+    assert np.abs(A @ Z - (B @ Z) * w[:nev]).max() <= 1e-11 * scale
+    assert np.abs(Z.T @ B @ Z - np.eye(nev)).max() <= 1e-11 * n
+    Ac, Bc = _zproblem(n, seed=5 * n + nev)
+    w, Z, _ = H.hegvd(torch.from_numpy(Ac).cuda(), torch.from_numpy(Bc).cuda(), nev)
+    torch.cuda.synchronize()
+    w, Z = w.cpu().numpy(), Z.cpu().numpy()
+    w_ref = scipy.linalg.eigh(Ac, Bc, eigvals_only=True)
+    scale = max(1.0, np.abs(Ac).max() * n)
+    assert np.abs(w - w_ref).max() <= 1e-11 * scale
+    assert np.abs(Ac @ Z - (Bc @ Z) * w[:nev]).max() <= 1e-11 * scale
+    assert np.abs(Z.conj().T @ Bc @ Z - np.eye(nev)).max() <= 1e-11 * n
 
 
 def test_sygvd_not_positive_definite(qm, H, torch_cuda):
