@@ -87,7 +87,11 @@ __device__ inline T rank2(T a, T vr, T wc, T wr, T vc) {
 // Every wave waits for its own stores to complete (they are then in this
 // XCD's L2); thread 0's agent-scope release writes the L2 back and its
 // acquire invalidates this CU's caches, so after the closing workgroup
-// barrier every wave reads the other workgroups' data from memory.
+// barrier every wave reads the other workgroups' data from memory.  The wait
+// for the generation polls with relaxed loads and takes the acquire once,
+// through the fence after the loop (the fence synchronizes with the release
+// store whose value the last poll read): an acquire load would invalidate
+// the caches at every poll.
 __device__ void coop_barrier(unsigned *count, unsigned *gen, unsigned nblk, unsigned &lg, int *err) {
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
@@ -99,7 +103,7 @@ __device__ void coop_barrier(unsigned *count, unsigned *gen, unsigned nblk, unsi
             __hip_atomic_store(gen, g + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         } else {
             long spins = 0;
-            while (__hip_atomic_load(gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
+            while (__hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
                 if (++spins > kCoopSpinLimit) {
                     __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     break;
