@@ -23,14 +23,15 @@
 // column that overwrites it.  Every workgroup computes the reflector from the
 // same data with the same operation order, so all hold the same v and tau.
 //
-// Output follows LAPACK's convention so rocSOLVER's stedc and ormtr / unmtr
-// consume it: d, e (real), tau, and v (v_{j+1} = 1 implied) below the
+// Output follows LAPACK's convention so rocSOLVER's stedc and the
+// back-transformation below (ormtr / unmtr's operation) consume it: d, e
+// (real), tau, and v (v_{j+1} = 1 implied) below the
 // subdiagonal of column j of A.  The input is read from A's upper triangle
 // (uplo 'U' defines the matrix; the diagonal's imaginary part is dropped).
 //
-// The barrier is a counter and a generation word in device memory, with
-// agent-scope release / acquire ordering on the arrival and the wait; its
-// waits are bounded: past
+// The barrier is an arrival counter in device memory that only grows within
+// a launch, with agent-scope release / acquire ordering on the arrival and
+// the wait; its waits are bounded: past
 // kCoopSpinLimit polls a workgroup sets the error word and goes on, so the
 // kernel always ends (the caller turns the error word into a failed solve).
 // hipLaunchCooperativeKernel guarantees that all G workgroups are resident.
@@ -85,39 +86,37 @@ __device__ inline T rank2(T a, T vr, T wc, T wr, T vc) {
 }
 
 // Every wave waits for its own stores to complete (they are then in this
-// XCD's L2); thread 0's agent-scope release writes the L2 back and its
-// acquire invalidates this CU's caches, so after the closing workgroup
-// barrier every wave reads the other workgroups' data from memory.  The wait
-// for the generation polls with relaxed loads and takes the acquire once,
-// through the fence after the loop (the fence synchronizes with the release
-// store whose value the last poll read): an acquire load would invalidate
-// the caches at every poll.
-__device__ void coop_barrier(unsigned *count, unsigned *gen, unsigned nblk, unsigned &lg, int *err) {
+// XCD's L2); thread 0's agent-scope release increment writes the L2 back.
+// Barrier b (b = 0, 1, ..) is passed when the counter reads (b + 1) nblk:
+// each waiter polls the counter itself, so no second store (a generation
+// word written by the last arrival) sits between the last arrival and the
+// waiters.  The polls are relaxed loads; the fence after the loop takes the
+// acquire once (every write to the counter is a release increment, so the
+// fence synchronizes with all of them) and invalidates this CU's caches, so
+// after the closing workgroup barrier every wave reads the other
+// workgroups' data from memory.  An acquire load would invalidate the
+// caches at every poll.
+__device__ void coop_barrier(unsigned *count, unsigned nblk, unsigned &lg, int *err) {
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (threadIdx.x == 0) {
-        const unsigned g = lg;
-        const unsigned arrived = __hip_atomic_fetch_add(count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (arrived == nblk - 1) {
-            __hip_atomic_store(count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(gen, g + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            long spins = 0;
-            while (__hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
-                if (++spins > kCoopSpinLimit) {
-                    __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
+        const unsigned target = (lg + 1) * nblk;
+        __hip_atomic_fetch_add(count, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        long spins = 0;
+        while (__hip_atomic_load(count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (++spins > kCoopSpinLimit) {
+                __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
             }
+            __builtin_amdgcn_s_sleep(1);
         }
-        lg = g + 1;
+        lg = lg + 1;
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
 }
 
-// sync[0]: arrival count, sync[32]: generation (separate 128-byte lines),
+// sync[0]: arrival count (grows by G per barrier; at most n G < 2^32),
 // sync[64]: error word.  All zero at launch.
 template <typename T, int PER>
 __global__ __launch_bounds__(kCoopThreads) void trd_coop_kernel(T *__restrict__ A, int lda, int n, int nr,
@@ -133,7 +132,7 @@ __global__ __launch_bounds__(kCoopThreads) void trd_coop_kernel(T *__restrict__ 
     T *v = xr + n;                          // reflector, global row index
     T *w = v + n;
     const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    unsigned *count = sync, *gen = sync + 32;
+    unsigned *count = sync;
     int *err = reinterpret_cast<int *>(sync + 64);
     unsigned lg = 0;
     const int nr_own = g < n ? min(nr, (n - g + G - 1) / G) : 0;  // own rows r = g + q G < n
@@ -154,7 +153,7 @@ __global__ __launch_bounds__(kCoopThreads) void trd_coop_kernel(T *__restrict__ 
             const T *Lq = L + (size_t)(r / G) * n;
             for (int c = tid; c < n; c += kCoopThreads) rowbuf[(size_t)r * n + c] = Lq[c];
         }
-    coop_barrier(count, gen, G, lg, err);
+    coop_barrier(count, G, lg, err);
     for (int c = tid; c < n; c += kCoopThreads) xr[c] = rowbuf[c];
     __syncthreads();
 
@@ -228,7 +227,7 @@ __global__ __launch_bounds__(kCoopThreads) void trd_coop_kernel(T *__restrict__ 
             for (int i = 0; i < kCoopWaves; ++i) t = add(t, wsum[i]);
             dots[(size_t)par * G + g] = t;
         }
-        coop_barrier(count, gen, G, lg, err);
+        coop_barrier(count, G, lg, err);
         // ---- one round of loads: the partial dots, p and the old row j+1
         const T *old = rowbuf + (size_t)((j + 1) & 1) * n;
         T pr[PER], orow[PER];
@@ -412,7 +411,7 @@ __global__ __launch_bounds__(kPcThreads) void potrf_coop_kernel(double *__restri
     __shared__ double piv[kPcNB];                // the panel's pivots
     __shared__ double Ux[kPcNB][kPcNB + 1];      // panel k's rows over this panel's columns
     const int g = blockIdx.x, tid = threadIdx.x;
-    unsigned *count = sync, *gen = sync + 32;
+    unsigned *count = sync;
     int *err = reinterpret_cast<int *>(sync + 64);
     int *info = reinterpret_cast<int *>(sync + 96);
     unsigned lg = 0;
@@ -463,7 +462,7 @@ __global__ __launch_bounds__(kPcThreads) void potrf_coop_kernel(double *__restri
                 }
             }
         }
-        coop_barrier(count, gen, P, lg, err);
+        coop_barrier(count, P, lg, err);
         if (__hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;  // all workgroups
         if (g <= k) continue;
         // own rows -= U(panel k rows, own columns)^T U(panel k rows, column c)
