@@ -378,6 +378,19 @@ int qm_hegvd_z64(qm_ctx *ctx, int n, const void *d_A, const void *d_B, double *d
  * (different summation orders). */
 int qm_subspace_force(int route);
 
+/* Bound of the grid-barrier waits in the cooperative kernels (the tridiagonal
+ * reduction and the Cholesky), in polls, process-wide; no reference
+ * counterpart (a diagnostic and test control).  A workgroup that waits past
+ * it sets the launch's error word and stops waiting, so does every other
+ * waiter within 256 polls, the kernel ends, and the solve reports it
+ * (info -77: qm_sygvd_f64 / qm_hegvd_z64 fail with QM_ERR_NUMERIC,
+ * qm_lobpcg_solve takes its fallback route).  The default,
+ * qm_coop_spin_limit_default() (2^24, ~0.5 s), is far above any barrier's
+ * wait; 0 makes every wait that is not over at its first poll fail, which
+ * the tests use to exercise that path.  QM_ERR_ARG for polls < 0. */
+int qm_coop_set_spin_limit(long polls);
+long qm_coop_spin_limit_default(void);
+
 /* HIP graphs for the iteration segments that hold no host decision (no
  * reference counterpart: the reference launches every kernel from the host).
  * Within one qm_lobpcg_solve call, A.R and the Gram products of an iteration

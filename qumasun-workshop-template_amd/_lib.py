@@ -25,7 +25,8 @@ EXPORTS = (
     "qm_td_taylor_f64", "qm_td_taylor_f32", "qm_td_expv_f64", "qm_td_expv_f32",
     "qm_set_kinetic", "qm_td_kinetic_fft_f64", "qm_td_kinetic_fft_f32",
     "qm_nonlocal_force", "qm_nonlocal_choice",
-    "qm_subspace_force", "qm_hegvd_z64", "qm_lobpcg_eigen_retries",
+    "qm_subspace_force", "qm_coop_set_spin_limit", "qm_coop_spin_limit_default", "qm_hegvd_z64",
+    "qm_lobpcg_eigen_retries",
     "qm_hartree_f64", "qm_xc_lda_f64", "qm_grid_dot_f64", "qm_mixer_create", "qm_mixer_destroy", "qm_mixer_init",
     "qm_mixer_predict", "qm_occupancy",
     "qm_upf_load", "qm_upf_free", "qm_upf_info", "qm_upf_radial", "qm_upf_eval", "qm_upf_tables",
@@ -98,6 +99,9 @@ def load():
     L.qm_sygvd_f64.argtypes = [P, I, P, P, P, P, P, I, P]
     L.qm_release_captures.argtypes = [P]
     L.qm_subspace_force.argtypes = [I]
+    L.qm_coop_set_spin_limit.argtypes = [ctypes.c_long]
+    L.qm_coop_spin_limit_default.argtypes = []
+    L.qm_coop_spin_limit_default.restype = ctypes.c_long
     L.qm_hegvd_z64.argtypes = [P, I, P, P, P, P, P, I, P]
     L.qm_lobpcg_eigen_retries.argtypes = [P, ctypes.c_char_p, I]
     L.qm_hartree_f64.argtypes = [P, P, P, P]

@@ -611,6 +611,7 @@ int qm_sygvd_f64(qm_ctx *ctx, int n, const double *d_A, const double *d_B, doubl
     int h_info = 0;
     QM_HIP(hipMemcpyAsync(&h_info, info, sizeof(int), hipMemcpyDeviceToHost, st));
     QM_HIP(hipStreamSynchronize(st));
+    QM_CHECK(h_info != -77, QM_ERR_NUMERIC, "qm_sygvd_f64: the cooperative Cholesky's grid barrier timed out (info -77)");
     QM_CHECK(h_info == 0, QM_ERR_NUMERIC,
              "qm_sygvd_f64: B is not positive definite (leading minor " + std::to_string(h_info) + ")");
     QM_TRY(chol_transform(h, n, d_U, C, Ui, T, st));
@@ -619,7 +620,8 @@ int qm_sygvd_f64(qm_ctx *ctx, int n, const double *d_A, const double *d_B, doubl
     QM_HIP(hipMemcpyAsync(&h_info, info, sizeof(int), hipMemcpyDeviceToHost, st));
     QM_HIP(hipStreamSynchronize(st));
     QM_CHECK(h_info == 0, QM_ERR_NUMERIC,
-             "qm_sygvd_f64: tridiagonal eigensolve failed (rocSOLVER stedc info " + std::to_string(h_info) + ")");
+             "qm_sygvd_f64: tridiagonal eigensolve failed (info " + std::to_string(h_info) +
+                 ": rocSOLVER stedc, or -77 where the cooperative reduction's grid barrier timed out)");
     return QM_OK;
 }
 

@@ -35,6 +35,8 @@
 // kCoopSpinLimit polls a workgroup sets the error word and goes on, so the
 // kernel always ends (the caller turns the error word into a failed solve).
 // hipLaunchCooperativeKernel guarantees that all G workgroups are resident.
+#include <atomic>
+
 #include "qm_internal.h"
 
 namespace qm {
@@ -44,6 +46,8 @@ constexpr int kCoopThreads = 256;
 constexpr int kCoopWaves = kCoopThreads / 64;
 constexpr int kCoopLdsBytes = 150 * 1024;
 constexpr long kCoopSpinLimit = 1L << 24;  // ~0.5 s of polling
+// the bound the launches use (qm_coop_set_spin_limit), process-wide
+std::atomic<long> g_coop_spin_limit{kCoopSpinLimit};
 constexpr int kCoopMaxPer = 8;              // vector elements per thread: n <= 2048
 constexpr int kCoopMaxGroups = 256;
 constexpr int kCoopRowsPerWave = 4;         // own rows per wave: nr <= 16
@@ -96,7 +100,7 @@ __device__ inline T rank2(T a, T vr, T wc, T wr, T vc) {
 // after the closing workgroup barrier every wave reads the other
 // workgroups' data from memory.  An acquire load would invalidate the
 // caches at every poll.
-__device__ void coop_barrier(unsigned *count, unsigned nblk, unsigned &lg, int *err) {
+__device__ void coop_barrier(unsigned *count, unsigned nblk, unsigned &lg, int *err, long limit) {
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -104,10 +108,14 @@ __device__ void coop_barrier(unsigned *count, unsigned nblk, unsigned &lg, int *
         __hip_atomic_fetch_add(count, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         long spins = 0;
         while (__hip_atomic_load(count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-            if (++spins > kCoopSpinLimit) {
+            if (++spins > limit) {
                 __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 break;
             }
+            // once any workgroup has timed out the launch has failed: stop
+            // waiting (every 256 polls), so a failed launch ends after about
+            // one limit in all, not one per barrier
+            if ((spins & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
             __builtin_amdgcn_s_sleep(1);
         }
         lg = lg + 1;
@@ -123,7 +131,7 @@ __global__ __launch_bounds__(kCoopThreads) void trd_coop_kernel(T *__restrict__ 
                                                                 double *__restrict__ dout,
                                                                 double *__restrict__ eout, T *__restrict__ tauout,
                                                                 T *__restrict__ rowbuf, T *__restrict__ pbuf,
-                                                                T *__restrict__ dots, unsigned *sync) {
+                                                                T *__restrict__ dots, unsigned *sync, long limit) {
     __shared__ __align__(16) unsigned char lds_raw[kCoopLdsBytes];
     __shared__ T wsum[kCoopWaves];
     __shared__ T dsh[kCoopMaxGroups];
@@ -153,7 +161,7 @@ __global__ __launch_bounds__(kCoopThreads) void trd_coop_kernel(T *__restrict__ 
             const T *Lq = L + (size_t)(r / G) * n;
             for (int c = tid; c < n; c += kCoopThreads) rowbuf[(size_t)r * n + c] = Lq[c];
         }
-    coop_barrier(count, G, lg, err);
+    coop_barrier(count, G, lg, err, limit);
     for (int c = tid; c < n; c += kCoopThreads) xr[c] = rowbuf[c];
     __syncthreads();
 
@@ -227,7 +235,7 @@ __global__ __launch_bounds__(kCoopThreads) void trd_coop_kernel(T *__restrict__ 
             for (int i = 0; i < kCoopWaves; ++i) t = add(t, wsum[i]);
             dots[(size_t)par * G + g] = t;
         }
-        coop_barrier(count, G, lg, err);
+        coop_barrier(count, G, lg, err, limit);
         // ---- one round of loads: the partial dots, p and the old row j+1
         const T *old = rowbuf + (size_t)((j + 1) & 1) * n;
         T pr[PER], orow[PER];
@@ -315,6 +323,15 @@ __global__ void coop_clear_kernel(unsigned *sync) {
     for (int i = threadIdx.x; i < 128; i += blockDim.x) sync[i] = 0u;
 }
 
+// d, e := 0 (n values each) where the reduction's error word is set
+__global__ void coop_guard_kernel(const unsigned *sync, double *d, double *e, int n) {
+    if (sync[64] == 0u) return;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        d[i] = 0.0;
+        e[i] = 0.0;
+    }
+}
+
 // info := -77 where the reduction's error word is set (and info is 0)
 __global__ void coop_err_kernel(const unsigned *sync, int *info) {
     if (threadIdx.x == 0 && sync[64] != 0u && *info == 0) *info = -77;
@@ -363,7 +380,8 @@ int trd_coop_impl(int n, T *A, int lda, double *d, double *e, T *tau, void *work
              "tridiagonal reduction: work buffer too small");
     hipLaunchKernelGGL(coop_clear_kernel, dim3(1), dim3(128), 0, st, sync);
     QM_HIP(hipGetLastError());
-    void *args[] = {&A, &lda, &n, (void *)&nr, &d, &e, &tau, &rowbuf, &pbuf, &dots, &sync};
+    long limit = g_coop_spin_limit.load();
+    void *args[] = {&A, &lda, &n, (void *)&nr, &d, &e, &tau, &rowbuf, &pbuf, &dots, &sync, &limit};
     const int per = (n + kCoopThreads - 1) / kCoopThreads;  // columns per thread
     const void *fn = per <= 2   ? reinterpret_cast<const void *>(&trd_coop_kernel<T, 2>)
                      : per <= 4 ? reinterpret_cast<const void *>(&trd_coop_kernel<T, 4>)
@@ -373,6 +391,11 @@ int trd_coop_impl(int n, T *A, int lda, double *d, double *e, T *tau, void *work
         (void)hipGetLastError();
         return QM_ERR_STATE;  // the caller takes the single-workgroup route
     }
+    // a launch whose barrier timed out leaves d and e undefined: they are
+    // zeroed (a valid tridiagonal) before any eigensolver reads them, and
+    // the caller's check of the error word rejects the solve
+    hipLaunchKernelGGL(coop_guard_kernel, dim3(1), dim3(256), 0, st, (const unsigned *)sync, d, e, n);
+    QM_HIP(hipGetLastError());
     if (info) {
         hipLaunchKernelGGL(coop_err_kernel, dim3(1), dim3(64), 0, st, sync, info);
         QM_HIP(hipGetLastError());
@@ -406,7 +429,7 @@ constexpr int kPcMaxN = 512;
 constexpr int kPcThreads = kPcMaxN;  // one thread per column
 
 __global__ __launch_bounds__(kPcThreads) void potrf_coop_kernel(double *__restrict__ A, int lda, int n,
-                                                                unsigned *sync) {
+                                                                unsigned *sync, long limit) {
     __shared__ double rowj[2][kPcMaxN];          // the row being eliminated (by parity of j)
     __shared__ double piv[kPcNB];                // the panel's pivots
     __shared__ double Ux[kPcNB][kPcNB + 1];      // panel k's rows over this panel's columns
@@ -462,7 +485,7 @@ __global__ __launch_bounds__(kPcThreads) void potrf_coop_kernel(double *__restri
                 }
             }
         }
-        coop_barrier(count, P, lg, err);
+        coop_barrier(count, P, lg, err, limit);
         if (__hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;  // all workgroups
         if (g <= k) continue;
         // own rows -= U(panel k rows, own columns)^T U(panel k rows, column c)
@@ -783,7 +806,8 @@ int potrf_coop_f64(int n, double *A, int lda, int *info, void *sync, hipStream_t
     unsigned *sy = static_cast<unsigned *>(sync);
     hipLaunchKernelGGL(coop_clear_kernel, dim3(1), dim3(128), 0, st, sy);
     QM_HIP(hipGetLastError());
-    void *args[] = {&A, &lda, &n, &sy};
+    long limit = g_coop_spin_limit.load();
+    void *args[] = {&A, &lda, &n, &sy, &limit};
     const hipError_t le = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&potrf_coop_kernel), dim3(P),
                                                      dim3(kPcThreads), args, 0, st);
     if (le != hipSuccess) {
@@ -796,3 +820,11 @@ int potrf_coop_f64(int n, double *A, int lda, int *info, void *sync, hipStream_t
 }
 
 }  // namespace qm
+
+extern "C" int qm_coop_set_spin_limit(long polls) {
+    QM_CHECK(polls >= 0, QM_ERR_ARG, "qm_coop_set_spin_limit: polls must be >= 0 (a negative value resets)");
+    qm::g_coop_spin_limit.store(polls);
+    return QM_OK;
+}
+
+extern "C" long qm_coop_spin_limit_default(void) { return qm::kCoopSpinLimit; }

@@ -32,6 +32,13 @@ def subspace_force(route):
     _lib.check(_lib.load().qm_subspace_force(int(route)))
 
 
+def coop_spin_limit(polls=None):
+    """Bound of the cooperative kernels' grid-barrier waits in polls,
+    process-wide (qm_coop_set_spin_limit); None restores the default."""
+    L = _lib.load()
+    _lib.check(L.qm_coop_set_spin_limit(int(L.qm_coop_spin_limit_default() if polls is None else polls)))
+
+
 class Hamiltonian:
     def __init__(self, system, device=0):
         self.lib = _lib.load()

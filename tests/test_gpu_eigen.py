@@ -102,6 +102,42 @@ def test_sygvd_hegvd_nev_extremes(H, torch_cuda, route, n, which):
     assert np.abs(Z.conj().T @ Bc @ Z - np.eye(nev)).max() <= 1e-11 * n
 
 
+# A grid-barrier wait that runs out ends the cooperative kernel and fails the
+# solve with info -77 (the bound forced to 0 polls from the host, so every
+# wait not over at its first poll runs out; every kernel still ends).  Route
+# -1: the cooperative Cholesky fails first; route 1: the single-workgroup
+# Cholesky, then the cooperative reduction fails, and its undefined
+# tridiagonal is zeroed before stedc reads it.  The default bound then solves
+# the same problems to LAPACK parity.
+@pytest.mark.parametrize("route", [-1, 1], indirect=True, ids=["coop", "coop-reduction"])
+def test_coop_barrier_timeout_is_reported(qm, H, torch_cuda, route):
+    torch = torch_cuda
+    n, nev = 384, 128
+    A, B = _problem(n, seed=11)
+    Ac, Bc = _zproblem(n, seed=12)
+    print(f"barrier timeout route {route}", flush=True)
+    try:
+        qm.hamiltonian.coop_spin_limit(0)
+        with pytest.raises(RuntimeError, match="-77"):
+            H.sygvd(torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda(), nev)
+        with pytest.raises(RuntimeError, match="-77"):
+            H.hegvd(torch.from_numpy(Ac).cuda(), torch.from_numpy(Bc).cuda(), nev)
+    finally:
+        qm.hamiltonian.coop_spin_limit(None)
+    w, Z, _ = H.sygvd(torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda(), nev)
+    torch.cuda.synchronize()
+    w, Z = w.cpu().numpy(), Z.cpu().numpy()
+    scale = max(1.0, np.abs(A).max() * n)
+    assert np.abs(w - scipy.linalg.eigh(A, B, eigvals_only=True)).max() <= 1e-11 * scale
+    assert np.abs(A @ Z - (B @ Z) * w[:nev]).max() <= 1e-11 * scale
+    w, Z, _ = H.hegvd(torch.from_numpy(Ac).cuda(), torch.from_numpy(Bc).cuda(), nev)
+    torch.cuda.synchronize()
+    w, Z = w.cpu().numpy(), Z.cpu().numpy()
+    scale = max(1.0, np.abs(Ac).max() * n)
+    assert np.abs(w - scipy.linalg.eigh(Ac, Bc, eigvals_only=True)).max() <= 1e-11 * scale
+    assert np.abs(Ac @ Z - (Bc @ Z) * w[:nev]).max() <= 1e-11 * scale
+
+
 def test_sygvd_not_positive_definite(qm, H, torch_cuda):
     torch = torch_cuda
     n = 40
